@@ -1,0 +1,215 @@
+/*
+ * ntlp_gpu.h -- C ABI of the MI355X droplet step (libntlp_gpu.so).
+ *
+ * Drop-in for the Lagrangian droplet path of NTLP (particles.f90).  The
+ * Fortran host binds these with ISO_C_BINDING (fortran/ntlp_gpu_mod.f90); the
+ * Python test and bench harness binds them with ctypes (ntlp_amd/__init__.py).
+ *
+ * Conventions
+ *  - Every function returns NTLP_OK (0) or a negative NTLP_E_* code; no
+ *    exception or abort crosses the ABI.  ntlp_gpu_last_error() describes the
+ *    last failure.
+ *  - Arrays use Fortran column-major index math and the reference's own index
+ *    ranges (documented per argument).  Host pointers are borrowed for the
+ *    duration of the call.
+ *  - One process-wide context on the current HIP device, like the module
+ *    globals of particles.f90.  Calls from several host threads must be
+ *    serialised by the caller.
+ *  - Droplets keep the order in which they were put: ntlp_gpu_particles_get
+ *    returns them in that order, whatever order the device keeps internally.
+ */
+#ifndef NTLP_GPU_H
+#define NTLP_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NTLP_OK 0
+#define NTLP_E_INVALID (-1)     /* bad argument or inconsistent sizes       */
+#define NTLP_E_NOT_INIT (-2)    /* ntlp_gpu_init has not been called         */
+#define NTLP_E_HIP (-3)         /* a HIP runtime call failed                 */
+#define NTLP_E_NOMEM (-4)       /* device allocation failed                  */
+#define NTLP_E_RANGE (-5)       /* size beyond a documented maximum          */
+#define NTLP_E_UNSUPPORTED (-6) /* flag combination this build does not run  */
+#define NTLP_E_STATE (-7)       /* call out of order (no fields, no droplets) */
+
+/* Column order of the droplet real SoA block: column c of droplet i lives at
+ * soa[c*n + i] (a Fortran array real(8) :: soa(n, NTLP_NREAL)).  These are
+ * the FP64 members of `type particle` (particles.f90:76-85) except the
+ * stochastic-subgrid members dist, u_sub and sigm_s. */
+enum ntlp_column {
+  NTLP_X = 0, NTLP_Y, NTLP_Z,
+  NTLP_VX, NTLP_VY, NTLP_VZ,
+  NTLP_UFX, NTLP_UFY, NTLP_UFZ,
+  NTLP_XRHSX, NTLP_XRHSY, NTLP_XRHSZ,
+  NTLP_VRHSX, NTLP_VRHSY, NTLP_VRHSZ,
+  NTLP_TP, NTLP_TPRHS_S, NTLP_TPRHS_L, NTLP_TF, NTLP_RADIUS, NTLP_RADRHS, NTLP_QINF, NTLP_QSTAR,
+  NTLP_RES, NTLP_M_S, NTLP_KAPPA_S, NTLP_RC, NTLP_ACTRES, NTLP_NUMACT,
+  NTLP_VXOLD, NTLP_VYOLD, NTLP_VZOLD, NTLP_TPOLD, NTLP_RADIUSOLD,
+  NTLP_NREAL
+};
+
+/* Grid metadata: nnx, nny, nnz (pars, defs.F:51-53); dx=xl/nnx, dy=yl/nny
+ * (les.F:5488-5489); z, zz are z(0:nnz+1), zz(0:nnz+1) and dzw, dzu are
+ * dzw(0:nnz+2), dzu(0:nnz+2) as get_dz builds them (les.F:5836-5874). */
+typedef struct ntlp_grid {
+  int32_t nnx, nny, nnz, reserved;
+  double dx, dy, xl, yl;
+  const double *z, *zz, *dzw, *dzu;
+} ntlp_grid;
+
+/* Physical constants (namelist &constants, defs.F:43-46, particles.f90:42)
+ * and the low-storage RK3 coefficients zetas/gama (les.F:856-861). */
+typedef struct ntlp_consts {
+  double dt;
+  double nuf, Cpa, Cpp, CpaCpp, Pra, Sc, Rd, rhow, rhos, Mw, Ru, Gam, Lv, Cpv;
+  double surf_rho, surf_p, tsfcc1, grav, radius_init, pi2;
+  double part_grav[3];
+  double zetas[3], gama[3];
+} ntlp_consts;
+
+/* Hot-path flags of namelist &flags (defs.F:8-14).  ilin=1 trilinear,
+ * otherwise 6th-order Lagrange; icase must be 0 (reflecting walls). */
+typedef struct ntlp_flags {
+  int32_t ilin, icouple, iTcouple, iHcouple, ievap, iexner, icase, reserved;
+} ntlp_flags;
+
+/* Newton bookkeeping of particle_update_BE (particles.f90:27-30, 3035-3039)
+ * plus iteration totals.  gn_hist[k] counts solves whose Gauss-Newton loop
+ * ran k iterations (k = 1..50); gn_hist[0] counts solves that fell back to
+ * Levenberg-Marquardt. */
+#define NTLP_GN_HIST 64
+typedef struct ntlp_counters {
+  int64_t num100, numimpos, actnum, denum;
+  int64_t gn_iters, lm_iters, solves, steps;
+  int64_t gn_hist[NTLP_GN_HIST];
+} ntlp_counters;
+
+/* Field layouts accepted by ntlp_gpu_set_fields. */
+#define NTLP_LAYOUT_EXT 0  /* uext(0:nnz+1, -1:nny+3, -1:nnx+3): fill_ext's output */
+#define NTLP_LAYOUT_CORE 1 /* (0:nnz+1, 1:nny, 1:nnx) z fastest; the device fills
+                              the periodic x-y halo (fill_ext on one domain)     */
+#define NTLP_LAYOUT_XPENCIL 2 /* u(1:nnx+2, 1:nny, 0:nnz+1) x fastest, the flow
+                                 solver's own arrays (les.F:8394-8397); the device
+                                 transposes to z fastest and fills the halo      */
+
+/* Step modes and precisions of ntlp_gpu_step. */
+#define NTLP_MODE_RK3 1        /* ipart_method=1: three particle_update_rk3 stages */
+#define NTLP_MODE_BE 2         /* ipart_method=2: particle_update_BE               */
+#define NTLP_PREC_FP64 0       /* FP64 throughout, reference Newton (FD Jacobian)   */
+#define NTLP_PREC_MIXED 1      /* FP32 gather weights, drag and explicit stages; FP64
+                                  positions, Newton solve (analytic Jacobian) and
+                                  coupling accumulation                            */
+#define NTLP_PREC_FP64_AJ 2    /* FP64 throughout with the analytic Jacobian         */
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the grid/constant set-up of particle_setup (particles.f90:1829-1986)
+ * on the current HIP device.  Copies grid metadata; may be called again to
+ * re-initialise (all device state is released first). */
+int ntlp_gpu_init(const ntlp_grid *grid, const ntlp_consts *consts, const ntlp_flags *flags);
+/* Releases every device buffer of the context. */
+int ntlp_gpu_finalize(void);
+/* Text of the last error (thread-local, never NULL). */
+const char *ntlp_gpu_last_error(void);
+/* Changes dt (the reference recomputes dt every step in get_dt, les.F:621-709). */
+int ntlp_gpu_set_dt(double dt);
+
+/* ----------------------------------------------------------------- fields */
+
+/* Replaces fill_ext (particles.f90:91-328): u, v, w, T (t(:,:,1,:)) and
+ * q (t(:,:,2,:)) as host FP64 arrays in `layout`.  Copied to the device and
+ * made resident; the ext halo is filled on the device. */
+int ntlp_gpu_set_fields(const double *u, const double *v, const double *w, const double *T,
+                        const double *q, int32_t layout);
+
+/* ---------------------------------------------------------------- droplets */
+
+/* Packs droplets in (the linked list of `type particle`, particles.f90:76-87).
+ * real_soa is real(8) :: soa(n, NTLP_NREAL); mult, pidx, procidx per droplet.
+ * Replaces any droplets already on the device. */
+int ntlp_gpu_particles_put(const double *real_soa, const int64_t *mult, const int32_t *pidx,
+                           const int32_t *procidx, int64_t n);
+/* Unpacks droplets in put order.  Any output pointer may be NULL. */
+int ntlp_gpu_particles_get(double *real_soa, int64_t *mult, int32_t *pidx, int32_t *procidx,
+                           int64_t n);
+/* Restart records (save_particles / read_part_res, particles.f90:1988-2146):
+ * 352-byte records of 4 int32 (pidx, procidx, nbr_pidx, nbr_procidx), 39 real(8)
+ * in `type particle` order, and 3 int64 (mult + two pointer slots). */
+#define NTLP_RECORD_BYTES 352
+int ntlp_gpu_particles_put_records(const void *records, int64_t n);
+int ntlp_gpu_particles_get_records(void *records, int64_t n);
+int64_t ntlp_gpu_particles_count(void);
+
+/* ------------------------------------------------- reference entry points */
+
+/* uf_interp_lin / uf_interp (particles.f90:554-847) for every droplet: fills
+ * uf, Tf and qinf.  set_vp / set_tp then copy uf to vp and Tf to Tp, the
+ * it<=1 initialisation of particle_update_rk3 (2907-2910). */
+int ntlp_gpu_interp(int32_t set_vp, int32_t set_tp);
+/* create_particle's critical radius for every droplet: rc = crit_radius(m_s,
+ * kappa_s, Tp) over setup_radval's 1024 radii (particles.f90:2264-2266,
+ * 5294-5327); rc = 0 when ievap /= 1. */
+int ntlp_gpu_crit_radius(void);
+/* particle_update_rk3(istage) (particles.f90:2845-2998) including its wall
+ * and periodic boundary conditions; `it` is the time-step counter. */
+int ntlp_gpu_particle_update_rk3(int32_t istage, int32_t it);
+/* particle_update_BE (particles.f90:3000-3280) including boundary conditions. */
+int ntlp_gpu_particle_update_be(int32_t it);
+/* The same two entry points with an explicit NTLP_PREC_* arithmetic. */
+int ntlp_gpu_particle_update_rk3_prec(int32_t istage, int32_t it, int32_t precision);
+int ntlp_gpu_particle_update_be_prec(int32_t it, int32_t precision);
+/* particle_coupling_update (particles.f90:1151-1276).  Sources stay resident;
+ * when src_out is not NULL it receives partsrc_t(0:nnz+1,1:nny+1,1:nnx+1,1:3),
+ * partTsrc_t, partHsrc_t, partTEsrc_t as 6 consecutive arrays.  deterministic=1
+ * uses the fixed-order node-centric sum instead of segmented atomics. */
+int ntlp_gpu_particle_coupling_update(double *src_out, int32_t deterministic);
+
+/* Copies the resident coupling sources (same layout as src_out above) as the
+ * last coupling update or fused step left them. */
+int ntlp_gpu_get_sources(double *src_out);
+
+/* ------------------------------------------------------------- binning */
+
+/* Cell indices of every droplet (put order) as int32 [n][4] = (ipt, jpt, kpt,
+ * kwpt) with the reference definitions (particles.f90:780-788), the u-cell
+ * counts (nnx+1)(nny+1)(nnz+1) and exclusive offsets of the stable counting
+ * sort by u cell, and the sorted permutation (put-order indices).  Any output
+ * may be NULL. */
+int ntlp_gpu_bin(int32_t *cellidx, int32_t *counts, int32_t *offsets, int32_t *perm);
+/* calc_tauc_min's per-w-cell count num_array(nnx,nny,nnz) (particles.f90:3320-3339). */
+int ntlp_gpu_wcell_counts(int32_t *num_array);
+int64_t ntlp_gpu_num_ucells(void);
+
+/* -------------------------------------------------------- fused stepping */
+
+/* nsteps whole droplet steps against the resident fields (which stay fixed),
+ * each: gather + advance (mode) + boundary conditions + coupling scatter +
+ * counting sort for the next step.  Step k uses it = it0 + k.  The result is
+ * what the same number of single-step calls would leave. */
+int ntlp_gpu_step(int32_t mode, int32_t precision, int32_t nsteps, int32_t it0);
+
+/* -------------------------------------------------- counters and timing */
+
+/* Sums of the Newton bookkeeping since the last reset. */
+int ntlp_gpu_counters(ntlp_counters *out, int32_t reset);
+/* Per-step rows of the same counters, oldest first, for the last steps (up to
+ * max_rows, at most 4095): row r holds NTLP_STEP_ROW int64 values num100,
+ * numimpos, actnum, denum, gn_iters, lm_iters, solves, steps, gn_hist[64].
+ * Returns the number of rows written. */
+#define NTLP_STEP_ROW 72
+int ntlp_gpu_step_counters(int64_t *rows, int32_t max_rows);
+/* Per-kernel timing with HIP events on the library's stream.  enable=1 starts
+ * recording (and clears totals); ntlp_gpu_kernel_times fills up to `cap`
+ * entries of name (32 bytes each), launches and total milliseconds. */
+int ntlp_gpu_timing(int32_t enable);
+int ntlp_gpu_kernel_times(char *names, int64_t *launches, double *total_ms, int32_t cap);
+/* Waits for all work of the context. */
+int ntlp_gpu_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,369 @@
+"""ntlp_amd -- MI355X-native Lagrangian droplet step of NTLP.
+
+Python host binding of libntlp_gpu.so (C ABI in include/ntlp_gpu.h).  The
+methods keep the names and argument meaning of the reference's particle
+entry points (particles.f90): ``particle_update_rk3(istage)``,
+``particle_update_BE()``, ``particle_coupling_update()``; failures raise
+:class:`NtlpError` carrying the library's error text, where the reference
+would stop.
+
+There is no CPU fallback: importing works without a GPU, but every compute
+call goes to the HIP library and raises if it cannot run.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import grid as _grid  # noqa: F401  (re-exported helpers)
+from .grid import Grid
+
+__all__ = ["Grid", "Consts", "Flags", "Particles", "DropletStep", "NtlpError", "load_library",
+           "NREAL", "COL", "MODE_RK3", "MODE_BE", "PREC_FP64", "PREC_MIXED", "PREC_FP64_AJ",
+           "LAYOUT_EXT", "LAYOUT_CORE", "LAYOUT_XPENCIL", "LIB_PATH"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libntlp_gpu.so")
+
+# droplet record columns (enum ntlp_column)
+COLUMNS = ["x", "y", "z", "vx", "vy", "vz", "ufx", "ufy", "ufz", "xrhsx", "xrhsy", "xrhsz",
+           "vrhsx", "vrhsy", "vrhsz", "Tp", "Tprhs_s", "Tprhs_L", "Tf", "radius", "radrhs", "qinf",
+           "qstar", "res", "m_s", "kappa_s", "rc", "actres", "numact", "vxold", "vyold", "vzold",
+           "Tpold", "radiusold"]
+COL = {name: i for i, name in enumerate(COLUMNS)}
+NREAL = len(COLUMNS)
+
+MODE_RK3, MODE_BE = 1, 2
+PREC_FP64, PREC_MIXED, PREC_FP64_AJ = 0, 1, 2
+LAYOUT_EXT, LAYOUT_CORE, LAYOUT_XPENCIL = 0, 1, 2
+GN_HIST = 64
+STEP_ROW = 72
+
+NTLP_ERRORS = {-1: "NTLP_E_INVALID", -2: "NTLP_E_NOT_INIT", -3: "NTLP_E_HIP", -4: "NTLP_E_NOMEM",
+               -5: "NTLP_E_RANGE", -6: "NTLP_E_UNSUPPORTED", -7: "NTLP_E_STATE"}
+
+
+class NtlpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{NTLP_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _CGrid(ctypes.Structure):
+    _fields_ = [("nnx", ctypes.c_int32), ("nny", ctypes.c_int32), ("nnz", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("xl", ctypes.c_double), ("yl", ctypes.c_double),
+                ("z", ctypes.c_void_p), ("zz", ctypes.c_void_p), ("dzw", ctypes.c_void_p),
+                ("dzu", ctypes.c_void_p)]
+
+
+_CONST_SCALARS = ["dt", "nuf", "Cpa", "Cpp", "CpaCpp", "Pra", "Sc", "Rd", "rhow", "rhos", "Mw", "Ru",
+                  "Gam", "Lv", "Cpv", "surf_rho", "surf_p", "tsfcc1", "grav", "radius_init", "pi2"]
+
+
+class _CConsts(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_double) for n in _CONST_SCALARS] +
+                [("part_grav", ctypes.c_double * 3), ("zetas", ctypes.c_double * 3),
+                 ("gama", ctypes.c_double * 3)])
+
+
+class _CFlags(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("ilin", "icouple", "iTcouple", "iHcouple", "ievap", "iexner", "icase", "reserved")]
+
+
+class _CCounters(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int64) for n in
+                 ("num100", "numimpos", "actnum", "denum", "gn_iters", "lm_iters", "solves", "steps")] +
+                [("gn_hist", ctypes.c_int64 * GN_HIST)])
+
+
+@dataclass
+class Consts:
+    """Namelist constants of the droplet path (defs.F:43-46, particles.f90:42,
+    test_cases/pi_chamber/params.in:105-139) and the RK3 coefficients of setcon
+    (les.F:856-861).  Defaults: the pi-chamber values with kappa_s = 1.2,
+    part_grav = (0, 0, -9.81) and surf_rho = 1.2."""
+    dt: float = 1.0e-3
+    nuf: float = 1.57e-5
+    Cpa: float = 1006.0
+    Cpp: float = 4179.0
+    Pra: float = 0.715
+    Sc: float = 0.615
+    Rd: float = 286.9
+    rhow: float = 1000.0
+    rhos: float = 2000.0
+    Mw: float = 0.018015
+    Ru: float = 8.3144
+    Gam: float = 7.28e-2
+    Lv: float = (25.0 - 0.02274 * 26.0) * 10.0 ** 5
+    Cpv: float = 1952.0
+    surf_rho: float = 1.2
+    surf_p: float = 101325.0
+    tsfcc1: float = 300.0
+    grav: float = 9.81
+    radius_init: float = 1.0e-5
+    part_grav: tuple = (0.0, 0.0, -9.81)
+    zetas: tuple = (0.0, -17.0 / 60.0, -5.0 / 12.0)
+    gama: tuple = (8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0)
+
+    @property
+    def CpaCpp(self) -> float:  # les.F:10759
+        return self.Cpa / self.Cpp
+
+    @property
+    def pi2(self) -> float:  # les.F:5387
+        return 2.0 * (4.0 * np.arctan(1.0))
+
+    def to_c(self) -> _CConsts:
+        c = _CConsts()
+        for n in _CONST_SCALARS:
+            setattr(c, n, float(getattr(self, n)))
+        c.part_grav = (ctypes.c_double * 3)(*self.part_grav)
+        c.zetas = (ctypes.c_double * 3)(*self.zetas)
+        c.gama = (ctypes.c_double * 3)(*self.gama)
+        return c
+
+
+@dataclass
+class Flags:
+    """Hot-path flags of namelist &flags (defs.F:8-14); defaults are those of
+    every shipped test case except icase (0: reflecting walls)."""
+    ilin: int = 1
+    icouple: int = 1
+    iTcouple: int = 1
+    iHcouple: int = 1
+    ievap: int = 1
+    iexner: int = 0
+    icase: int = 0
+
+    def to_c(self) -> _CFlags:
+        return _CFlags(self.ilin, self.icouple, self.iTcouple, self.iHcouple, self.ievap, self.iexner,
+                       self.icase, 0)
+
+
+@dataclass
+class Particles:
+    """Droplets as a structure of arrays: ``soa[COL[name], i]`` (FP64),
+    multiplicity, and the (pidx, procidx) identity of the reference record."""
+    soa: np.ndarray
+    mult: np.ndarray
+    pidx: np.ndarray
+    procidx: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.soa.shape[1])
+
+    def __getitem__(self, name: str) -> np.ndarray:
+        return self.soa[COL[name]]
+
+    def __setitem__(self, name: str, value) -> None:
+        self.soa[COL[name]] = value
+
+    def copy(self) -> "Particles":
+        return Particles(self.soa.copy(), self.mult.copy(), self.pidx.copy(), self.procidx.copy())
+
+    @staticmethod
+    def empty(n: int) -> "Particles":
+        return Particles(np.zeros((NREAL, n)), np.ones(n, dtype=np.int64),
+                         np.arange(1, n + 1, dtype=np.int32), np.zeros(n, dtype=np.int32))
+
+
+_lib_handle = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libntlp_gpu.so (built by ``python -m ntlp_amd.build``).  Raises if
+    it is missing: there is no other implementation to fall back to."""
+    global _lib_handle
+    if _lib_handle is not None:
+        return _lib_handle
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing; build it with `python -m ntlp_amd.build` "
+                           "(hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(path)
+    i32, i64, vp, dp = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double
+    sig = {
+        "ntlp_gpu_init": (ctypes.c_int, [ctypes.POINTER(_CGrid), ctypes.POINTER(_CConsts), ctypes.POINTER(_CFlags)]),
+        "ntlp_gpu_finalize": (ctypes.c_int, []),
+        "ntlp_gpu_last_error": (ctypes.c_char_p, []),
+        "ntlp_gpu_set_dt": (ctypes.c_int, [dp]),
+        "ntlp_gpu_set_fields": (ctypes.c_int, [vp, vp, vp, vp, vp, i32]),
+        "ntlp_gpu_particles_put": (ctypes.c_int, [vp, vp, vp, vp, i64]),
+        "ntlp_gpu_particles_get": (ctypes.c_int, [vp, vp, vp, vp, i64]),
+        "ntlp_gpu_particles_put_records": (ctypes.c_int, [vp, i64]),
+        "ntlp_gpu_particles_get_records": (ctypes.c_int, [vp, i64]),
+        "ntlp_gpu_particles_count": (i64, []),
+        "ntlp_gpu_interp": (ctypes.c_int, [i32, i32]),
+        "ntlp_gpu_crit_radius": (ctypes.c_int, []),
+        "ntlp_gpu_particle_update_rk3": (ctypes.c_int, [i32, i32]),
+        "ntlp_gpu_particle_update_be": (ctypes.c_int, [i32]),
+        "ntlp_gpu_particle_update_rk3_prec": (ctypes.c_int, [i32, i32, i32]),
+        "ntlp_gpu_particle_update_be_prec": (ctypes.c_int, [i32, i32]),
+        "ntlp_gpu_particle_coupling_update": (ctypes.c_int, [vp, i32]),
+        "ntlp_gpu_get_sources": (ctypes.c_int, [vp]),
+        "ntlp_gpu_bin": (ctypes.c_int, [vp, vp, vp, vp]),
+        "ntlp_gpu_wcell_counts": (ctypes.c_int, [vp]),
+        "ntlp_gpu_num_ucells": (i64, []),
+        "ntlp_gpu_step": (ctypes.c_int, [i32, i32, i32, i32]),
+        "ntlp_gpu_counters": (ctypes.c_int, [ctypes.POINTER(_CCounters), i32]),
+        "ntlp_gpu_step_counters": (ctypes.c_int, [vp, i32]),
+        "ntlp_gpu_timing": (ctypes.c_int, [i32]),
+        "ntlp_gpu_kernel_times": (ctypes.c_int, [vp, vp, vp, i32]),
+        "ntlp_gpu_sync": (ctypes.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib_handle = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class DropletStep:
+    """The droplet-step context on the current GPU (the module state of
+    particles.f90).  One per process, like the reference module."""
+
+    def __init__(self, grid: Grid, consts: Consts | None = None, flags: Flags | None = None):
+        self.lib = load_library()
+        self.grid = grid
+        self.consts = consts or Consts()
+        self.flags = flags or Flags()
+        self._cgrid = grid.to_c(_CGrid)
+        self._check(self.lib.ntlp_gpu_init(ctypes.byref(self._cgrid), ctypes.byref(self.consts.to_c()),
+                                           ctypes.byref(self.flags.to_c())))
+        self.n = 0
+
+    # ------------------------------------------------------------ plumbing
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            raise NtlpError(rc, self.lib.ntlp_gpu_last_error().decode())
+        return rc
+
+    def close(self) -> None:
+        self._check(self.lib.ntlp_gpu_finalize())
+
+    def set_dt(self, dt: float) -> None:
+        self.consts.dt = dt
+        self._check(self.lib.ntlp_gpu_set_dt(dt))
+
+    # --------------------------------------------------------------- fields
+    def set_fields(self, u, v, w, T, q, layout: int = LAYOUT_CORE) -> None:
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, T, q)]
+        self._check(self.lib.ntlp_gpu_set_fields(*[_ptr(a) for a in arrs], layout))
+
+    # ------------------------------------------------------------- droplets
+    def put(self, p: Particles) -> None:
+        soa = np.ascontiguousarray(p.soa, dtype=np.float64)
+        mult = np.ascontiguousarray(p.mult, dtype=np.int64)
+        pidx = np.ascontiguousarray(p.pidx, dtype=np.int32)
+        procidx = np.ascontiguousarray(p.procidx, dtype=np.int32)
+        self._check(self.lib.ntlp_gpu_particles_put(_ptr(soa), _ptr(mult), _ptr(pidx), _ptr(procidx), p.n))
+        self.n = p.n
+
+    def get(self) -> Particles:
+        n = self.n
+        p = Particles(np.empty((NREAL, n)), np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int32),
+                      np.empty(n, dtype=np.int32))
+        self._check(self.lib.ntlp_gpu_particles_get(_ptr(p.soa), _ptr(p.mult), _ptr(p.pidx), _ptr(p.procidx), n))
+        return p
+
+    def put_records(self, records: np.ndarray) -> None:
+        rec = np.ascontiguousarray(records)
+        n = rec.nbytes // 352
+        self._check(self.lib.ntlp_gpu_particles_put_records(_ptr(rec), n))
+        self.n = n
+
+    def get_records(self) -> np.ndarray:
+        rec = np.empty(self.n * 352, dtype=np.uint8)
+        self._check(self.lib.ntlp_gpu_particles_get_records(_ptr(rec), self.n))
+        return rec
+
+    # ----------------------------------------------- reference entry points
+    def interp(self, set_vp: bool = False, set_tp: bool = False) -> None:
+        """uf_interp_lin / uf_interp for every droplet (particles.f90:554-847)."""
+        self._check(self.lib.ntlp_gpu_interp(int(set_vp), int(set_tp)))
+
+    def crit_radius(self) -> None:
+        """rc = crit_radius(m_s, kappa_s, Tp) for every droplet (particles.f90:2264-2266)."""
+        self._check(self.lib.ntlp_gpu_crit_radius())
+
+    def particle_update_rk3(self, istage: int, it: int = 2, precision: int = PREC_FP64) -> None:
+        self._check(self.lib.ntlp_gpu_particle_update_rk3_prec(istage, it, precision))
+
+    def particle_update_BE(self, it: int = 2, precision: int = PREC_FP64) -> None:
+        self._check(self.lib.ntlp_gpu_particle_update_be_prec(it, precision))
+
+    def particle_coupling_update(self, deterministic: bool = False) -> np.ndarray:
+        """Returns partsrc_t(:,:,:,1:3), partTsrc_t, partHsrc_t, partTEsrc_t as
+        an array (6, nnx+1, nny+1, nnz+2) (C order; z fastest)."""
+        g = self.grid
+        out = np.empty((6, g.nnx + 1, g.nny + 1, g.nnz + 2))
+        self._check(self.lib.ntlp_gpu_particle_coupling_update(_ptr(out), int(deterministic)))
+        return out
+
+    def sources(self) -> np.ndarray:
+        """The resident coupling sources, (6, nnx+1, nny+1, nnz+2)."""
+        g = self.grid
+        out = np.empty((6, g.nnx + 1, g.nny + 1, g.nnz + 2))
+        self._check(self.lib.ntlp_gpu_get_sources(_ptr(out)))
+        return out
+
+    # -------------------------------------------------------------- binning
+    def bin(self):
+        n, nc = self.n, int(self.lib.ntlp_gpu_num_ucells())
+        cellidx = np.empty((n, 4), dtype=np.int32)
+        counts = np.empty(nc, dtype=np.int32)
+        offsets = np.empty(nc + 1, dtype=np.int32)
+        perm = np.empty(n, dtype=np.int32)
+        self._check(self.lib.ntlp_gpu_bin(_ptr(cellidx), _ptr(counts), _ptr(offsets), _ptr(perm)))
+        return cellidx, counts, offsets, perm
+
+    def wcell_counts(self) -> np.ndarray:
+        g = self.grid
+        out = np.empty((g.nnz, g.nny, g.nnx), dtype=np.int32)
+        self._check(self.lib.ntlp_gpu_wcell_counts(_ptr(out)))
+        return out
+
+    # ------------------------------------------------------------ stepping
+    def step(self, mode: int = MODE_BE, precision: int = PREC_FP64, nsteps: int = 1, it0: int = 2) -> None:
+        self._check(self.lib.ntlp_gpu_step(mode, precision, nsteps, it0))
+
+    def counters(self, reset: bool = False) -> dict:
+        c = _CCounters()
+        self._check(self.lib.ntlp_gpu_counters(ctypes.byref(c), int(reset)))
+        d = {n: int(getattr(c, n)) for n in
+             ("num100", "numimpos", "actnum", "denum", "gn_iters", "lm_iters", "solves", "steps")}
+        d["gn_hist"] = [int(v) for v in c.gn_hist]
+        return d
+
+    def step_counters(self, max_rows: int = 4095) -> np.ndarray:
+        rows = np.zeros((max_rows, STEP_ROW), dtype=np.int64)
+        nr = self._check(self.lib.ntlp_gpu_step_counters(_ptr(rows), max_rows))
+        return rows[:nr]
+
+    def timing(self, enable: bool) -> None:
+        self._check(self.lib.ntlp_gpu_timing(int(enable)))
+
+    def kernel_times(self) -> dict:
+        cap = 64
+        names = ctypes.create_string_buffer(32 * cap)
+        launches = np.zeros(cap, dtype=np.int64)
+        total = np.zeros(cap, dtype=np.float64)
+        k = self._check(self.lib.ntlp_gpu_kernel_times(ctypes.cast(names, ctypes.c_void_p), _ptr(launches),
+                                                       _ptr(total), cap))
+        out = {}
+        for i in range(k):
+            nm = names.raw[32 * i:32 * (i + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (int(launches[i]), float(total[i]))
+        return out
+
+    def sync(self) -> None:
+        self._check(self.lib.ntlp_gpu_sync())

@@ -1,0 +1,326 @@
+// device_common.hpp -- device-side data structures, cell indexing and the
+// trilinear / 6th-order gathers of the droplet step (gfx950).
+//
+// Index conventions are the reference's (particles.f90): ipt/jpt are 1-based
+// horizontal node indices, kpt/kwpt are u-level / w-level vertical indices in
+// 0..nnz+1, ext arrays are (0:nnz+1, -1:nny+3, -1:nnx+3) with z fastest.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ntlp {
+
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kMaxLevels = 1024;  // nnz + 2 <= kMaxLevels (LDS copies of z, zz)
+
+struct DevGrid {
+  int nnx, nny, nnz;
+  int interleaved;  // zz(k) < z(k) < zz(k+1) strictly: one search yields kpt and kwpt
+  int nsteps;       // steps of the fixed-length binary search over nnz+2 levels
+  int pad_;
+  double dx, dy, xl, yl;
+  const double *z, *zz, *dzw, *dzu;  // device copies of the grid metadata
+};
+
+struct DevConsts {
+  double dt, nuf, Cpa, Cpp, CpaCpp, Pra, Sc, Rd, rhow, rhos, Mw, Ru, Gam, Lv, Cpv;
+  double surf_rho, surf_p, tsfcc1, grav, radius_init, pi2;
+  double g[3], zetas[3], gama[3];
+  double Sc13, Pra13;  // Sc**(1./3.), Pra**(1./3.): loop invariants of ie_vrt_nd
+};
+
+struct DevFlags {
+  int ilin, icouple, iTcouple, iHcouple, ievap, iexner, icase, pad_;
+};
+
+struct DevFields {
+  const double *u, *v, *w, *T, *q;  // ext layout, FP64
+  const float *u32, *v32, *w32;     // FP32 copies of u, v, w (mixed precision gathers)
+  int64_t sj, si;                   // element strides of j and i (k stride is 1)
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ int64_t ext_off(const DevFields& F, int k, int j, int i) {
+  return (int64_t)k + F.sj * (int64_t)(j + 1) + F.si * (int64_t)(i + 1);
+}
+
+// Number of leading elements of a(0:len-1) that are not greater than x, i.e.
+// the 0-based index of the first element > x, or len when there is none.  For
+// a monotone array this is exactly what Fortran's
+// minloc(a, 1, mask=(a .gt. x)) selects (as a 1-based position minus one);
+// a NaN x compares false everywhere and yields len, like the empty mask.
+// Fixed trip count (branch-free, lane-uniform).
+__device__ __forceinline__ int first_gt(const double* a, int len, int nsteps, double x) {
+  int pos = 0;
+  for (int s = nsteps - 1; s >= 0; --s) {
+    int cand = pos + (1 << s);
+    if (cand <= len && !(a[cand - 1] > x)) pos = cand;
+  }
+  return pos;
+}
+
+struct Cell {
+  int ipt, jpt, kpt, kwpt;
+};
+
+// Cell indices of particles.f90:780-788: ipt = floor(x/dx)+1,
+// jpt = floor(y/dy)+1, kpt = minloc(zz, zz>z) - 2, kwpt = minloc(z, z>z) - 2
+// (minloc = 0 when empty, giving -2).  s_z / s_zz are LDS copies.
+__device__ __forceinline__ Cell cell_of(const DevGrid& g, const double* s_z, const double* s_zz,
+                                        double x, double y, double z) {
+  Cell c;
+  c.ipt = (int)floor(x / g.dx) + 1;
+  c.jpt = (int)floor(y / g.dy) + 1;
+  const int len = g.nnz + 2;
+  int pzz = first_gt(s_zz, len, g.nsteps, z);
+  int pz;
+  if (g.interleaved) {
+    // z(m-2) < zz(m-1) <= x < zz(m) < z(m): the first z above x is m-1 or m.
+    pz = (pzz >= 1 && s_z[pzz - 1] > z) ? pzz - 1 : pzz;
+  } else {
+    pz = first_gt(s_z, len, g.nsteps, z);
+  }
+  c.kpt = (pzz == len) ? -2 : pzz - 1;
+  c.kwpt = (pz == len) ? -2 : pz - 1;
+  return c;
+}
+
+// w-cell index of calc_tauc_min (particles.f90:3334): minloc(z, z>z) - 1.
+__device__ __forceinline__ int wcell_of(const DevGrid& g, const double* s_z, double z) {
+  const int len = g.nnz + 2;
+  int pz = first_gt(s_z, len, g.nsteps, z);
+  return (pz == len) ? -1 : pz;
+}
+
+// u-cell sort key: (ipt-1, jpt-1, kpt) clamped into the box
+// [0,nnx] x [0,nny] x [0,nnz] that every in-domain droplet occupies, k fastest
+// so that consecutive keys walk a z column of the ext arrays.
+__device__ __forceinline__ int ukey_of(const DevGrid& g, const Cell& c) {
+  int i0 = clampi(c.ipt - 1, 0, g.nnx);
+  int j0 = clampi(c.jpt - 1, 0, g.nny);
+  int k0 = clampi(c.kpt, 0, g.nnz);
+  return k0 + (g.nnz + 1) * (j0 + (g.nny + 1) * i0);
+}
+
+// Load the vertical grid into LDS (call by all threads, then __syncthreads).
+__device__ __forceinline__ void stage_grid(const DevGrid& g, double* s_z, double* s_zz) {
+  for (int k = threadIdx.x; k < g.nnz + 2; k += blockDim.x) {
+    s_z[k] = g.z[k];
+    s_zz[k] = g.zz[k];
+  }
+}
+
+// ---------------------------------------------------------------- gathers
+
+struct Fluid {
+  double u[3];
+  double T, q;
+};
+
+// uf_interp_lin (particles.f90:767-847) in FP64, same product order
+// ((f*wtx)*wty)*wtz and (i,j,k) accumulation order.  The zeroth-order wall
+// override of 831-842 leaves the node (ipt+1, jpt+1) values in uf(1:2), Tf and
+// qinf.  Indices are clamped only where the reference would leave its arrays.
+template <bool TQ>
+__device__ __forceinline__ Fluid gather_lin_f64(const DevGrid& g, const DevFields& F,
+                                                const double* s_z, const double* s_zz, double x,
+                                                double y, double z, const Cell& c) {
+  const int nnz = g.nnz, kmax = nnz + 1;
+  const int kpt = c.kpt, kwpt = c.kwpt;
+  const double dzu1 = g.dzu[clampi(kpt + 1, 0, nnz + 2)];
+  const double dzw1 = g.dzw[clampi(kwpt + 1, 0, nnz + 2)];
+  const int ku[2] = {clampi(kpt, 0, kmax), clampi(kpt + 1, 0, kmax)};
+  const int kw[2] = {clampi(kwpt, 0, kmax), clampi(kwpt + 1, 0, kmax)};
+  double wtz[2], wtw[2], wtx[2], wty[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    wtz[k] = 1.0 - fabs(z - s_zz[ku[k]]) / dzu1;
+    wtw[k] = 1.0 - fabs(z - s_z[kw[k]]) / dzw1;
+    wtx[k] = 1.0 - fabs(x - g.dx * (double)(k + c.ipt - 1)) / g.dx;
+    wty[k] = 1.0 - fabs(y - g.dy * (double)(k + c.jpt - 1)) / g.dy;
+  }
+  const int ic = clampi(c.ipt, -1, g.nnx + 2), jc = clampi(c.jpt, -1, g.nny + 2);
+  Fluid f = {{0.0, 0.0, 0.0}, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t col = F.sj * (int64_t)(jc + j + 1) + F.si * (int64_t)(ic + i + 1);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int64_t o = col + ku[k], ow = col + kw[k];
+        f.u[0] = f.u[0] + F.u[o] * wtx[i] * wty[j] * wtz[k];
+        f.u[1] = f.u[1] + F.v[o] * wtx[i] * wty[j] * wtz[k];
+        f.u[2] = f.u[2] + F.w[ow] * wtx[i] * wty[j] * wtw[k];
+        if (TQ) {
+          f.T = f.T + F.T[o] * wtx[i] * wty[j] * wtz[k];
+          f.q = f.q + F.q[o] * wtx[i] * wty[j] * wtz[k];
+        }
+      }
+    }
+  if (kpt == nnz || kpt == 0) {
+    const int64_t o = ext_off(F, kpt == 0 ? 1 : kpt, jc + 1, ic + 1);
+    f.u[0] = F.u[o];
+    f.u[1] = F.v[o];
+    if (TQ) {
+      f.T = F.T[o];
+      f.q = F.q[o];
+    }
+  }
+  return f;
+}
+
+// Mixed-precision trilinear gather: FP32 fractional weights (computed from the
+// FP64 offset of the droplet from its cell origin), u, v, w from FP32 field
+// copies accumulated in FP32; T and q from the FP64 fields in the form
+// f000 + sum w_n (f_n - f000), which equals the plain weighted sum because the
+// trilinear weights sum to one, and keeps the FP32 weight rounding off the
+// 300 K offset of T.  Same wall override as the FP64 gather.
+template <bool TQ>
+__device__ __forceinline__ Fluid gather_lin_mixed(const DevGrid& g, const DevFields& F,
+                                                  const double* s_z, const double* s_zz,
+                                                  double x, double y, double z, const Cell& c) {
+  const int nnz = g.nnz, kmax = nnz + 1;
+  const int kpt = c.kpt, kwpt = c.kwpt;
+  const float dzu1 = (float)g.dzu[clampi(kpt + 1, 0, nnz + 2)];
+  const float dzw1 = (float)g.dzw[clampi(kwpt + 1, 0, nnz + 2)];
+  const int ku0 = clampi(kpt, 0, kmax), ku1 = clampi(kpt + 1, 0, kmax);
+  const int kw0 = clampi(kwpt, 0, kmax), kw1 = clampi(kwpt + 1, 0, kmax);
+  const float fx = (float)(x - g.dx * (double)(c.ipt - 1)) / (float)g.dx;
+  const float fy = (float)(y - g.dy * (double)(c.jpt - 1)) / (float)g.dy;
+  const float fz = (float)(z - s_zz[ku0]) / dzu1;
+  const float fw = (float)(z - s_z[kw0]) / dzw1;
+  const float wtx[2] = {1.0f - fx, fx}, wty[2] = {1.0f - fy, fy};
+  const float wtz[2] = {1.0f - fz, fz}, wtw[2] = {1.0f - fw, fw};
+  const int ic = clampi(c.ipt, -1, g.nnx + 2), jc = clampi(c.jpt, -1, g.nny + 2);
+  float u0 = 0.f, u1 = 0.f, u2 = 0.f;
+  double T = 0.0, q = 0.0, T0 = 0.0, q0 = 0.0;
+  const int64_t col00 = F.sj * (int64_t)(jc + 1) + F.si * (int64_t)(ic + 1);
+  if (TQ) {
+    T0 = F.T[col00 + ku0];
+    q0 = F.q[col00 + ku0];
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int64_t col = col00 + F.sj * j + F.si * i;
+      const float wxy = wtx[i] * wty[j];
+      const float a0 = wxy * wtz[0], a1 = wxy * wtz[1];
+      u0 += F.u32[col + ku0] * a0 + F.u32[col + ku1] * a1;
+      u1 += F.v32[col + ku0] * a0 + F.v32[col + ku1] * a1;
+      u2 += F.w32[col + kw0] * (wxy * wtw[0]) + F.w32[col + kw1] * (wxy * wtw[1]);
+      if (TQ) {
+        T += (F.T[col + ku0] - T0) * (double)a0 + (F.T[col + ku1] - T0) * (double)a1;
+        q += (F.q[col + ku0] - q0) * (double)a0 + (F.q[col + ku1] - q0) * (double)a1;
+      }
+    }
+  Fluid f;
+  f.u[0] = u0;
+  f.u[1] = u1;
+  f.u[2] = u2;
+  f.T = T0 + T;
+  f.q = q0 + q;
+  if (kpt == nnz || kpt == 0) {
+    const int64_t o = ext_off(F, kpt == 0 ? 1 : kpt, jc + 1, ic + 1);
+    f.u[0] = F.u32[o];
+    f.u[1] = F.v32[o];
+    if (TQ) {
+      f.T = F.T[o];
+      f.q = F.q[o];
+    }
+  }
+  return f;
+}
+
+// uf_interp (particles.f90:554-765): 6th-order Lagrange on 6x6x6 nodes with
+// the reference's one-sided truncation near the walls.  FP64 in all
+// precisions.  The out-of-domain branches with first=last=0 leave the weights
+// zero (the reference evaluates one stray iteration outside its arrays there).
+__device__ __noinline__ Fluid gather6_f64(const DevGrid& g, const DevFields& F, const double* s_z,
+                                          const double* s_zz, double x, double y, double z,
+                                          const Cell& c) {
+  const int nnz = g.nnz;
+  int kuv[6], kwv[6];
+  double wt[4][6];
+  kuv[2] = c.kpt;
+  kuv[3] = kuv[2] + 1; kuv[4] = kuv[3] + 1; kuv[5] = kuv[4] + 1;
+  kuv[1] = kuv[2] - 1; kuv[0] = kuv[1] - 1;
+  kwv[2] = c.kwpt;
+  kwv[3] = kwv[2] + 1; kwv[4] = kwv[3] + 1; kwv[5] = kwv[4] + 1;
+  kwv[1] = kwv[2] - 1; kwv[0] = kwv[1] - 1;
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 6; ++b) wt[a][b] = 0.0;
+  {
+    const double dxv[2] = {g.dx, g.dy};
+    const double xp[2] = {x, y};
+    const int base[2] = {c.ipt, c.jpt};
+    for (int d = 0; d < 2; ++d)
+      for (int j = 0; j < 6; ++j) {
+        double xj = dxv[d] * (double)(base[d] + j - 3);
+        double pj = 1.0;
+        for (int k = 0; k < 6; ++k) {
+          double xk = dxv[d] * (double)(base[d] + k - 3);
+          if (j != k) pj = pj * (xp[d] - xk) / (xj - xk);
+        }
+        wt[d][j] = pj;
+      }
+  }
+  int first, last;
+  if (kuv[2] == 1) { first = 3; last = 4; kuv[0] = 1; kuv[1] = 1; }
+  else if (kuv[2] == 0) { first = 4; last = 5; kuv[0] = 1; kuv[1] = 1; kuv[2] = 1; }
+  else if (kuv[2] < 0) { first = 0; last = 0; }
+  else if (kuv[2] == 2) { first = 2; last = 5; }
+  else if (kuv[2] == nnz) { first = 2; last = 3; kuv[3] = nnz; kuv[4] = nnz; kuv[5] = nnz; }
+  else if (kuv[2] > nnz) { first = 0; last = 0; }
+  else if (kuv[2] == nnz - 1) { first = 3; last = 4; kuv[4] = nnz; kuv[5] = nnz; }
+  else if (kuv[2] == nnz - 2) { first = 2; last = 5; }
+  else { first = 1; last = 6; }
+  if (first > 0)
+    for (int j = first; j <= last; ++j) {
+      double xj = s_zz[clampi(kuv[j - 1], 0, nnz + 1)];
+      double pj = 1.0;
+      for (int k = first; k <= last; ++k) {
+        double xk = s_zz[clampi(kuv[k - 1], 0, nnz + 1)];
+        if (j != k) pj = pj * (z - xk) / (xj - xk);
+      }
+      wt[2][j - 1] = pj;
+    }
+  if (kwv[2] == 0) { first = 3; last = 4; kwv[0] = 1; kwv[1] = 1; }
+  else if (kwv[2] < 0) { first = 0; last = 0; kwv[0] = 1; kwv[1] = 1; kwv[2] = 1; }
+  else if (kwv[2] == 1) { first = 2; last = 5; kwv[0] = 1; }
+  else if (kwv[2] == nnz - 1) { first = 3; last = 4; kwv[4] = nnz; kwv[5] = nnz; }
+  else if (kwv[2] >= nnz) { first = 0; last = 0; kwv[2] = nnz; kwv[3] = nnz; kwv[4] = nnz; kwv[5] = nnz; }
+  else if (kwv[2] == nnz - 2) { first = 2; last = 5; kwv[5] = nnz; }
+  else { first = 1; last = 6; }
+  if (first > 0)
+    for (int j = first; j <= last; ++j) {
+      double xj = s_z[clampi(kwv[j - 1], 0, nnz + 1)];
+      double pj = 1.0;
+      for (int k = first; k <= last; ++k) {
+        double xk = s_z[clampi(kwv[k - 1], 0, nnz + 1)];
+        if (j != k) pj = pj * (z - xk) / (xj - xk);
+      }
+      wt[3][j - 1] = pj;
+    }
+  Fluid f = {{0.0, 0.0, 0.0}, 0.0, 0.0};
+  for (int k = 0; k < 6; ++k) {
+    const int izuv = clampi(kuv[k], 0, nnz + 1), izw = clampi(kwv[k], 0, nnz + 1);
+    for (int j = 0; j < 6; ++j) {
+      const int iy = clampi(c.jpt + j - 2, -1, g.nny + 3);
+      for (int i = 0; i < 6; ++i) {
+        const int ix = clampi(c.ipt + i - 2, -1, g.nnx + 3);
+        const int64_t o = ext_off(F, izuv, iy, ix), ow = ext_off(F, izw, iy, ix);
+        f.u[0] = f.u[0] + F.u[o] * wt[0][i] * wt[1][j] * wt[2][k];
+        f.u[1] = f.u[1] + F.v[o] * wt[0][i] * wt[1][j] * wt[2][k];
+        f.u[2] = f.u[2] + F.w[ow] * wt[0][i] * wt[1][j] * wt[3][k];
+        f.T = f.T + F.T[o] * wt[0][i] * wt[1][j] * wt[2][k];
+        f.q = f.q + F.q[o] * wt[0][i] * wt[1][j] * wt[2][k];
+      }
+    }
+  }
+  return f;
+}
+
+}  // namespace ntlp

@@ -1,0 +1,788 @@
+// kernels.hpp -- HIP kernels of the droplet step (gfx950, wave64).
+//
+// Device layout of the droplets (capacity `cap`, slot t):
+//   hot[c*cap + t]   14 FP64 columns the step reads and writes every step
+//                    (kHot* below), kept in cell-sorted slot order and
+//                    ping-ponged between two buffers by the fused step;
+//   mult[t], home[t] multiplicity and the droplet's put-order index;
+//   cold[c*cap + h]  the remaining FP64 record columns, in put order h, only
+//                    touched through home[t] (diagnostics, RK3 stage
+//                    registers, pre-step values for the coupling).
+#pragma once
+#include "physics.hpp"
+
+namespace ntlp {
+
+// hot column indices
+enum {
+  kHX = 0, kHY, kHZ, kHVX, kHVY, kHVZ, kHTP, kHR, kHMS, kHKAPPA, kHRC, kHRES, kHACTRES, kHNUMACT, kNHot
+};
+// cold column indices
+enum {
+  kCUFX = 0, kCUFY, kCUFZ, kCXRHSX, kCXRHSY, kCXRHSZ, kCVRHSX, kCVRHSY, kCVRHSZ, kCTPRHS_S, kCTPRHS_L,
+  kCTF, kCRADRHS, kCQINF, kCQSTAR, kCVXOLD, kCVYOLD, kCVZOLD, kCTPOLD, kCRADIUSOLD, kNCold
+};
+
+// Tally slots in the per-step counter rows.
+enum { kTNum100 = 0, kTNumImpos, kTActnum, kTDenum, kTGnIters, kTLmIters, kTSolves, kTSteps, kTHist0, kTallyWidth = kTHist0 + 64 };
+
+struct DropletBufs {
+  double* hot;
+  int64_t* mult;
+  int32_t* home;
+};
+
+struct StepArgs {
+  DevGrid g;
+  DevConsts C;
+  DevFlags fl;
+  DevFields F;
+  int64_t n, cap;
+  DropletBufs src, dst;  // fused step reads src through perm, writes dst in slot order
+  const int32_t* perm;   // sorted index -> src slot (nullptr: identity)
+  double* cold;          // put-order cold columns
+  double* amounts;       // [6][cap] coupling amounts in dst slot order
+  int32_t* keys;         // [cap] u-cell key of the new position (dst slot order)
+  int32_t* counts;       // u-cell histogram (must be zero on entry)
+  unsigned long long* tally;  // row of kTallyWidth counters
+  int it;
+  int istage;
+  int write_cold;        // write the record diagnostics (final step of a fused call)
+};
+
+// Workgroup -> droplet chunk mapping that keeps consecutive chunks on one XCD:
+// gfx950 deals workgroups to its 8 XCDs round-robin, so workgroup b lands on
+// XCD b%8; giving XCD x the contiguous range x*(nb/8) ... keeps neighbouring
+// (cell-sorted) droplets, and so the same field columns, in one L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const unsigned nb = gridDim.x;  // launched as a multiple of 8
+  const unsigned b = blockIdx.x;
+  return (int64_t)(b & 7u) * (nb >> 3) + (b >> 3);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// --------------------------------------------------------------- tallies
+
+__device__ __forceinline__ long long wave_sum(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Adds a thread's Newton tally to the counter row: wave reductions for the
+// scalar counters and an LDS histogram for the iteration counts.
+__device__ __forceinline__ void flush_tally(const Tally& tl, unsigned long long* row, int* s_hist) {
+  long long v[7] = {tl.num100, tl.numimpos, tl.actnum, tl.denum, tl.gn_iters, tl.lm_iters, tl.solves};
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    long long s = wave_sum(v[k]);
+    if (lane_id() == 0 && s != 0) atomicAdd(&row[k], (unsigned long long)s);
+  }
+  if (tl.hist >= 0) atomicAdd(&s_hist[tl.hist < 63 ? tl.hist : 63], 1);
+  __syncthreads();
+  if (threadIdx.x < 64 && s_hist[threadIdx.x] != 0)
+    atomicAdd(&row[kTHist0 + threadIdx.x], (unsigned long long)s_hist[threadIdx.x]);
+}
+
+// ------------------------------------------------------------ load/store
+
+__device__ __forceinline__ Drop load_drop(const double* hot, int64_t cap, int64_t s) {
+  Drop d;
+  d.x = hot[kHX * cap + s];
+  d.y = hot[kHY * cap + s];
+  d.z = hot[kHZ * cap + s];
+  d.vx = hot[kHVX * cap + s];
+  d.vy = hot[kHVY * cap + s];
+  d.vz = hot[kHVZ * cap + s];
+  d.Tp = hot[kHTP * cap + s];
+  d.r = hot[kHR * cap + s];
+  d.m_s = hot[kHMS * cap + s];
+  d.kappa = hot[kHKAPPA * cap + s];
+  d.rc = hot[kHRC * cap + s];
+  d.res = hot[kHRES * cap + s];
+  d.actres = hot[kHACTRES * cap + s];
+  d.numact = hot[kHNUMACT * cap + s];
+  return d;
+}
+
+__device__ __forceinline__ void store_drop(double* hot, int64_t cap, int64_t s, const Drop& d) {
+  hot[kHX * cap + s] = d.x;
+  hot[kHY * cap + s] = d.y;
+  hot[kHZ * cap + s] = d.z;
+  hot[kHVX * cap + s] = d.vx;
+  hot[kHVY * cap + s] = d.vy;
+  hot[kHVZ * cap + s] = d.vz;
+  hot[kHTP * cap + s] = d.Tp;
+  hot[kHR * cap + s] = d.r;
+  hot[kHMS * cap + s] = d.m_s;
+  hot[kHKAPPA * cap + s] = d.kappa;
+  hot[kHRC * cap + s] = d.rc;
+  hot[kHRES * cap + s] = d.res;
+  hot[kHACTRES * cap + s] = d.actres;
+  hot[kHNUMACT * cap + s] = d.numact;
+}
+
+// State that moves with a droplet in the fused step but is constant in it.
+__device__ __forceinline__ void store_dynamic(double* hot, int64_t cap, int64_t s, const Drop& d) {
+  hot[kHX * cap + s] = d.x;
+  hot[kHY * cap + s] = d.y;
+  hot[kHZ * cap + s] = d.z;
+  hot[kHVX * cap + s] = d.vx;
+  hot[kHVY * cap + s] = d.vy;
+  hot[kHVZ * cap + s] = d.vz;
+  hot[kHTP * cap + s] = d.Tp;
+  hot[kHR * cap + s] = d.r;
+  hot[kHRES * cap + s] = d.res;
+  hot[kHACTRES * cap + s] = d.actres;
+  hot[kHNUMACT * cap + s] = d.numact;
+}
+
+__device__ __forceinline__ void load_rhs(const double* cold, int64_t cap, int64_t h, Diag& o) {
+  o.xrhs[0] = cold[kCXRHSX * cap + h];
+  o.xrhs[1] = cold[kCXRHSY * cap + h];
+  o.xrhs[2] = cold[kCXRHSZ * cap + h];
+  o.vrhs[0] = cold[kCVRHSX * cap + h];
+  o.vrhs[1] = cold[kCVRHSY * cap + h];
+  o.vrhs[2] = cold[kCVRHSZ * cap + h];
+  o.Tprhs_s = cold[kCTPRHS_S * cap + h];
+  o.Tprhs_L = cold[kCTPRHS_L * cap + h];
+  o.radrhs = cold[kCRADRHS * cap + h];
+}
+
+__device__ __forceinline__ void store_gathered(double* cold, int64_t cap, int64_t h, const Diag& o) {
+  cold[kCUFX * cap + h] = o.uf[0];
+  cold[kCUFY * cap + h] = o.uf[1];
+  cold[kCUFZ * cap + h] = o.uf[2];
+  cold[kCTF * cap + h] = o.Tf;
+  cold[kCQINF * cap + h] = o.qinf;
+  cold[kCQSTAR * cap + h] = o.qstar;
+}
+
+__device__ __forceinline__ void store_old(double* cold, int64_t cap, int64_t h, const Diag& o) {
+  cold[kCVXOLD * cap + h] = o.vold[0];
+  cold[kCVYOLD * cap + h] = o.vold[1];
+  cold[kCVZOLD * cap + h] = o.vold[2];
+  cold[kCTPOLD * cap + h] = o.Tpold;
+  cold[kCRADIUSOLD * cap + h] = o.rold;
+}
+
+__device__ __forceinline__ void store_rhs(double* cold, int64_t cap, int64_t h, const Diag& o) {
+  cold[kCXRHSX * cap + h] = o.xrhs[0];
+  cold[kCXRHSY * cap + h] = o.xrhs[1];
+  cold[kCXRHSZ * cap + h] = o.xrhs[2];
+  cold[kCVRHSX * cap + h] = o.vrhs[0];
+  cold[kCVRHSY * cap + h] = o.vrhs[1];
+  cold[kCVRHSZ * cap + h] = o.vrhs[2];
+  cold[kCTPRHS_S * cap + h] = o.Tprhs_s;
+  cold[kCTPRHS_L * cap + h] = o.Tprhs_L;
+  cold[kCRADRHS * cap + h] = o.radrhs;
+}
+
+// Wave-aggregated histogram increment for runs of equal keys in lane order
+// (the droplets are cell sorted, so runs are long): one atomic per run.
+__device__ __forceinline__ void count_key(int* counts, int key, bool valid) {
+  const int lane = lane_id();
+  int prev = __shfl_up(key, 1, 64);
+  bool head = valid && (lane == 0 || prev != key);
+  unsigned long long heads = __ballot(head);
+  unsigned long long vmask = __ballot(valid);
+  if (head) {
+    // run ends before the next head (or at the last valid lane)
+    unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+    int end = after ? (__ffsll((long long)after) - 1) : (64 - __clzll((long long)vmask));
+    atomicAdd(&counts[key], end - lane);
+  }
+}
+
+// ---------------------------------------------------------- fused step
+
+// One whole droplet step of a fused call: gather + advance (MODE) + boundary
+// conditions + coupling amounts + new u-cell key and histogram.  Reads the
+// droplet through perm from the source buffers and writes it to slot t of
+// the destination buffers, so that the destination is cell sorted.
+template <int MODE, int PREC, int ILIN>
+__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  __shared__ int s_hist[64];
+  stage_grid(a.g, s_z, s_zz);
+  if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  const bool valid = t < a.n;
+  Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
+  int key = -1;
+  if (valid) {
+    const int64_t s = a.perm ? (int64_t)a.perm[t] : t;
+    Drop d = load_drop(a.src.hot, a.cap, s);
+    const int64_t mult = a.src.mult[s];
+    const int32_t home = a.src.home[s];
+    Diag o;
+    if (MODE == 1) {  // three particle_update_rk3 stages; stage registers stay in VGPRs
+      o.xrhs[0] = o.xrhs[1] = o.xrhs[2] = 0.0;  // zetas(1) = 0 discards the stale values
+      o.vrhs[0] = o.vrhs[1] = o.vrhs[2] = 0.0;
+      o.Tprhs_s = o.Tprhs_L = o.radrhs = 0.0;
+      for (int st = 1; st <= 3; ++st) rk3_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, st, a.it, o);
+    } else {
+      be_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl);
+    }
+    double am[6];
+    coupling_amounts(a.C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.amounts[k * a.cap + t] = am[k];
+    store_dynamic(a.dst.hot, a.cap, t, d);
+    a.dst.hot[kHMS * a.cap + t] = d.m_s;
+    a.dst.hot[kHKAPPA * a.cap + t] = d.kappa;
+    a.dst.hot[kHRC * a.cap + t] = d.rc;
+    a.dst.mult[t] = mult;
+    a.dst.home[t] = home;
+    const Cell c = cell_of(a.g, s_z, s_zz, d.x, d.y, d.z);
+    key = ukey_of(a.g, c);
+    a.keys[t] = key;
+    if (a.write_cold) {
+      store_gathered(a.cold, a.cap, home, o);
+      store_old(a.cold, a.cap, home, o);
+      if (MODE == 1) store_rhs(a.cold, a.cap, home, o);
+    }
+  }
+  count_key(a.counts, key, valid);
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.tally[kTSteps] = 1ull;
+  if (MODE == 2) flush_tally(tl, a.tally, s_hist);
+}
+
+// ------------------------------------------------- reference entry points
+
+// particle_update_rk3(istage) / particle_update_BE in place on the current
+// buffers (no reordering), reading and writing the record columns.
+template <int MODE, int PREC, int ILIN>
+__global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  __shared__ int s_hist[64];
+  stage_grid(a.g, s_z, s_zz);
+  if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
+  if (t < a.n) {
+    Drop d = load_drop(a.src.hot, a.cap, t);
+    const int32_t home = a.src.home[t];
+    Diag o;
+    if (MODE == 1) {
+      load_rhs(a.cold, a.cap, home, o);
+      rk3_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.istage, a.it, o);
+      store_rhs(a.cold, a.cap, home, o);
+      if (a.istage == 1) store_old(a.cold, a.cap, home, o);
+    } else {
+      be_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl);
+      store_old(a.cold, a.cap, home, o);
+    }
+    store_gathered(a.cold, a.cap, home, o);
+    store_dynamic(a.src.hot, a.cap, t, d);
+  }
+  if (MODE == 2 && blockIdx.x == 0 && threadIdx.x == 0) a.tally[kTSteps] = 1ull;
+  if (MODE == 2) flush_tally(tl, a.tally, s_hist);
+}
+
+// uf_interp_lin / uf_interp for every droplet, optionally vp=uf and Tp=Tf.
+template <int ILIN>
+__global__ void __launch_bounds__(kBlock) k_interp(StepArgs a, int set_vp, int set_tp) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  stage_grid(a.g, s_z, s_zz);
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  if (t >= a.n) return;
+  const double x = a.src.hot[kHX * a.cap + t], y = a.src.hot[kHY * a.cap + t], z = a.src.hot[kHZ * a.cap + t];
+  const Cell c = cell_of(a.g, s_z, s_zz, x, y, z);
+  Fluid f = (ILIN == 1) ? gather_lin_f64<true>(a.g, a.F, s_z, s_zz, x, y, z, c)
+                        : gather6_f64(a.g, a.F, s_z, s_zz, x, y, z, c);
+  const int32_t h = a.src.home[t];
+  a.cold[kCUFX * a.cap + h] = f.u[0];
+  a.cold[kCUFY * a.cap + h] = f.u[1];
+  a.cold[kCUFZ * a.cap + h] = f.u[2];
+  a.cold[kCTF * a.cap + h] = f.T;
+  a.cold[kCQINF * a.cap + h] = f.q;
+  if (set_vp) {
+    a.src.hot[kHVX * a.cap + t] = f.u[0];
+    a.src.hot[kHVY * a.cap + t] = f.u[1];
+    a.src.hot[kHVZ * a.cap + t] = f.u[2];
+  }
+  if (set_tp) a.src.hot[kHTP * a.cap + t] = f.T;
+}
+
+// Coupling amounts of the current state from the stored pre-step values
+// (particle_coupling_update's per-droplet part).
+__global__ void __launch_bounds__(kBlock) k_amounts(StepArgs a) {
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  if (t >= a.n) return;
+  const Drop d = load_drop(a.src.hot, a.cap, t);
+  const int32_t h = a.src.home[t];
+  const double vold[3] = {a.cold[kCVXOLD * a.cap + h], a.cold[kCVYOLD * a.cap + h], a.cold[kCVZOLD * a.cap + h]};
+  double am[6];
+  coupling_amounts(a.C, a.fl, d, a.cold[kCTF * a.cap + h], vold, a.cold[kCTPOLD * a.cap + h],
+                   a.cold[kCRADIUSOLD * a.cap + h], a.src.mult[t], am);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) a.amounts[k * a.cap + t] = am[k];
+}
+
+// -------------------------------------------------------- coupling scatter
+
+struct ScatterArgs {
+  DevGrid g;
+  const double* hot;      // positions (slot order)
+  const double* amounts;  // [6][cap]
+  int64_t n, cap;
+  double* src;            // 6 fields (0:nnz+1, 1:nny+1, 1:nnx+1)
+  int64_t nsrc;           // elements per field
+  unsigned fmask;         // bit f set: field f has nonzero amounts
+};
+
+// Deposit geometry of one droplet (particles.f90:1179-1212): node offsets of
+// its 8 corners in the source arrays and the weights wtt/dV.
+struct Deposit {
+  int key;
+  int64_t node[8];
+  double w[8];
+};
+
+__device__ __forceinline__ Deposit deposit_of(const DevGrid& g, const double* s_z, const double* s_zz,
+                                              double x, double y, double z) {
+  Deposit dp;
+  const Cell c = cell_of(g, s_z, s_zz, x, y, z);
+  dp.key = ukey_of(g, c);
+  const int kpt = clampi(c.kpt, 0, g.nnz);
+  const double dzu1 = g.dzu[kpt + 1];
+  const double dV = g.dx * g.dy * dzu1;
+  double wtx[2], wty[2], wtz[2];
+  int ix[2], iy[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    wtx[k] = 1.0 - fabs(x - g.dx * (double)(k + c.ipt - 1)) / g.dx;
+    wty[k] = 1.0 - fabs(y - g.dy * (double)(k + c.jpt - 1)) / g.dy;
+    wtz[k] = 1.0 - fabs(z - s_zz[kpt + k]) / dzu1;
+    int xi = c.ipt + k, yj = c.jpt + k;
+    // the +1 ghost column holds node nnx+1; nnx+2 (x == xl exactly) is its
+    // periodic image 2.  Out-of-domain indices are clamped.
+    if (xi > g.nnx + 1) xi -= g.nnx;
+    if (xi < 1) xi += g.nnx;
+    if (yj > g.nny + 1) yj -= g.nny;
+    if (yj < 1) yj += g.nny;
+    ix[k] = clampi(xi, 1, g.nnx + 1);
+    iy[k] = clampi(yj, 1, g.nny + 1);
+  }
+  const int64_t sj = g.nnz + 2, si = (int64_t)(g.nnz + 2) * (g.nny + 1);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int c8 = (i * 2 + j) * 2 + k;
+        dp.node[c8] = (int64_t)(kpt + k) + sj * (iy[j] - 1) + si * (ix[i] - 1);
+        dp.w[c8] = wtx[i] * wty[j] * wtz[k] / dV;
+      }
+  return dp;
+}
+
+// Two-way coupling scatter: every droplet deposits amount*wtt/dV on its 8
+// nodes.  Within a wave the droplets are cell sorted, so runs of equal u-cell
+// keys share all 8 nodes: a segmented inclusive scan (shuffles, log2 of the
+// longest run steps) sums each run into its last lane, which issues the FP64
+// global atomics once for the whole run.
+__global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  stage_grid(a.g, s_z, s_zz);
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  const bool valid = t < a.n;
+  const int lane = lane_id();
+  Deposit dp;
+  if (valid) {
+    dp = deposit_of(a.g, s_z, s_zz, a.hot[kHX * a.cap + t], a.hot[kHY * a.cap + t], a.hot[kHZ * a.cap + t]);
+  } else {
+    dp.key = -1 - lane;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      dp.node[c] = 0;
+      dp.w[c] = 0.0;
+    }
+  }
+  // segment bookkeeping
+  const int prev = __shfl_up(dp.key, 1, 64);
+  const bool head = (lane == 0) || (prev != dp.key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int seg_start = 63 - __clzll((long long)(heads & upto));
+  const bool tail = (lane == 63) || ((heads >> (lane + 1)) & 1ull);
+  int len = lane - seg_start + 1;
+  int maxlen = len;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, o, 64));
+  int nsteps = 0;
+  while ((1 << nsteps) < maxlen) ++nsteps;
+  for (int f = 0; f < 6; ++f) {
+    if (!((a.fmask >> f) & 1u)) continue;
+    const double am = valid ? a.amounts[f * a.cap + t] : 0.0;
+    double v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = am * dp.w[c];
+    for (int s = 0; s < nsteps; ++s) {
+      const int off = 1 << s;
+      const bool take = (lane - off) >= seg_start;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const double o = __shfl_up(v[c], off, 64);
+        if (take) v[c] += o;
+      }
+    }
+    if (tail && valid) {
+      double* fld = a.src + (int64_t)f * a.nsrc;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) atomicAdd(&fld[dp.node[c]], v[c]);
+    }
+  }
+}
+
+// Deterministic coupling: each source node sums, in a fixed order, the
+// deposits of the droplets of the (up to 12) u cells that touch it, walking
+// the cell-sorted permutation.  Same result on every run.
+struct PullArgs {
+  DevGrid g;
+  const double* hot;
+  const double* amounts;
+  const int32_t* perm;     // sorted index -> slot
+  const int32_t* offsets;  // u-cell offsets
+  int64_t cap;
+  double* src;
+  int64_t nsrc;
+};
+
+__global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  stage_grid(a.g, s_z, s_zz);
+  __syncthreads();
+  const int64_t node = xcd_block() * kBlock + threadIdx.x;
+  if (node >= a.nsrc) return;
+  const int nz2 = a.g.nnz + 2;
+  const int kz = (int)(node % nz2);
+  const int64_t rest = node / nz2;
+  const int iy = (int)(rest % (a.g.nny + 1)) + 1;
+  const int ix = (int)(rest / (a.g.nny + 1)) + 1;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  // candidate cells along x: (i0, corner i) with i0 = ix-1-i; node 2 also
+  // receives the wrapped corner of cell nnx (x == xl exactly)
+  int cx[3], ci[3], ncx = 0, cy[3], cj[3], ncy = 0;
+  for (int i = 0; i < 2; ++i) {
+    int i0 = ix - 1 - i;
+    if (i0 >= 0 && i0 <= a.g.nnx) { cx[ncx] = i0; ci[ncx] = i; ++ncx; }
+  }
+  if (ix == 2) { cx[ncx] = a.g.nnx; ci[ncx] = 1; ++ncx; }
+  for (int j = 0; j < 2; ++j) {
+    int j0 = iy - 1 - j;
+    if (j0 >= 0 && j0 <= a.g.nny) { cy[ncy] = j0; cj[ncy] = j; ++ncy; }
+  }
+  if (iy == 2) { cy[ncy] = a.g.nny; cj[ncy] = 1; ++ncy; }
+  for (int xa = 0; xa < ncx; ++xa)
+    for (int ya = 0; ya < ncy; ++ya)
+      for (int k = 0; k < 2; ++k) {
+        const int k0 = kz - k;
+        if (k0 < 0 || k0 > a.g.nnz) continue;
+        const int key = k0 + (a.g.nnz + 1) * (cy[ya] + (a.g.nny + 1) * cx[xa]);
+        const int b = a.offsets[key], e = a.offsets[key + 1];
+        const int corner = (ci[xa] * 2 + cj[ya]) * 2 + k;
+        for (int p = b; p < e; ++p) {
+          const int64_t s = a.perm[p];
+          const Deposit dp = deposit_of(a.g, s_z, s_zz, a.hot[kHX * a.cap + s], a.hot[kHY * a.cap + s],
+                                        a.hot[kHZ * a.cap + s]);
+          if (dp.node[corner] != node) continue;  // clamped out-of-domain droplets
+          const double w = dp.w[corner];
+#pragma unroll
+          for (int f = 0; f < 6; ++f) acc[f] += a.amounts[f * a.cap + s] * w;
+        }
+      }
+#pragma unroll
+  for (int f = 0; f < 6; ++f) a.src[(int64_t)f * a.nsrc + node] = acc[f];
+}
+
+// ---------------------------------------------------------------- binning
+
+struct BinArgs {
+  DevGrid g;
+  const double* hot;
+  int64_t n, cap;
+  int32_t* keys;
+  int32_t* counts;
+  int32_t* cellidx;  // optional [n][4] in slot order
+};
+
+__global__ void __launch_bounds__(kBlock) k_keys(BinArgs a) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  stage_grid(a.g, s_z, s_zz);
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  const bool valid = t < a.n;
+  int key = -1;
+  if (valid) {
+    const Cell c = cell_of(a.g, s_z, s_zz, a.hot[kHX * a.cap + t], a.hot[kHY * a.cap + t], a.hot[kHZ * a.cap + t]);
+    key = ukey_of(a.g, c);
+    a.keys[t] = key;
+    if (a.cellidx) {
+      a.cellidx[4 * t + 0] = c.ipt;
+      a.cellidx[4 * t + 1] = c.jpt;
+      a.cellidx[4 * t + 2] = c.kpt;
+      a.cellidx[4 * t + 3] = c.kwpt;
+    }
+  }
+  count_key(a.counts, key, valid);
+}
+
+// calc_tauc_min's num_array over w cells (particles.f90:3320-3339).
+__global__ void __launch_bounds__(kBlock) k_wcells(BinArgs a, int32_t* num) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  stage_grid(a.g, s_z, s_zz);
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  if (t >= a.n) return;
+  const double x = a.hot[kHX * a.cap + t], y = a.hot[kHY * a.cap + t], z = a.hot[kHZ * a.cap + t];
+  const Cell c = cell_of(a.g, s_z, s_zz, x, y, z);
+  const int kc = wcell_of(a.g, s_z, z);
+  if (c.ipt < 1 || c.ipt > a.g.nnx || c.jpt < 1 || c.jpt > a.g.nny || kc < 1 || kc > a.g.nnz) return;
+  atomicAdd(&num[(c.ipt - 1) + (int64_t)a.g.nnx * ((c.jpt - 1) + (int64_t)a.g.nny * (kc - 1))], 1);
+}
+
+// Exclusive scan of the u-cell histogram: per-block sums, one-block scan of
+// the block sums, then a block-local scan with the block's prefix.
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int& total) {
+  const int lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    int w = (lane < nw) ? s_wave[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(w, o, 64);
+      if (lane >= o) w += y;
+    }
+    if (lane < nw) s_wave[lane] = w;
+  }
+  __syncthreads();
+  total = s_wave[nw - 1];
+  const int before = (wid > 0) ? s_wave[wid - 1] : 0;
+  __syncthreads();
+  return before + x - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const int32_t* in, int64_t n, int32_t* bsum) {
+  __shared__ int s_wave[16];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + (int64_t)k * kScanThreads + threadIdx.x;
+    if (i < n) v += in[i];
+  }
+  int total;
+  block_exclusive_scan(v, s_wave, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_bsum(int32_t* bsum, int64_t nb) {
+  __shared__ int s_wave[16];
+  int carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += kScanThreads) {
+    const int64_t i = b0 + threadIdx.x;
+    const int v = (i < nb) ? bsum[i] : 0;
+    int total;
+    const int ex = block_exclusive_scan(v, s_wave, total);
+    if (i < nb) bsum[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_down(const int32_t* in, int64_t n, const int32_t* bsum,
+                                                            int32_t* out) {
+  __shared__ int s_wave[16];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  // thread-contiguous items for a cheap serial prefix
+  const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
+  int v[kScanItems], sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (i0 + k < n) ? in[i0 + k] : 0;
+    sum += v[k];
+  }
+  int total;
+  int run = block_exclusive_scan(sum, s_wave, total) + bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (i0 + k < n) out[i0 + k] = run;
+    run += v[k];
+  }
+  // grand total at out[n]
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = bsum[blockIdx.x] + total;
+}
+
+// Counting-sort scatter: each run of equal keys in a wave takes a block of
+// slots inside its cell with one atomic; counts are decremented back to zero.
+__global__ void __launch_bounds__(kBlock) k_bin_scatter(const int32_t* keys, int64_t n, const int32_t* offsets,
+                                                        int32_t* counts, int32_t* perm) {
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  const bool valid = t < n;
+  const int lane = lane_id();
+  const int key = valid ? keys[t] : -1;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = valid && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long vmask = __ballot(valid);
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int seg_start = max(0, 63 - __clzll((long long)(heads & upto)));
+  int base = 0;
+  if (head) {
+    const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1)) << (lane + 1);
+    const int end = after ? (__ffsll((long long)after) - 1) : (64 - __clzll((long long)vmask));
+    const int len = end - lane;
+    const int old = atomicSub(&counts[key], len);
+    base = offsets[key] + old - len;
+  }
+  base = __shfl(base, seg_start, 64);
+  if (valid) perm[base + (lane - seg_start)] = (int32_t)t;
+}
+
+// Makes the counting sort stable and deterministic: orders every cell's
+// segment of the permutation by tie[perm[p]] (or perm[p] when tie is null).
+__global__ void __launch_bounds__(kBlock) k_cell_sort(const int32_t* offsets, int64_t ncell, int32_t* perm,
+                                                      const int32_t* tie) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= ncell) return;
+  const int b = offsets[c], e = offsets[c + 1];
+  if (e - b < 2) return;
+  for (int i = b + 1; i < e; ++i) {
+    const int32_t v = perm[i];
+    const int32_t kv = tie ? tie[v] : v;
+    int j = i - 1;
+    while (j >= b) {
+      const int32_t u = perm[j];
+      if ((tie ? tie[u] : u) <= kv) break;
+      perm[j + 1] = u;
+      --j;
+    }
+    perm[j + 1] = v;
+  }
+}
+
+// ------------------------------------------------------------ utilities
+
+__global__ void k_fill_f64(double* p, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+__global__ void k_iota(int32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = (int32_t)i;
+}
+
+// Halo fill of fill_ext on one periodic domain: ext(k, j, i) for
+// j in -1..nny+3, i in -1..nnx+3 from core(k, 1..nny, 1..nnx).
+__global__ void k_fill_ext(const double* core, double* ext, int nnx, int nny, int nnz) {
+  const int64_t nz2 = nnz + 2;
+  const int64_t total = nz2 * (nny + 5) * (int64_t)(nnx + 5);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(e % nz2);
+    const int64_t r = e / nz2;
+    const int j = (int)(r % (nny + 5)) - 1;
+    const int i = (int)(r / (nny + 5)) - 1;
+    const int jc = ((j - 1) % nny + nny) % nny + 1;
+    const int ic = ((i - 1) % nnx + nnx) % nnx + 1;
+    ext[e] = core[k + nz2 * ((jc - 1) + (int64_t)nny * (ic - 1))];
+  }
+}
+
+// xtoz_trans + fill_ext from the flow solver's x-fastest array
+// u(1:nnx+2, 1:nny, 0:nnz+1) (les.F:8394-8397, 10060-10131) to ext layout.
+__global__ void k_xpencil_to_ext(const double* xp, double* ext, int nnx, int nny, int nnz) {
+  const int64_t nz2 = nnz + 2;
+  const int64_t total = nz2 * (nny + 5) * (int64_t)(nnx + 5);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(e % nz2);
+    const int64_t r = e / nz2;
+    const int j = (int)(r % (nny + 5)) - 1;
+    const int i = (int)(r / (nny + 5)) - 1;
+    const int jc = ((j - 1) % nny + nny) % nny + 1;
+    const int ic = ((i - 1) % nnx + nnx) % nnx + 1;
+    ext[e] = xp[(ic - 1) + (int64_t)(nnx + 2) * ((jc - 1) + (int64_t)nny * k)];
+  }
+}
+
+__global__ void k_to_f32(const double* in, float* out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) out[i] = (float)in[i];
+}
+
+// Critical radius at droplet creation (create_particle, particles.f90:2264-2266):
+// rc = the table radius maximising the Koehler curve without its exponent
+// (crit_radius, 5307-5327) over the 1024 log-spaced radii of setup_radval
+// (5294-5305); the first maximum wins, like maxloc.
+__constant__ double c_radval[1024];
+
+__global__ void __launch_bounds__(kBlock) k_crit_radius(StepArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n) return;
+  double* hot = a.src.hot;
+  if (a.fl.ievap != 1) {
+    hot[kHRC * a.cap + t] = 0.0;
+    return;
+  }
+  const double m_s = hot[kHMS * a.cap + t], kappa = hot[kHKAPPA * a.cap + t], Tf = hot[kHTP * a.cap + t];
+  const DevConsts& C = a.C;
+  const double A = 2 * C.Mw * C.Gam / C.Ru / C.rhow / Tf;
+  const double B = kappa * m_s;
+  const double D = C.rhos * C.pi2 * 2.0 / 3.0;
+  double best = 0.0;
+  int ib = 0;
+  for (int i = 0; i < 1024; ++i) {
+    const double rv = c_radval[i];
+    const double ss = A / rv - B / (D * (rv * rv * rv));
+    if (i == 0 || ss > best) {
+      best = ss;
+      ib = i;
+    }
+  }
+  hot[kHRC * a.cap + t] = c_radval[ib];
+}
+
+// slot-order column -> put order via home
+__global__ void k_scatter_col_f64(const double* col, const int32_t* home, int64_t n, double* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[home[t]] = col[t];
+}
+__global__ void k_scatter_col_i64(const int64_t* col, const int32_t* home, int64_t n, int64_t* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[home[t]] = col[t];
+}
+
+// slot-order values -> put order via home
+__global__ void k_scatter_home_i32x4(const int32_t* in, const int32_t* home, int64_t n, int32_t* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t h = home[t];
+  for (int k = 0; k < 4; ++k) out[4 * h + k] = in[4 * t + k];
+}
+__global__ void k_map_home(int32_t* perm, const int32_t* home, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  perm[t] = home[perm[t]];
+}
+
+}  // namespace ntlp

@@ -1,0 +1,961 @@
+// ntlp_gpu.hip -- C ABI of the MI355X droplet step (see include/ntlp_gpu.h).
+//
+// One process-wide context on the current HIP device, mirroring the module
+// globals of particles.f90.  All work runs on the context's own stream
+// (default priority).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ntlp_gpu.h"
+#include "kernels.hpp"
+
+using namespace ntlp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return fail(NTLP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(expr)        \
+  do {                   \
+    int rc_ = (expr);    \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+struct Timed {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+constexpr int64_t kTallyRows = 4096;
+
+struct Ctx {
+  bool init = false;
+  hipStream_t stream = nullptr;
+  DevGrid g{};
+  DevConsts C{};
+  DevFlags fl{};
+  double* d_grid = nullptr;  // z | zz | dzw | dzu
+  // fields
+  double* ext[5] = {};
+  float* f32[3] = {};
+  double* field_stage = nullptr;
+  bool have_fields = false;
+  int64_t ext_elems = 0;
+  // droplets
+  int64_t n = 0, cap = 0;
+  double* hot[2] = {};
+  int64_t* mult[2] = {};
+  int32_t* home[2] = {};
+  int cur = 0;
+  bool home_identity = true;
+  double* cold = nullptr;
+  int32_t* pidx = nullptr;
+  int32_t* procidx = nullptr;
+  double* amounts = nullptr;
+  int32_t* keys = nullptr;
+  int32_t* perm = nullptr;
+  bool perm_valid = false;
+  double* col_tmp = nullptr;
+  // bins
+  int32_t* counts = nullptr;
+  int32_t* offsets = nullptr;
+  int32_t* bsum = nullptr;
+  int64_t ncell = 0, nbsum = 0;
+  // coupling sources
+  double* src = nullptr;
+  int64_t nsrc = 0;
+  // counters
+  unsigned long long* tally = nullptr;
+  int64_t step_row = 0;
+  // timing
+  bool timing = false;
+  std::vector<Timed> pending;
+  std::map<std::string, std::pair<int64_t, double>> totals;
+};
+
+Ctx ctx;
+
+int hot_of(int c) {
+  switch (c) {
+    case NTLP_X: return kHX;
+    case NTLP_Y: return kHY;
+    case NTLP_Z: return kHZ;
+    case NTLP_VX: return kHVX;
+    case NTLP_VY: return kHVY;
+    case NTLP_VZ: return kHVZ;
+    case NTLP_TP: return kHTP;
+    case NTLP_RADIUS: return kHR;
+    case NTLP_M_S: return kHMS;
+    case NTLP_KAPPA_S: return kHKAPPA;
+    case NTLP_RC: return kHRC;
+    case NTLP_RES: return kHRES;
+    case NTLP_ACTRES: return kHACTRES;
+    case NTLP_NUMACT: return kHNUMACT;
+    default: return -1;
+  }
+}
+
+int cold_of(int c) {
+  switch (c) {
+    case NTLP_UFX: return kCUFX;
+    case NTLP_UFY: return kCUFY;
+    case NTLP_UFZ: return kCUFZ;
+    case NTLP_XRHSX: return kCXRHSX;
+    case NTLP_XRHSY: return kCXRHSY;
+    case NTLP_XRHSZ: return kCXRHSZ;
+    case NTLP_VRHSX: return kCVRHSX;
+    case NTLP_VRHSY: return kCVRHSY;
+    case NTLP_VRHSZ: return kCVRHSZ;
+    case NTLP_TPRHS_S: return kCTPRHS_S;
+    case NTLP_TPRHS_L: return kCTPRHS_L;
+    case NTLP_TF: return kCTF;
+    case NTLP_RADRHS: return kCRADRHS;
+    case NTLP_QINF: return kCQINF;
+    case NTLP_QSTAR: return kCQSTAR;
+    case NTLP_VXOLD: return kCVXOLD;
+    case NTLP_VYOLD: return kCVYOLD;
+    case NTLP_VZOLD: return kCVZOLD;
+    case NTLP_TPOLD: return kCTPOLD;
+    case NTLP_RADIUSOLD: return kCRADIUSOLD;
+    default: return -1;
+  }
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+template <typename T>
+int dalloc(T*& p, int64_t count, const char* what) {
+  dfree(p);
+  if (count <= 0) count = 1;
+  hipError_t e = hipMalloc((void**)&p, sizeof(T) * (size_t)count);
+  if (e != hipSuccess) {
+    p = nullptr;
+    return fail(NTLP_E_NOMEM, std::string("hipMalloc failed for ") + what + " (" +
+                                  std::to_string(sizeof(T) * (size_t)count) + " bytes): " + hipGetErrorString(e));
+  }
+  return 0;
+}
+
+unsigned grid8(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  b = (b + 7) / 8 * 8;
+  return (unsigned)std::max<int64_t>(b, 8);
+}
+unsigned grid_flat(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  return (unsigned)std::min<int64_t>(std::max<int64_t>(b, 1), 1 << 20);
+}
+
+// Records HIP events around a launch when timing is on.
+template <typename F>
+int launch(const char* name, F&& f) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (ctx.timing) {
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, ctx.stream));
+  }
+  f();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(NTLP_E_HIP, std::string("launch of ") + name + ": " + hipGetErrorString(e));
+  if (ctx.timing) {
+    HIPCHK(hipEventRecord(b, ctx.stream));
+    ctx.pending.push_back({name, a, b});
+  }
+  return 0;
+}
+
+int resolve_timers() {
+  if (ctx.pending.empty()) return 0;
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  for (auto& t : ctx.pending) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
+    auto& e = ctx.totals[t.name];
+    e.first += 1;
+    e.second += ms;
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  ctx.pending.clear();
+  return 0;
+}
+
+int need_init() {
+  if (!ctx.init) return fail(NTLP_E_NOT_INIT, "ntlp_gpu_init has not been called");
+  return 0;
+}
+int need_fields() {
+  CHK(need_init());
+  if (!ctx.have_fields) return fail(NTLP_E_STATE, "no fields: call ntlp_gpu_set_fields first");
+  return 0;
+}
+int need_drops() {
+  CHK(need_fields());
+  if (ctx.n <= 0) return fail(NTLP_E_STATE, "no droplets: call ntlp_gpu_particles_put first");
+  return 0;
+}
+
+int fill_f64(double* p, int64_t n, double v) {
+  return launch("fill", [&] { k_fill_f64<<<grid_flat(n), kBlock, 0, ctx.stream>>>(p, n, v); });
+}
+int fill_i32(int32_t* p, int64_t n, int32_t v) {
+  return launch("fill", [&] { k_fill_i32<<<grid_flat(n), kBlock, 0, ctx.stream>>>(p, n, v); });
+}
+
+void release_all() {
+  for (auto& p : ctx.ext) dfree(p);
+  for (auto& p : ctx.f32) dfree(p);
+  dfree(ctx.field_stage);
+  for (int b = 0; b < 2; ++b) {
+    dfree(ctx.hot[b]);
+    dfree(ctx.mult[b]);
+    dfree(ctx.home[b]);
+  }
+  dfree(ctx.cold);
+  dfree(ctx.pidx);
+  dfree(ctx.procidx);
+  dfree(ctx.amounts);
+  dfree(ctx.keys);
+  dfree(ctx.perm);
+  dfree(ctx.col_tmp);
+  dfree(ctx.counts);
+  dfree(ctx.offsets);
+  dfree(ctx.bsum);
+  dfree(ctx.src);
+  dfree(ctx.tally);
+  dfree(ctx.d_grid);
+  for (auto& t : ctx.pending) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  ctx.pending.clear();
+  ctx.totals.clear();
+  if (ctx.stream) (void)hipStreamDestroy(ctx.stream);
+  ctx = Ctx();
+}
+
+DevFields fields_view() {
+  DevFields F;
+  F.u = ctx.ext[0];
+  F.v = ctx.ext[1];
+  F.w = ctx.ext[2];
+  F.T = ctx.ext[3];
+  F.q = ctx.ext[4];
+  F.u32 = ctx.f32[0];
+  F.v32 = ctx.f32[1];
+  F.w32 = ctx.f32[2];
+  F.sj = ctx.g.nnz + 2;
+  F.si = (int64_t)(ctx.g.nnz + 2) * (ctx.g.nny + 5);
+  return F;
+}
+
+StepArgs base_args() {
+  StepArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = ctx.g;
+  a.C = ctx.C;
+  a.fl = ctx.fl;
+  a.F = fields_view();
+  a.n = ctx.n;
+  a.cap = ctx.cap;
+  a.src = {ctx.hot[ctx.cur], ctx.mult[ctx.cur], ctx.home[ctx.cur]};
+  a.dst = {ctx.hot[1 - ctx.cur], ctx.mult[1 - ctx.cur], ctx.home[1 - ctx.cur]};
+  a.perm = nullptr;
+  a.cold = ctx.cold;
+  a.amounts = ctx.amounts;
+  a.keys = ctx.keys;
+  a.counts = ctx.counts;
+  a.tally = ctx.tally + (ctx.step_row % kTallyRows) * kTallyWidth;
+  return a;
+}
+
+// exclusive scan counts -> offsets (ncell + 1 entries)
+int scan_counts() {
+  const int64_t nb = (ctx.ncell + kScanTile - 1) / kScanTile;
+  CHK(launch("bin_scan", [&] { k_scan_reduce<<<(unsigned)nb, kScanThreads, 0, ctx.stream>>>(ctx.counts, ctx.ncell, ctx.bsum); }));
+  CHK(launch("bin_scan", [&] { k_scan_bsum<<<1, kScanThreads, 0, ctx.stream>>>(ctx.bsum, nb); }));
+  CHK(launch("bin_scan", [&] {
+    k_scan_down<<<(unsigned)nb, kScanThreads, 0, ctx.stream>>>(ctx.counts, ctx.ncell, ctx.bsum, ctx.offsets);
+  }));
+  return 0;
+}
+
+// keys + counts of the current slot order must be in place; leaves perm
+// sorted by u cell (ties by `tie`, or by slot) and counts zeroed.
+int finish_bin(const int32_t* tie) {
+  CHK(scan_counts());
+  CHK(launch("bin_scatter", [&] {
+    k_bin_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(ctx.keys, ctx.n, ctx.offsets, ctx.counts, ctx.perm);
+  }));
+  CHK(launch("bin_cellsort", [&] {
+    k_cell_sort<<<grid_flat(ctx.ncell), kBlock, 0, ctx.stream>>>(ctx.offsets, ctx.ncell, ctx.perm, tie);
+  }));
+  ctx.perm_valid = true;
+  return 0;
+}
+
+// bin the current slot order from the positions
+int bin_current(int32_t* cellidx_dev, const int32_t* tie) {
+  BinArgs b;
+  b.g = ctx.g;
+  b.hot = ctx.hot[ctx.cur];
+  b.n = ctx.n;
+  b.cap = ctx.cap;
+  b.keys = ctx.keys;
+  b.counts = ctx.counts;
+  b.cellidx = cellidx_dev;
+  CHK(launch("bin_keys", [&] { k_keys<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(b); }));
+  return finish_bin(tie);
+}
+
+unsigned src_mask() {
+  unsigned m = 0;
+  if (ctx.fl.icouple == 1) m |= 7u;
+  if (ctx.fl.iTcouple == 1) m |= 8u;
+  if (ctx.fl.iHcouple == 1) m |= 48u;
+  return m;
+}
+
+int scatter_atomic() {
+  CHK(fill_f64(ctx.src, 6 * ctx.nsrc, 0.0));
+  ScatterArgs s;
+  s.g = ctx.g;
+  s.hot = ctx.hot[ctx.cur];
+  s.amounts = ctx.amounts;
+  s.n = ctx.n;
+  s.cap = ctx.cap;
+  s.src = ctx.src;
+  s.nsrc = ctx.nsrc;
+  s.fmask = src_mask();
+  return launch("scatter", [&] { k_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(s); });
+}
+
+int scatter_pull() {
+  CHK(bin_current(nullptr, nullptr));
+  PullArgs p;
+  p.g = ctx.g;
+  p.hot = ctx.hot[ctx.cur];
+  p.amounts = ctx.amounts;
+  p.perm = ctx.perm;
+  p.offsets = ctx.offsets;
+  p.cap = ctx.cap;
+  p.src = ctx.src;
+  p.nsrc = ctx.nsrc;
+  return launch("scatter_pull", [&] { k_scatter_pull<<<grid8(ctx.nsrc), kBlock, 0, ctx.stream>>>(p); });
+}
+
+template <int MODE, int PREC, int ILIN>
+void launch_update_t(const StepArgs& a, unsigned grid) {
+  k_update<MODE, PREC, ILIN><<<grid, kBlock, 0, ctx.stream>>>(a);
+}
+template <int MODE, int PREC, int ILIN>
+void launch_step_t(const StepArgs& a, unsigned grid) {
+  k_step<MODE, PREC, ILIN><<<grid, kBlock, 0, ctx.stream>>>(a);
+}
+
+template <int MODE>
+int dispatch_update(const StepArgs& a, int prec) {
+  const unsigned gr = grid8(a.n);
+  const bool lin = ctx.fl.ilin == 1;
+  const char* name = MODE == 1 ? "update_rk3" : "update_be";
+  return launch(name, [&] {
+    if (prec == PREC_FP64) lin ? launch_update_t<MODE, 0, 1>(a, gr) : launch_update_t<MODE, 0, 0>(a, gr);
+    else if (prec == PREC_MIXED) lin ? launch_update_t<MODE, 1, 1>(a, gr) : launch_update_t<MODE, 1, 0>(a, gr);
+    else lin ? launch_update_t<MODE, 2, 1>(a, gr) : launch_update_t<MODE, 2, 0>(a, gr);
+  });
+}
+
+template <int MODE>
+int dispatch_step(const StepArgs& a, int prec) {
+  const unsigned gr = grid8(a.n);
+  const bool lin = ctx.fl.ilin == 1;
+  const char* name = MODE == 1 ? "step_rk3" : "step_be";
+  return launch(name, [&] {
+    if (prec == PREC_FP64) lin ? launch_step_t<MODE, 0, 1>(a, gr) : launch_step_t<MODE, 0, 0>(a, gr);
+    else if (prec == PREC_MIXED) lin ? launch_step_t<MODE, 1, 1>(a, gr) : launch_step_t<MODE, 1, 0>(a, gr);
+    else lin ? launch_step_t<MODE, 2, 1>(a, gr) : launch_step_t<MODE, 2, 0>(a, gr);
+  });
+}
+
+int bump_step_row() {
+  // count the step in its row and open the next row (cleared before use)
+  ctx.step_row++;
+  unsigned long long* row = ctx.tally + (ctx.step_row % kTallyRows) * kTallyWidth;
+  return launch("fill", [&] { k_fill_u64<<<1, kBlock, 0, ctx.stream>>>(row, kTallyWidth); });
+}
+
+}  // namespace
+
+// ===================================================================== ABI
+
+extern "C" {
+
+const char* ntlp_gpu_last_error(void) { return g_err.c_str(); }
+
+int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags* f) {
+  if (!grid || !c || !f) return fail(NTLP_E_INVALID, "null argument to ntlp_gpu_init");
+  if (grid->nnx < 2 || grid->nny < 2 || grid->nnz < 3) return fail(NTLP_E_INVALID, "grid too small (nnx,nny >= 2, nnz >= 3)");
+  if (grid->nnz + 2 > kMaxLevels) return fail(NTLP_E_RANGE, "nnz + 2 exceeds the LDS grid limit of 1024 levels");
+  if (!(grid->dx > 0) || !(grid->dy > 0)) return fail(NTLP_E_INVALID, "dx and dy must be positive");
+  if (!grid->z || !grid->zz || !grid->dzw || !grid->dzu) return fail(NTLP_E_INVALID, "null grid metadata");
+  if (f->icase != 0) return fail(NTLP_E_UNSUPPORTED, "only icase=0 (reflecting walls) is supported");
+  const int64_t cells_u = (int64_t)(grid->nnx + 1) * (grid->nny + 1) * (grid->nnz + 1);
+  if (cells_u >= (int64_t)INT32_MAX) return fail(NTLP_E_RANGE, "grid has too many cells for 32-bit cell keys");
+  release_all();
+  HIPCHK(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
+  const int nz = grid->nnz;
+  const int nlev = nz + 2, nd = nz + 3;
+  // monotone z and zz are required by the minloc semantics
+  for (int k = 1; k < nlev; ++k)
+    if (!(grid->z[k] > grid->z[k - 1]) || !(grid->zz[k] > grid->zz[k - 1]))
+      return fail(NTLP_E_INVALID, "z and zz must be strictly increasing");
+  CHK(dalloc(ctx.d_grid, 2 * nlev + 2 * nd, "grid"));
+  HIPCHK(hipMemcpy(ctx.d_grid, grid->z, sizeof(double) * nlev, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx.d_grid + nlev, grid->zz, sizeof(double) * nlev, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx.d_grid + 2 * nlev, grid->dzw, sizeof(double) * nd, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ctx.d_grid + 2 * nlev + nd, grid->dzu, sizeof(double) * nd, hipMemcpyHostToDevice));
+  DevGrid& g = ctx.g;
+  g.nnx = grid->nnx;
+  g.nny = grid->nny;
+  g.nnz = nz;
+  g.dx = grid->dx;
+  g.dy = grid->dy;
+  g.xl = grid->xl;
+  g.yl = grid->yl;
+  g.z = ctx.d_grid;
+  g.zz = ctx.d_grid + nlev;
+  g.dzw = ctx.d_grid + 2 * nlev;
+  g.dzu = ctx.d_grid + 2 * nlev + nd;
+  int steps = 0;
+  while ((1 << steps) <= nlev) ++steps;
+  g.nsteps = steps;
+  bool inter = true;
+  for (int k = 0; k < nlev; ++k) {
+    if (!(grid->zz[k] < grid->z[k])) inter = false;
+    if (k >= 1 && !(grid->z[k - 1] < grid->zz[k])) inter = false;
+  }
+  g.interleaved = inter ? 1 : 0;
+  DevConsts& C = ctx.C;
+  C.dt = c->dt;
+  C.nuf = c->nuf;
+  C.Cpa = c->Cpa;
+  C.Cpp = c->Cpp;
+  C.CpaCpp = c->CpaCpp;
+  C.Pra = c->Pra;
+  C.Sc = c->Sc;
+  C.Rd = c->Rd;
+  C.rhow = c->rhow;
+  C.rhos = c->rhos;
+  C.Mw = c->Mw;
+  C.Ru = c->Ru;
+  C.Gam = c->Gam;
+  C.Lv = c->Lv;
+  C.Cpv = c->Cpv;
+  C.surf_rho = c->surf_rho;
+  C.surf_p = c->surf_p;
+  C.tsfcc1 = c->tsfcc1;
+  C.grav = c->grav;
+  C.radius_init = c->radius_init;
+  C.pi2 = c->pi2;
+  for (int d = 0; d < 3; ++d) {
+    C.g[d] = c->part_grav[d];
+    C.zetas[d] = c->zetas[d];
+    C.gama[d] = c->gama[d];
+  }
+  C.Sc13 = std::pow(c->Sc, 1. / 3.);
+  C.Pra13 = std::pow(c->Pra, 1. / 3.);
+  DevFlags& fl = ctx.fl;
+  fl.ilin = f->ilin;
+  fl.icouple = f->icouple;
+  fl.iTcouple = f->iTcouple;
+  fl.iHcouple = f->iHcouple;
+  fl.ievap = f->ievap;
+  fl.iexner = f->iexner;
+  fl.icase = f->icase;
+  ctx.ncell = cells_u;
+  ctx.nbsum = (cells_u + kScanTile - 1) / kScanTile;
+  CHK(dalloc(ctx.counts, ctx.ncell, "counts"));
+  CHK(dalloc(ctx.offsets, ctx.ncell + 1, "offsets"));
+  CHK(dalloc(ctx.bsum, ctx.nbsum, "scan block sums"));
+  ctx.nsrc = (int64_t)(nz + 2) * (grid->nny + 1) * (grid->nnx + 1);
+  CHK(dalloc(ctx.src, 6 * ctx.nsrc, "coupling sources"));
+  CHK(dalloc(ctx.tally, kTallyRows * kTallyWidth, "counters"));
+  ctx.ext_elems = (int64_t)(nz + 2) * (grid->nny + 5) * (grid->nnx + 5);
+  ctx.init = true;
+  CHK(fill_i32(ctx.counts, ctx.ncell, 0));
+  CHK(fill_f64(ctx.src, 6 * ctx.nsrc, 0.0));
+  CHK(launch("fill", [&] { k_fill_u64<<<64, kBlock, 0, ctx.stream>>>(ctx.tally, kTallyRows * kTallyWidth); }));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+int ntlp_gpu_finalize(void) {
+  if (ctx.stream) (void)hipStreamSynchronize(ctx.stream);
+  release_all();
+  return NTLP_OK;
+}
+
+int ntlp_gpu_set_dt(double dt) {
+  CHK(need_init());
+  if (!(dt > 0)) return fail(NTLP_E_INVALID, "dt must be positive");
+  ctx.C.dt = dt;
+  return NTLP_OK;
+}
+
+int ntlp_gpu_set_fields(const double* u, const double* v, const double* w, const double* T, const double* q,
+                        int32_t layout) {
+  CHK(need_init());
+  const double* in[5] = {u, v, w, T, q};
+  for (auto p : in)
+    if (!p) return fail(NTLP_E_INVALID, "null field pointer");
+  const int nx = ctx.g.nnx, ny = ctx.g.nny, nz = ctx.g.nnz;
+  int64_t in_elems;
+  if (layout == NTLP_LAYOUT_EXT) in_elems = ctx.ext_elems;
+  else if (layout == NTLP_LAYOUT_CORE) in_elems = (int64_t)(nz + 2) * ny * nx;
+  else if (layout == NTLP_LAYOUT_XPENCIL) in_elems = (int64_t)(nx + 2) * ny * (nz + 2);
+  else return fail(NTLP_E_INVALID, "unknown field layout");
+  for (int f = 0; f < 5; ++f)
+    if (!ctx.ext[f]) CHK(dalloc(ctx.ext[f], ctx.ext_elems, "ext field"));
+  for (int f = 0; f < 3; ++f)
+    if (!ctx.f32[f]) CHK(dalloc(ctx.f32[f], ctx.ext_elems, "fp32 field copy"));
+  if (layout != NTLP_LAYOUT_EXT && !ctx.field_stage) CHK(dalloc(ctx.field_stage, in_elems, "field staging"));
+  for (int f = 0; f < 5; ++f) {
+    if (layout == NTLP_LAYOUT_EXT) {
+      HIPCHK(hipMemcpyAsync(ctx.ext[f], in[f], sizeof(double) * in_elems, hipMemcpyHostToDevice, ctx.stream));
+    } else {
+      HIPCHK(hipMemcpyAsync(ctx.field_stage, in[f], sizeof(double) * in_elems, hipMemcpyHostToDevice, ctx.stream));
+      double* dst = ctx.ext[f];
+      const double* stg = ctx.field_stage;
+      if (layout == NTLP_LAYOUT_CORE)
+        CHK(launch("fill_ext", [&] { k_fill_ext<<<grid_flat(ctx.ext_elems), kBlock, 0, ctx.stream>>>(stg, dst, nx, ny, nz); }));
+      else
+        CHK(launch("fill_ext", [&] {
+          k_xpencil_to_ext<<<grid_flat(ctx.ext_elems), kBlock, 0, ctx.stream>>>(stg, dst, nx, ny, nz);
+        }));
+    }
+    if (f < 3) {
+      double* s = ctx.ext[f];
+      float* o = ctx.f32[f];
+      CHK(launch("to_f32", [&] { k_to_f32<<<grid_flat(ctx.ext_elems), kBlock, 0, ctx.stream>>>(s, o, ctx.ext_elems); }));
+    }
+    HIPCHK(hipStreamSynchronize(ctx.stream));  // staging buffer reuse
+  }
+  dfree(ctx.field_stage);
+  ctx.have_fields = true;
+  return NTLP_OK;
+}
+
+int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t* pidx, const int32_t* procidx,
+                           int64_t n) {
+  CHK(need_init());
+  if (n < 0 || (n > 0 && (!soa || !mult))) return fail(NTLP_E_INVALID, "bad droplet arrays");
+  if (n >= (int64_t)INT32_MAX) return fail(NTLP_E_RANGE, "more than 2^31-1 droplets");
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  if (n > ctx.cap) {
+    for (int b = 0; b < 2; ++b) {
+      CHK(dalloc(ctx.hot[b], kNHot * n, "hot state"));
+      CHK(dalloc(ctx.mult[b], n, "mult"));
+      CHK(dalloc(ctx.home[b], n, "home"));
+    }
+    CHK(dalloc(ctx.cold, kNCold * n, "cold state"));
+    CHK(dalloc(ctx.pidx, n, "pidx"));
+    CHK(dalloc(ctx.procidx, n, "procidx"));
+    CHK(dalloc(ctx.amounts, 6 * n, "coupling amounts"));
+    CHK(dalloc(ctx.keys, n, "keys"));
+    CHK(dalloc(ctx.perm, n, "perm"));
+    CHK(dalloc(ctx.col_tmp, n, "column staging"));
+    ctx.cap = n;
+  }
+  ctx.n = n;
+  ctx.cur = 0;
+  ctx.home_identity = true;
+  ctx.perm_valid = false;
+  if (n == 0) return NTLP_OK;
+  for (int c = 0; c < NTLP_NREAL; ++c) {
+    double* dst = hot_of(c) >= 0 ? ctx.hot[0] + (int64_t)hot_of(c) * ctx.cap : ctx.cold + (int64_t)cold_of(c) * ctx.cap;
+    HIPCHK(hipMemcpyAsync(dst, soa + (int64_t)c * n, sizeof(double) * n, hipMemcpyHostToDevice, ctx.stream));
+  }
+  HIPCHK(hipMemcpyAsync(ctx.mult[0], mult, sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx.stream));
+  if (pidx) HIPCHK(hipMemcpyAsync(ctx.pidx, pidx, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx.stream));
+  else CHK(fill_i32(ctx.pidx, n, 0));
+  if (procidx) HIPCHK(hipMemcpyAsync(ctx.procidx, procidx, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx.stream));
+  else CHK(fill_i32(ctx.procidx, n, 0));
+  CHK(launch("iota", [&] { k_iota<<<grid_flat(n), kBlock, 0, ctx.stream>>>(ctx.home[0], n); }));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+int ntlp_gpu_particles_get(double* soa, int64_t* mult, int32_t* pidx, int32_t* procidx, int64_t n) {
+  CHK(need_init());
+  if (n != ctx.n) return fail(NTLP_E_INVALID, "n does not match the droplet count on the device");
+  if (n == 0) return NTLP_OK;
+  const int64_t cap = ctx.cap;
+  const int32_t* home = ctx.home[ctx.cur];
+  if (soa) {
+    for (int c = 0; c < NTLP_NREAL; ++c) {
+      if (hot_of(c) >= 0) {
+        const double* col = ctx.hot[ctx.cur] + (int64_t)hot_of(c) * cap;
+        if (!ctx.home_identity) {
+          double* tmp = ctx.col_tmp;
+          CHK(launch("get", [&] { k_scatter_col_f64<<<grid_flat(n), kBlock, 0, ctx.stream>>>(col, home, n, tmp); }));
+          col = tmp;
+        }
+        HIPCHK(hipMemcpyAsync(soa + (int64_t)c * n, col, sizeof(double) * n, hipMemcpyDeviceToHost, ctx.stream));
+        HIPCHK(hipStreamSynchronize(ctx.stream));  // col_tmp reuse
+      } else {
+        HIPCHK(hipMemcpyAsync(soa + (int64_t)c * n, ctx.cold + (int64_t)cold_of(c) * cap, sizeof(double) * n,
+                              hipMemcpyDeviceToHost, ctx.stream));
+      }
+    }
+  }
+  if (mult) {
+    const int64_t* col = ctx.mult[ctx.cur];
+    if (!ctx.home_identity) {
+      int64_t* tmp = (int64_t*)ctx.col_tmp;
+      CHK(launch("get", [&] { k_scatter_col_i64<<<grid_flat(n), kBlock, 0, ctx.stream>>>(col, home, n, tmp); }));
+      col = tmp;
+    }
+    HIPCHK(hipMemcpyAsync(mult, col, sizeof(int64_t) * n, hipMemcpyDeviceToHost, ctx.stream));
+    HIPCHK(hipStreamSynchronize(ctx.stream));
+  }
+  if (pidx) HIPCHK(hipMemcpyAsync(pidx, ctx.pidx, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx.stream));
+  if (procidx) HIPCHK(hipMemcpyAsync(procidx, ctx.procidx, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx.stream));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+int64_t ntlp_gpu_particles_count(void) { return ctx.init ? ctx.n : 0; }
+
+// 352-byte record of `type particle` in the ifort layout (particles.f90:76-85,
+// 1929-1950): 4 int32, 39 real(8), mult and two pointer slots.
+namespace {
+constexpr int kRecReals = 39;
+// record real index -> NTLP column (-1: dist, u_sub(3), sigm_s, not kept)
+const int kRecCol[kRecReals] = {
+    NTLP_VX, NTLP_VY, NTLP_VZ, NTLP_X, NTLP_Y, NTLP_Z, NTLP_UFX, NTLP_UFY, NTLP_UFZ,
+    NTLP_XRHSX, NTLP_XRHSY, NTLP_XRHSZ, NTLP_VRHSX, NTLP_VRHSY, NTLP_VRHSZ,
+    NTLP_TP, NTLP_TPRHS_S, NTLP_TPRHS_L, NTLP_TF, NTLP_RADIUS, NTLP_RADRHS, NTLP_QINF, NTLP_QSTAR,
+    -1 /*dist*/, NTLP_RES, NTLP_M_S, NTLP_KAPPA_S, NTLP_RC, NTLP_ACTRES, NTLP_NUMACT,
+    -1, -1, -1 /*u_sub*/, -1 /*sigm_s*/, NTLP_VXOLD, NTLP_VYOLD, NTLP_VZOLD, NTLP_TPOLD, NTLP_RADIUSOLD};
+}  // namespace
+
+int ntlp_gpu_particles_put_records(const void* records, int64_t n) {
+  if (n < 0 || (n > 0 && !records)) return fail(NTLP_E_INVALID, "bad record buffer");
+  std::vector<double> soa((size_t)n * NTLP_NREAL, 0.0);
+  std::vector<int64_t> mult((size_t)n);
+  std::vector<int32_t> pidx((size_t)n), procidx((size_t)n);
+  const unsigned char* base = (const unsigned char*)records;
+  for (int64_t i = 0; i < n; ++i) {
+    const unsigned char* r = base + i * NTLP_RECORD_BYTES;
+    int32_t ints[4];
+    double reals[kRecReals];
+    memcpy(ints, r, 16);
+    memcpy(reals, r + 16, 8 * kRecReals);
+    memcpy(&mult[i], r + 16 + 8 * kRecReals, 8);
+    pidx[i] = ints[0];
+    procidx[i] = ints[1];
+    for (int k = 0; k < kRecReals; ++k)
+      if (kRecCol[k] >= 0) soa[(size_t)kRecCol[k] * n + i] = reals[k];
+  }
+  return ntlp_gpu_particles_put(soa.data(), mult.data(), pidx.data(), procidx.data(), n);
+}
+
+int ntlp_gpu_particles_get_records(void* records, int64_t n) {
+  CHK(need_init());
+  if (n != ctx.n || (n > 0 && !records)) return fail(NTLP_E_INVALID, "bad record buffer");
+  std::vector<double> soa((size_t)n * NTLP_NREAL);
+  std::vector<int64_t> mult((size_t)n);
+  std::vector<int32_t> pidx((size_t)n), procidx((size_t)n);
+  CHK(ntlp_gpu_particles_get(soa.data(), mult.data(), pidx.data(), procidx.data(), n));
+  unsigned char* base = (unsigned char*)records;
+  for (int64_t i = 0; i < n; ++i) {
+    unsigned char* r = base + i * NTLP_RECORD_BYTES;
+    const int32_t ints[4] = {pidx[i], procidx[i], -1, -1};
+    double reals[kRecReals];
+    for (int k = 0; k < kRecReals; ++k) reals[k] = kRecCol[k] >= 0 ? soa[(size_t)kRecCol[k] * n + i] : 0.0;
+    memcpy(r, ints, 16);
+    memcpy(r + 16, reals, 8 * kRecReals);
+    memcpy(r + 16 + 8 * kRecReals, &mult[i], 8);
+    memset(r + 16 + 8 * kRecReals + 8, 0, 16);
+  }
+  return NTLP_OK;
+}
+
+int ntlp_gpu_crit_radius(void) {
+  CHK(need_init());
+  if (ctx.n <= 0) return fail(NTLP_E_STATE, "no droplets");
+  static double table[1024];
+  const double radstart = -8.0, radend = -3.0, dr = (radstart - radend) / 1024.0;
+  for (int i = 1; i <= 1024; ++i) table[i - 1] = std::pow(10.0, radstart - (double)(i - 1) * dr);
+  HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_radval), table, sizeof(table), 0, hipMemcpyHostToDevice, ctx.stream));
+  StepArgs a = base_args();
+  CHK(launch("crit_radius", [&] { k_crit_radius<<<grid_flat(ctx.n), kBlock, 0, ctx.stream>>>(a); }));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+int ntlp_gpu_interp(int32_t set_vp, int32_t set_tp) {
+  CHK(need_drops());
+  StepArgs a = base_args();
+  const bool lin = ctx.fl.ilin == 1;
+  CHK(launch("interp", [&] {
+    if (lin) k_interp<1><<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(a, set_vp, set_tp);
+    else k_interp<0><<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(a, set_vp, set_tp);
+  }));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+static int check_prec(int prec) {
+  if (prec != PREC_FP64 && prec != PREC_MIXED && prec != PREC_FP64_AJ) return fail(NTLP_E_INVALID, "unknown precision");
+  return 0;
+}
+
+int ntlp_gpu_particle_update_rk3_prec(int32_t istage, int32_t it, int32_t prec) {
+  CHK(need_drops());
+  CHK(check_prec(prec));
+  if (istage < 1 || istage > 3) return fail(NTLP_E_INVALID, "istage must be 1, 2 or 3");
+  StepArgs a = base_args();
+  a.istage = istage;
+  a.it = it;
+  CHK(dispatch_update<1>(a, prec));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+int ntlp_gpu_particle_update_rk3(int32_t istage, int32_t it) {
+  return ntlp_gpu_particle_update_rk3_prec(istage, it, PREC_FP64);
+}
+
+int ntlp_gpu_particle_update_be_prec(int32_t it, int32_t prec) {
+  CHK(need_drops());
+  CHK(check_prec(prec));
+  CHK(bump_step_row());
+  StepArgs a = base_args();
+  a.it = it;
+  CHK(dispatch_update<2>(a, prec));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+int ntlp_gpu_particle_update_be(int32_t it) { return ntlp_gpu_particle_update_be_prec(it, PREC_FP64); }
+
+int ntlp_gpu_particle_coupling_update(double* src_out, int32_t deterministic) {
+  CHK(need_drops());
+  StepArgs a = base_args();
+  CHK(launch("amounts", [&] { k_amounts<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(a); }));
+  if (deterministic) CHK(scatter_pull());
+  else CHK(scatter_atomic());
+  if (src_out)
+    HIPCHK(hipMemcpyAsync(src_out, ctx.src, sizeof(double) * 6 * ctx.nsrc, hipMemcpyDeviceToHost, ctx.stream));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+int ntlp_gpu_get_sources(double* src_out) {
+  CHK(need_init());
+  if (!src_out) return fail(NTLP_E_INVALID, "null output");
+  HIPCHK(hipMemcpyAsync(src_out, ctx.src, sizeof(double) * 6 * ctx.nsrc, hipMemcpyDeviceToHost, ctx.stream));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+int ntlp_gpu_bin(int32_t* cellidx, int32_t* counts, int32_t* offsets, int32_t* perm) {
+  CHK(need_init());
+  if (ctx.n <= 0) return fail(NTLP_E_STATE, "no droplets");
+  const int64_t n = ctx.n;
+  int32_t* ci_dev = nullptr;
+  int32_t* ci_put = nullptr;
+  if (cellidx) {
+    CHK(dalloc(ci_dev, 4 * n, "cell indices"));
+    CHK(dalloc(ci_put, 4 * n, "cell indices"));
+  }
+  int rc = bin_current(ci_dev, ctx.home[ctx.cur]);
+  if (rc == 0 && cellidx) {
+    const int32_t* home = ctx.home[ctx.cur];
+    rc = launch("bin_out", [&] { k_scatter_home_i32x4<<<grid_flat(n), kBlock, 0, ctx.stream>>>(ci_dev, home, n, ci_put); });
+    if (rc == 0 && hipMemcpyAsync(cellidx, ci_put, sizeof(int32_t) * 4 * n, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess)
+      rc = fail(NTLP_E_HIP, "copy of cell indices failed");
+  }
+  if (rc == 0 && offsets &&
+      hipMemcpyAsync(offsets, ctx.offsets, sizeof(int32_t) * (ctx.ncell + 1), hipMemcpyDeviceToHost, ctx.stream) != hipSuccess)
+    rc = fail(NTLP_E_HIP, "copy of offsets failed");
+  if (rc == 0 && counts) {
+    std::vector<int32_t> off((size_t)ctx.ncell + 1);
+    if (hipMemcpyAsync(off.data(), ctx.offsets, sizeof(int32_t) * (ctx.ncell + 1), hipMemcpyDeviceToHost, ctx.stream) != hipSuccess ||
+        hipStreamSynchronize(ctx.stream) != hipSuccess)
+      rc = fail(NTLP_E_HIP, "copy of offsets failed");
+    else
+      for (int64_t c = 0; c < ctx.ncell; ++c) counts[c] = off[c + 1] - off[c];
+  }
+  if (rc == 0 && perm) {
+    // slot indices -> put-order indices, on a copy (the slot permutation
+    // stays valid for the next fused step)
+    int32_t* pp = nullptr;
+    rc = dalloc(pp, n, "perm copy");
+    if (rc == 0 && hipMemcpyAsync(pp, ctx.perm, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, ctx.stream) != hipSuccess)
+      rc = fail(NTLP_E_HIP, "perm copy failed");
+    const int32_t* home = ctx.home[ctx.cur];
+    if (rc == 0) rc = launch("bin_out", [&] { k_map_home<<<grid_flat(n), kBlock, 0, ctx.stream>>>(pp, home, n); });
+    if (rc == 0 && hipMemcpyAsync(perm, pp, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess)
+      rc = fail(NTLP_E_HIP, "copy of perm failed");
+    if (hipStreamSynchronize(ctx.stream) != hipSuccess && rc == 0) rc = fail(NTLP_E_HIP, "sync failed");
+    dfree(pp);
+  }
+  if (hipStreamSynchronize(ctx.stream) != hipSuccess && rc == 0) rc = fail(NTLP_E_HIP, "sync failed");
+  dfree(ci_dev);
+  dfree(ci_put);
+  return rc;
+}
+
+int ntlp_gpu_wcell_counts(int32_t* num_array) {
+  CHK(need_init());
+  if (!num_array) return fail(NTLP_E_INVALID, "null output");
+  const int64_t nw = (int64_t)ctx.g.nnx * ctx.g.nny * ctx.g.nnz;
+  int32_t* d = nullptr;
+  CHK(dalloc(d, nw, "w-cell counts"));
+  int rc = fill_i32(d, nw, 0);
+  if (rc == 0 && ctx.n > 0) {
+    BinArgs b;
+    b.g = ctx.g;
+    b.hot = ctx.hot[ctx.cur];
+    b.n = ctx.n;
+    b.cap = ctx.cap;
+    b.keys = nullptr;
+    b.counts = nullptr;
+    b.cellidx = nullptr;
+    rc = launch("wcells", [&] { k_wcells<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(b, d); });
+  }
+  if (rc == 0 && hipMemcpyAsync(num_array, d, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess)
+    rc = fail(NTLP_E_HIP, "copy failed");
+  if (hipStreamSynchronize(ctx.stream) != hipSuccess && rc == 0) rc = fail(NTLP_E_HIP, "sync failed");
+  dfree(d);
+  return rc;
+}
+
+int64_t ntlp_gpu_num_ucells(void) { return ctx.init ? ctx.ncell : 0; }
+
+int ntlp_gpu_step(int32_t mode, int32_t prec, int32_t nsteps, int32_t it0) {
+  CHK(need_drops());
+  CHK(check_prec(prec));
+  if (mode != NTLP_MODE_RK3 && mode != NTLP_MODE_BE) return fail(NTLP_E_INVALID, "unknown mode");
+  if (nsteps < 1) return fail(NTLP_E_INVALID, "nsteps must be >= 1");
+  if (!ctx.perm_valid) {
+    CHK(launch("iota", [&] { k_iota<<<grid_flat(ctx.n), kBlock, 0, ctx.stream>>>(ctx.perm, ctx.n); }));
+    ctx.perm_valid = true;
+  }
+  for (int k = 0; k < nsteps; ++k) {
+    CHK(bump_step_row());
+    StepArgs a = base_args();
+    a.perm = ctx.perm;
+    a.it = it0 + k;
+    a.write_cold = (k == nsteps - 1) ? 1 : 0;
+    if (mode == NTLP_MODE_RK3) CHK(dispatch_step<1>(a, prec));
+    else CHK(dispatch_step<2>(a, prec));
+    ctx.cur = 1 - ctx.cur;
+    ctx.home_identity = false;
+    CHK(scatter_atomic());
+    CHK(finish_bin(nullptr));
+  }
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+int ntlp_gpu_counters(ntlp_counters* out, int32_t reset) {
+  CHK(need_init());
+  std::vector<unsigned long long> rows((size_t)kTallyRows * kTallyWidth);
+  HIPCHK(hipMemcpyAsync(rows.data(), ctx.tally, sizeof(unsigned long long) * rows.size(), hipMemcpyDeviceToHost, ctx.stream));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  if (out) {
+    memset(out, 0, sizeof(*out));
+    for (int64_t r = 0; r < kTallyRows; ++r) {
+      const unsigned long long* row = rows.data() + r * kTallyWidth;
+      out->num100 += row[kTNum100];
+      out->numimpos += row[kTNumImpos];
+      out->actnum += row[kTActnum];
+      out->denum += row[kTDenum];
+      out->gn_iters += row[kTGnIters];
+      out->lm_iters += row[kTLmIters];
+      out->solves += row[kTSolves];
+      out->steps += row[kTSteps];
+      for (int h = 0; h < NTLP_GN_HIST; ++h) out->gn_hist[h] += row[kTHist0 + h];
+    }
+  }
+  if (reset) {
+    CHK(launch("fill", [&] { k_fill_u64<<<64, kBlock, 0, ctx.stream>>>(ctx.tally, kTallyRows * kTallyWidth); }));
+    HIPCHK(hipStreamSynchronize(ctx.stream));
+    ctx.step_row = 0;
+  }
+  return NTLP_OK;
+}
+
+int ntlp_gpu_step_counters(int64_t* rows_out, int32_t max_rows) {
+  CHK(need_init());
+  if (!rows_out || max_rows <= 0) return fail(NTLP_E_INVALID, "bad output");
+  const int64_t nr = std::min<int64_t>(std::min<int64_t>(ctx.step_row, kTallyRows - 1), max_rows);
+  std::vector<unsigned long long> rows((size_t)kTallyRows * kTallyWidth);
+  HIPCHK(hipMemcpyAsync(rows.data(), ctx.tally, sizeof(unsigned long long) * rows.size(), hipMemcpyDeviceToHost, ctx.stream));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  // rows 1..step_row hold the steps in order (row 0 is the pre-step row)
+  for (int64_t r = 0; r < nr; ++r) {
+    const int64_t src = (ctx.step_row - nr + 1 + r) % kTallyRows;
+    for (int k = 0; k < kTallyWidth; ++k) rows_out[r * kTallyWidth + k] = (int64_t)rows[src * kTallyWidth + k];
+  }
+  return (int)nr;
+}
+
+int ntlp_gpu_timing(int32_t enable) {
+  CHK(need_init());
+  CHK(resolve_timers());
+  ctx.totals.clear();
+  ctx.timing = enable != 0;
+  return NTLP_OK;
+}
+
+int ntlp_gpu_kernel_times(char* names, int64_t* launches, double* total_ms, int32_t cap) {
+  CHK(need_init());
+  CHK(resolve_timers());
+  int k = 0;
+  for (auto& e : ctx.totals) {
+    if (k >= cap) break;
+    if (names) {
+      memset(names + 32 * k, 0, 32);
+      strncpy(names + 32 * k, e.first.c_str(), 31);
+    }
+    if (launches) launches[k] = e.second.first;
+    if (total_ms) total_ms[k] = e.second.second;
+    ++k;
+  }
+  return k;
+}
+
+int ntlp_gpu_sync(void) {
+  CHK(need_init());
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,666 @@
+// physics.hpp -- per-droplet physics of the droplet step on gfx950.
+//
+// FP64 ("reference") paths keep the operation order of the Fortran routines
+// they restate; loop invariants are hoisted (Sc**(1/3), Pr**(1/3), taup0,
+// mod_magnus(Tf), |u_f - v|), which changes no result.  Dead reference
+// computations whose values no caller reads are not formed: the velocity
+// residual of ie_vrt_nd and the backward evaluations of jacob_approx_2d.
+//
+// The analytic-Jacobian Newton (NTLP_PREC_MIXED, NTLP_PREC_FP64_AJ) solves
+// the same backward-Euler equations (App. of particle_update_BE,
+// particles.f90:4263-4327) with the exact 2x2 Jacobian from the chain rule,
+// the same damping, guards, tolerance and Levenberg-Marquardt fallback.
+#pragma once
+#include "device_common.hpp"
+
+namespace ntlp {
+
+enum { PREC_FP64 = 0, PREC_MIXED = 1, PREC_FP64_AJ = 2 };
+
+struct Drop {
+  double x, y, z, vx, vy, vz, Tp, r, m_s, kappa, rc, res, actres, numact;
+};
+
+// Values the reference stores in the droplet record besides the hot state.
+struct Diag {
+  double uf[3], Tf, qinf, qstar;
+  double vold[3], Tpold, rold;
+  double xrhs[3], vrhs[3], Tprhs_s, Tprhs_L, radrhs;
+};
+
+// Per-thread Newton bookkeeping (particles.f90:3035-3039, 3128, 3147-3174).
+struct Tally {
+  int num100, numimpos, actnum, denum;
+  int gn_iters, lm_iters, solves, hist;  // hist: GN iterations of this solve, 0 = LM used, -1 none
+};
+
+__device__ __forceinline__ double mod_magnus(double T) {  // les.F:10764-10774
+  return 610.94 * exp((17.6257 * (T - 273.15)) / (243.04 + (T - 273.15)));
+}
+__device__ __forceinline__ double func_p_base(const DevConsts& C, double z) {  // les.F:10787-10795
+  return C.surf_p * pow(1.0 - z * C.grav / C.Cpa / C.tsfcc1, C.Cpa / C.Rd);
+}
+__device__ __forceinline__ double exner(const DevConsts& C, double p) {  // les.F:10775-10784
+  return pow(C.surf_p / p, -C.Rd / C.Cpa);
+}
+
+// Air density and the potential-to-actual temperature conversion of Tf
+// (particles.f90:3064-3072).
+__device__ __forceinline__ double air_rho(const DevConsts& C, const DevFlags& fl, double z, double& Tf) {
+  if (fl.iexner == 1) {
+    double pb = func_p_base(C, z);
+    Tf = Tf * exner(C, pb);
+    return pb / C.Rd / Tf;
+  }
+  return C.surf_rho;
+}
+
+// particle_bcs_nonperiodic, icase=0 (2683-2812), then particle_bcs_periodic
+// (2814-2843).
+__device__ __forceinline__ void apply_bcs(const DevGrid& g, const double* s_z, Drop& d) {
+  const double top = s_z[g.nnz] - d.r;
+  const double bot = 0.0 + d.r;
+  if (d.z > top) {
+    d.z = top - (d.z - top);
+    d.vz = -d.vz;
+  } else if (d.z < bot) {
+    d.z = bot + (bot - d.z);
+    d.vz = -d.vz;
+  }
+  if (d.x > g.xl) d.x = d.x - g.xl;
+  else if (d.x < 0) d.x = g.xl + d.x;
+  if (d.y > g.yl) d.y = d.y - g.yl;
+  else if (d.y < 0) d.y = g.yl + d.y;
+}
+
+// ----------------------------------------------------------- Newton solve
+
+struct NewtonIn {
+  double r, Tp, Tf, qinf, m_s, kappa, rhoa, esa, taup0, h, diffnorm;
+};
+
+// ie_vrt_nd (particles.f90:4263-4327), reference operation order.
+__device__ __forceinline__ void resid_ref(const DevConsts& C, const NewtonIn& P, double tempr,
+                                          double tempt, double F[2]) {
+  const double rnext = tempr * P.r;
+  const double Tnext = tempt * P.Tp;
+  const double dnext = rnext * 2.;
+  const double VolP = (2. / 3.) * C.pi2 * (rnext * rnext * rnext);
+  const double rhop = (P.m_s + VolP * C.rhow) / VolP;
+  const double Rep = dnext * P.diffnorm / C.nuf;
+  const double taup = (rhop * (dnext * dnext)) / (18.0 * P.rhoa * C.nuf);
+  const double qstr = (C.Mw / (C.Ru * Tnext * P.rhoa)) * P.esa *
+                      exp(((C.Lv * C.Mw / C.Ru) * ((1. / P.Tf) - (1. / Tnext))) +
+                          ((2. * C.Mw * C.Gam) / (C.Ru * C.rhow * rnext * Tnext)) -
+                          ((P.kappa * P.m_s * C.rhow / C.rhos) / (VolP * rhop - P.m_s)));
+  const double sRe = sqrt(Rep);
+  const double Shp = 2. + 0.6 * sRe * C.Sc13;
+  double rprime = (1. / 9.) * (Shp / C.Sc) * (rhop / C.rhow) * (rnext / taup) * (P.qinf - qstr);
+  rprime = rprime * (P.taup0 / P.r);
+  const double Nup = 2. + 0.6 * sRe * C.Pra13;
+  double Tprime = -((1. / 3.) * (Nup / C.Pra) * C.CpaCpp * (rhop / C.rhow) * (1. / taup) * (Tnext - P.Tf)) +
+                  3. * C.Lv * (1. / (rnext * C.Cpp)) * rprime * (P.r / P.taup0);
+  Tprime = Tprime * (P.taup0 / P.Tp);
+  F[0] = rnext / P.r - 1.0 - P.h * rprime;
+  F[1] = Tnext / P.Tp - 1.0 - P.h * Tprime;
+}
+
+// jacob_approx_2d (4224-4252): forward differences with step 1e-12; f0 is the
+// residual at (tr, tt), which the reference evaluates as well.
+__device__ __forceinline__ void jacob_fd(const DevConsts& C, const NewtonIn& P, double tr, double tt,
+                                         const double f0[2], double J[2][2]) {
+  const double diff = 1e-12;
+  double fx[2];
+  resid_ref(C, P, tr + diff, tt, fx);
+  J[0][0] = (fx[0] - f0[0]) / diff;
+  J[1][0] = (fx[1] - f0[1]) / diff;
+  resid_ref(C, P, tr, tt + diff, fx);
+  J[0][1] = (fx[0] - f0[0]) / diff;
+  J[1][1] = (fx[1] - f0[1]) / diff;
+}
+
+// Residual and exact Jacobian of the same equations.  With rn = tr*r,
+// Tn = tt*Tp the reference's groupings reduce to
+//   rprime = A*Sh*(qinf - q*)/rn,          A = ρa ν taup0 / (2 Sc ρw r)
+//   Tprime = -B*Nu*(Tn - Tf)/rn^2 + Cc*rprime/rn,
+//            B = 3/2 (Cpa/Cpp) ρa ν taup0 / (Pr ρw Tp),  Cc = 3 Lv r / (Cpp Tp)
+//   q*     = Mw esa / (Ru ρa Tn) * exp(E),
+//   E      = Lv Mw/Ru (1/Tf - 1/Tn) + 2 Mw Γ/(Ru ρw rn Tn) - κ m_s/(ρs V(rn))
+// (using ρp V - m_s = ρw V), differentiated in rn and Tn.
+__device__ __forceinline__ void resid_jac(const DevConsts& C, const NewtonIn& P, double tr, double tt,
+                                          double F[2], double J[2][2]) {
+  const double rn = tr * P.r, Tn = tt * P.Tp;
+  const double irn = 1.0 / rn, iTn = 1.0 / Tn;
+  const double V = (2. / 3.) * C.pi2 * (rn * rn * rn);
+  const double curv = (2. * C.Mw * C.Gam) / (C.Ru * C.rhow) * irn * iTn;
+  const double solu = (P.kappa * P.m_s / C.rhos) / V;
+  const double LMR = C.Lv * C.Mw / C.Ru;
+  const double E = LMR * ((Tn - P.Tf) / (P.Tf * Tn)) + curv - solu;
+  const double qs = (C.Mw / (C.Ru * P.rhoa)) * P.esa * iTn * exp(E);
+  const double dE_dr = -curv * irn + 3.0 * solu * irn;
+  const double dE_dT = LMR * iTn * iTn - curv * iTn;
+  const double dq_dr = qs * dE_dr;
+  const double dq_dT = qs * (dE_dT - iTn);
+  const double sRe = sqrt(2.0 * rn * P.diffnorm / C.nuf);
+  const double Sh = 2. + 0.6 * sRe * C.Sc13;
+  const double Nu = 2. + 0.6 * sRe * C.Pra13;
+  const double dSh_dr = 0.3 * sRe * C.Sc13 * irn;
+  const double dNu_dr = 0.3 * sRe * C.Pra13 * irn;
+  const double A = P.rhoa * C.nuf * P.taup0 / (2.0 * C.Sc * C.rhow * P.r);
+  const double B = 1.5 * C.CpaCpp * P.rhoa * C.nuf * P.taup0 / (C.Pra * C.rhow * P.Tp);
+  const double Cc = 3.0 * C.Lv * P.r / (C.Cpp * P.Tp);
+  const double dq = P.qinf - qs;
+  const double rp = A * Sh * dq * irn;
+  const double drp_dr = A * (dSh_dr * dq * irn - Sh * dq_dr * irn - Sh * dq * irn * irn);
+  const double drp_dT = -A * Sh * dq_dT * irn;
+  const double dT = Tn - P.Tf;
+  const double irn2 = irn * irn;
+  const double Tpr = -B * Nu * dT * irn2 + Cc * rp * irn;
+  const double dTp_dr = -B * (dNu_dr * dT * irn2 - 2.0 * Nu * dT * irn2 * irn) + Cc * (drp_dr * irn - rp * irn2);
+  const double dTp_dT = -B * Nu * irn2 + Cc * drp_dT * irn;
+  F[0] = tr - 1.0 - P.h * rp;
+  F[1] = tt - 1.0 - P.h * Tpr;
+  J[0][0] = 1.0 - P.h * P.r * drp_dr;
+  J[0][1] = -P.h * P.Tp * drp_dT;
+  J[1][0] = -P.h * P.r * dTp_dr;
+  J[1][1] = 1.0 - P.h * P.Tp * dTp_dT;
+}
+
+__device__ __forceinline__ void resid_fast(const DevConsts& C, const NewtonIn& P, double tr, double tt,
+                                           double F[2]) {
+  double J[2][2];
+  resid_jac(C, P, tr, tt, F, J);
+}
+
+// gauss_newton_2d (4071-4142).  AJ selects the analytic Jacobian.
+template <bool AJ>
+__device__ __noinline__ int gauss_newton(const DevConsts& C, const NewtonIn& P, const double vec1[2],
+                                         double vec2[2], int& iters) {
+  int iterations = 0, flag = 0;
+  const double error = 1.0e-8;
+  double v1[2] = {vec1[0], vec1[1]};
+  double correct[2] = {0.0, 0.0};
+  vec2[0] = v1[0];
+  vec2[1] = v1[1];
+  while (iterations < 50) {
+    iterations++;
+    double fv1[2], J[2][2];
+    if (AJ) {
+      resid_jac(C, P, v1[0], v1[1], fv1, J);
+    } else {
+      resid_ref(C, P, v1[0], v1[1], fv1);
+      jacob_fd(C, P, v1[0], v1[1], fv1, J);
+    }
+    double fJ[2][2];
+    if (AJ) {
+      // (J^T J)^-1 J^T = J^-1 for the square system; det(J^T J) = det(J)^2.
+      const double detJ = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      if (fabs(detJ * detJ) < 1.0e-10) { flag = 1; break; }
+      const double s = 1.0 / detJ;
+      fJ[0][0] = s * J[1][1];
+      fJ[0][1] = -s * J[0][1];
+      fJ[1][0] = -s * J[1][0];
+      fJ[1][1] = s * J[0][0];
+    } else {
+      double fancy[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) fancy[i][j] = J[0][i] * J[0][j] + J[1][i] * J[1][j];
+      const double det = fancy[0][0] * fancy[1][1] - fancy[0][1] * fancy[1][0];
+      if (fabs(det) < 1.0e-10) { flag = 1; break; }
+      const double s = 1. / det;
+      const double inv00 = s * fancy[1][1], inv10 = s * (-fancy[1][0]);
+      const double inv01 = s * (-fancy[0][1]), inv11 = s * fancy[0][0];
+      fJ[0][0] = inv00 * J[0][0] + inv01 * J[0][1];
+      fJ[0][1] = inv00 * J[1][0] + inv01 * J[1][1];
+      fJ[1][0] = inv10 * J[0][0] + inv11 * J[0][1];
+      fJ[1][1] = inv10 * J[1][0] + inv11 * J[1][1];
+    }
+    correct[0] = fJ[0][0] * fv1[0] + fJ[0][1] * fv1[1];
+    correct[1] = fJ[1][0] * fv1[0] + fJ[1][1] * fv1[1];
+    vec2[0] = v1[0] - correct[0];
+    vec2[1] = v1[1] - correct[1];
+    int counts = 0;
+    double relax = 1.0;
+    while ((vec2[0] < 0) || (vec2[1] < 0) || isnan(vec2[0])) {
+      counts++;
+      relax = relax * 0.5;
+      vec2[0] = v1[0] - correct[0] * relax;
+      vec2[1] = v1[1] - correct[1] * relax;
+      if (counts > 10) break;
+    }
+    if (vec2[0] < 0.01 || vec2[1] < 0.01 || isnan(vec2[0]) || isnan(vec2[1])) { flag = 1; break; }
+    if (sqrt(correct[0] * correct[0] + correct[1] * correct[1]) < error) break;
+    v1[0] = vec2[0];
+    v1[1] = vec2[1];
+  }
+  if (iterations == 50) flag = 1;
+  if (isnan(vec2[0]) || vec2[0] < 0 || isnan(vec2[1]) || vec2[1] < 0) flag = 1;
+  iters = iterations;
+  return flag;
+}
+
+// LV_solver (4143-4223).  vec2 keeps what the failed Gauss-Newton left, as the
+// shared actual argument rt_zeroes does in the reference.
+template <bool AJ>
+__device__ __noinline__ int lv_solver(const DevConsts& C, const NewtonIn& P, const double vec1[2],
+                                      double vec2[2], int& iters) {
+  int iterations = 0, flag = 0;
+  const double error = 1.0e-8;
+  double v1[2] = {vec1[0], vec1[1]};
+  double lambda = 0.001;
+  const double lup = 2.0, ldown = 2.0;
+  while (iterations < 1000) {
+    iterations++;
+    double J[2][2], fv1[2];
+    if (AJ) {
+      resid_jac(C, P, v1[0], v1[1], fv1, J);
+    } else {
+      resid_ref(C, P, v1[0], v1[1], fv1);
+      jacob_fd(C, P, v1[0], v1[1], fv1, J);
+    }
+    double gm[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) gm[i][j] = (J[0][i] * J[0][j] + J[1][i] * J[1][j]) + lambda * (i == j ? 1.0 : 0.0);
+    const double gradC0 = J[0][0] * fv1[0] + J[1][0] * fv1[1];
+    const double gradC1 = J[0][1] * fv1[0] + J[1][1] * fv1[1];
+    const double C0 = 0.5 * fv1[0] * fv1[0], C1 = 0.5 * fv1[1] * fv1[1];
+    const double det = gm[0][0] * gm[1][1] - gm[0][1] * gm[1][0];
+    if (fabs(det) < 1.0e-10) { flag = 1; break; }
+    const double s = 1. / det;
+    const double i00 = s * gm[1][1], i10 = s * (-gm[1][0]), i01 = s * (-gm[0][1]), i11 = s * gm[0][0];
+    const double c0 = i00 * gradC0 + i01 * gradC1;
+    const double c1 = i10 * gradC0 + i11 * gradC1;
+    vec2[0] = v1[0] - c0;
+    vec2[1] = v1[1] - c1;
+    if (sqrt(c0 * c0 + c1 * c1) < error) break;
+    if (vec2[0] < 0.0 || vec2[1] < 0.0 || isnan(vec2[0]) || isnan(vec2[1])) { flag = 1; break; }
+    double fv2[2];
+    if (AJ) resid_fast(C, P, vec2[0], vec2[1], fv2);
+    else resid_ref(C, P, vec2[0], vec2[1], fv2);
+    const double n0 = 0.5 * fv2[0] * fv2[0], n1 = 0.5 * fv2[1] * fv2[1];
+    if (sqrt(n0 * n0 + n1 * n1) < sqrt(C0 * C0 + C1 * C1)) {
+      v1[0] = vec2[0];
+      v1[1] = vec2[1];
+      lambda = lambda / ldown;
+    } else {
+      lambda = lambda * lup;
+    }
+  }
+  if (iterations == 1000) flag = 1;
+  if (vec2[0] < 0 || vec2[1] < 0) flag = 1;
+  iters = iterations;
+  return flag;
+}
+
+// rad_solver2 (4328-4371): Koehler equilibrium radius, Cardano on
+// r^3 + a r^2 + c = 0.
+__device__ __forceinline__ double rad_solver2(const DevConsts& C, const NewtonIn& P, double esa,
+                                              int& mflag) {
+  mflag = 0;
+  const double Tf = P.Tf;
+  const double L = log((C.Ru * Tf * P.rhoa * P.qinf) / (C.Mw * esa));
+  const double a = -((2 * C.Mw * C.Gam) / (C.Ru * C.rhow * Tf) / L);
+  const double c = (P.kappa * P.m_s) / ((2.0 / 3.0) * C.pi2 * C.rhos) / L;
+  const double Q = (a * a) / 9.0;
+  const double R = (2.0 * (a * a * a) + 27.0 * c) / 54.0;
+  const double Q3 = Q * Q * Q;
+  const double M = R * R - Q3;
+  double guess;
+  if (M < 0) {
+    const double theta = acos(R / sqrt(Q3));
+    guess = -(2 * sqrt(Q) * cos((theta - C.pi2) / 3.0)) - a / 3.0;
+    if (guess < 0) guess = -(2 * sqrt(Q) * cos((theta + C.pi2) / 3.0)) - a / 3.0;
+  } else {
+    const double S = -(R / fabs(R)) * cbrt(fabs(R) + sqrt(M));
+    const double T = Q / S;
+    guess = S + T - a / 3.0;
+    if (guess < 0) {
+      guess = P.r;
+      mflag = 1;
+    }
+  }
+  return guess;
+}
+
+// Radius/temperature block of particle_update_BE (3103-3178) on a droplet
+// whose velocity is already the new one.  Updates d.r, d.Tp, activation
+// bookkeeping and the tally.  AJ: analytic Jacobian.  WARM (mixed precision
+// only): start from whichever of the Koehler guess and the unchanged state
+// (1, Tf/Tp) has the smaller residual.
+template <bool AJ, bool WARM>
+__device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFlags& fl, Drop& d,
+                                                double Tf, double qinf, double rhoa, double ufx,
+                                                double ufy, double ufz, double Rep, double rhop,
+                                                double taup_i, double esa, Tally& tl) {
+  const double dt = C.dt;
+  const double ri = C.radius_init;
+  if (fl.ievap == 1 && qinf > 0.0) {
+    NewtonIn P;
+    P.r = d.r;
+    P.Tp = d.Tp;
+    P.Tf = Tf;
+    P.qinf = qinf;
+    P.m_s = d.m_s;
+    P.kappa = d.kappa;
+    P.rhoa = rhoa;
+    P.esa = esa;
+    P.taup0 = (((d.m_s) / ((2. / 3.) * C.pi2 * (ri * ri * ri)) + C.rhow) * ((ri * 2) * (ri * 2))) /
+              (18 * rhoa * C.nuf);
+    P.h = dt / P.taup0;
+    {
+      const double a0 = ufx - d.vx, a1 = ufy - d.vy, a2 = ufz - d.vz;
+      P.diffnorm = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    }
+    int mflag;
+    const double guess = rad_solver2(C, P, esa, mflag);
+    double rt_start[2], rt_zeroes[2];
+    if (mflag == 0) {
+      rt_start[0] = guess / d.r;
+      rt_start[1] = Tf / d.Tp;
+    } else {
+      rt_start[0] = 1.0;
+      rt_start[1] = 1.0;
+    }
+    if (WARM) {
+      const double alt[2] = {1.0, Tf / d.Tp};
+      double fa[2], fb[2];
+      resid_fast(C, P, rt_start[0], rt_start[1], fa);
+      resid_fast(C, P, alt[0], alt[1], fb);
+      const double na = fa[0] * fa[0] + fa[1] * fa[1], nb = fb[0] * fb[0] + fb[1] * fb[1];
+      if (!(na <= nb)) {
+        rt_start[0] = alt[0];
+        rt_start[1] = alt[1];
+      }
+    }
+    int it1 = 0, it2 = 0;
+    int flag = gauss_newton<AJ>(C, P, rt_start, rt_zeroes, it1);
+    tl.gn_iters += it1;
+    tl.solves += 1;
+    tl.hist = it1;
+    if (flag == 1) {
+      tl.num100 += 1;
+      tl.hist = 0;
+      flag = lv_solver<AJ>(C, P, rt_start, rt_zeroes, it2);
+      tl.lm_iters += it2;
+    }
+    if ((flag == 1) || isnan(rt_zeroes[0]) || (rt_zeroes[0] * d.r < 0) || isnan(rt_zeroes[1]) ||
+        (rt_zeroes[1] < 0) || (rt_zeroes[0] * d.r > 1.0e-2)) {
+      tl.numimpos += 1;
+      rt_zeroes[0] = 1.0;
+      rt_zeroes[1] = Tf / d.Tp;
+    }
+    if (d.r > d.rc && d.r * rt_zeroes[0] < d.rc) {
+      tl.denum += 1;
+    } else if (d.r < d.rc && d.r * rt_zeroes[0] > d.rc) {
+      tl.actnum += 1;
+      d.numact = d.numact + 1.0;
+      d.actres = 0.0;
+    }
+    d.r = rt_zeroes[0] * d.r;
+    d.Tp = rt_zeroes[1] * d.Tp;
+  } else {
+    const double Nup = 2.0 + 0.6 * sqrt(Rep) * C.Pra13;
+    const double tmp_coeff = Nup / 3.0 / C.Pra * C.CpaCpp * rhop / C.rhow * taup_i;
+    d.Tp = (d.Tp + tmp_coeff * dt * Tf) / (1 + dt * tmp_coeff);
+  }
+}
+
+// qstar of the updated droplet (particles.f90:3192-3201), FP64.
+__device__ __forceinline__ double qstar_f64(const DevConsts& C, const Drop& d, double Tf, double rhoa,
+                                            double einf) {
+  const double r = d.r;
+  const double Volp = C.pi2 * 2.0 / 3.0 * (r * r * r);
+  const double rhop = (d.m_s + Volp * C.rhow) / Volp;
+  return (C.Mw / (C.Ru * d.Tp * rhoa)) * einf *
+         exp(((C.Lv * C.Mw / C.Ru) * ((1. / Tf) - (1. / d.Tp))) + ((2. * C.Mw * C.Gam) / (C.Ru * C.rhow * r * d.Tp)) -
+             ((d.kappa * d.m_s * C.rhow / C.rhos) / (Volp * rhop - d.m_s)));
+}
+
+// The same diagnostic in FP32 (mixed precision): 1/Tf - 1/Tp formed as
+// (Tp - Tf)/(Tf Tp) from the FP64 difference.
+__device__ __forceinline__ double qstar_f32(const DevConsts& C, const Drop& d, double Tf, double rhoa,
+                                            double einf) {
+  const float r = (float)d.r, Tp = (float)d.Tp;
+  const float V = (float)(C.pi2 * 2.0 / 3.0) * r * r * r;
+  const float E = (float)(C.Lv * C.Mw / C.Ru) * ((float)(d.Tp - Tf) / ((float)Tf * Tp)) +
+                  (float)((2. * C.Mw * C.Gam) / (C.Ru * C.rhow)) / (r * Tp) -
+                  (float)(d.kappa * d.m_s / C.rhos) / V;
+  return (double)((float)(C.Mw / C.Ru) / (Tp * (float)rhoa) * (float)einf * __expf(E));
+}
+
+// ---------------------------------------------------------- BE droplet
+
+// One droplet of particle_update_BE (3043-3206) plus its boundary
+// conditions; PREC chooses the arithmetic (see the header comment).
+template <int PREC, int ILIN>
+__device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
+                                           const DevFields& F, const double* s_z, const double* s_zz,
+                                           Drop& d, int it, Diag& o, Tally& tl) {
+  const double dt = C.dt;
+  o.vold[0] = d.vx;
+  o.vold[1] = d.vy;
+  o.vold[2] = d.vz;
+  o.Tpold = d.Tp;
+  o.rold = d.r;
+  const Cell c = cell_of(g, s_z, s_zz, d.x, d.y, d.z);
+  Fluid f;
+  if (ILIN != 1) f = gather6_f64(g, F, s_z, s_zz, d.x, d.y, d.z, c);
+  else if (PREC == PREC_MIXED) f = gather_lin_mixed<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
+  else f = gather_lin_f64<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
+  if (it <= 1) {
+    d.vx = f.u[0];
+    d.vy = f.u[1];
+    d.vz = f.u[2];
+  }
+  double Tf = f.T;
+  const double rhoa = air_rho(C, fl, d.z, Tf);
+  const double r = d.r;
+  double Rep, rhop, taup_i;
+  if (PREC == PREC_MIXED) {
+    // drag and the implicit velocity update in FP32, position in FP64
+    const float rf = (float)r;
+    const float a0 = (float)(d.vx - f.u[0]), a1 = (float)(d.vy - f.u[1]), a2 = (float)(d.vz - f.u[2]);
+    const float Volp = (float)(C.pi2 * 2.0 / 3.0) * rf * rf * rf;
+    const float rhopf = ((float)d.m_s + Volp * (float)C.rhow) / Volp;
+    const float tpi = (float)(18.0 * rhoa * C.nuf) / rhopf / ((2.0f * rf) * (2.0f * rf));
+    Rep = (double)(2.0f * rf * sqrtf(a0 * a0 + a1 * a1 + a2 * a2) / (float)C.nuf);
+    rhop = rhopf;
+    taup_i = tpi;
+    d.x = d.x + dt * d.vx;
+    d.y = d.y + dt * d.vy;
+    d.z = d.z + dt * d.vz;
+    const float dtf = (float)dt, den = 1.0f + dtf * tpi;
+    d.vx = (double)(((float)d.vx + tpi * dtf * (float)f.u[0] + dtf * (float)C.g[0]) / den);
+    d.vy = (double)(((float)d.vy + tpi * dtf * (float)f.u[1] + dtf * (float)C.g[1]) / den);
+    d.vz = (double)(((float)d.vz + tpi * dtf * (float)f.u[2] + dtf * (float)C.g[2]) / den);
+  } else {
+    const double a0 = d.vx - f.u[0], a1 = d.vy - f.u[1], a2 = d.vz - f.u[2];
+    const double diffnorm = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    const double Volp = C.pi2 * 2.0 / 3.0 * (r * r * r);
+    rhop = (d.m_s + Volp * C.rhow) / Volp;
+    taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
+    Rep = 2.0 * r * diffnorm / C.nuf;
+    d.x = d.x + dt * d.vx;
+    d.y = d.y + dt * d.vy;
+    d.z = d.z + dt * d.vz;
+    const double den = (1 + dt * 1.0 * taup_i);
+    d.vx = (d.vx + taup_i * dt * 1.0 * f.u[0] + dt * C.g[0]) / den;
+    d.vy = (d.vy + taup_i * dt * 1.0 * f.u[1] + dt * C.g[1]) / den;
+    d.vz = (d.vz + taup_i * dt * 1.0 * f.u[2] + dt * C.g[2]) / den;
+  }
+  const double esa = mod_magnus(Tf);
+  if (PREC == PREC_FP64)
+    be_microphysics<false, false>(C, fl, d, Tf, f.q, rhoa, f.u[0], f.u[1], f.u[2], Rep, rhop, taup_i, esa, tl);
+  else if (PREC == PREC_FP64_AJ)
+    be_microphysics<true, false>(C, fl, d, Tf, f.q, rhoa, f.u[0], f.u[1], f.u[2], Rep, rhop, taup_i, esa, tl);
+  else
+    be_microphysics<true, true>(C, fl, d, Tf, f.q, rhoa, f.u[0], f.u[1], f.u[2], Rep, rhop, taup_i, esa, tl);
+  o.qstar = (PREC == PREC_MIXED) ? qstar_f32(C, d, Tf, rhoa, esa) : qstar_f64(C, d, Tf, rhoa, esa);
+  d.res = d.res + dt;
+  d.actres = d.actres + dt;
+  o.uf[0] = f.u[0];
+  o.uf[1] = f.u[1];
+  o.uf[2] = f.u[2];
+  o.Tf = Tf;
+  o.qinf = f.q;
+  apply_bcs(g, s_z, d);
+}
+
+// ---------------------------------------------------------- RK3 droplet
+
+// One droplet of one stage of particle_update_rk3 (2879-2973) plus boundary
+// conditions.  The stage reads the stored right-hand sides (o.xrhs ...) of
+// the previous stage.  MIXED: drag, Reynolds/Nusselt/Sherwood numbers, q* and
+// the explicit right-hand sides in FP32; positions, velocities, temperature
+// and radius accumulate in FP64.
+template <int PREC, int ILIN>
+__device__ __forceinline__ void rk3_droplet(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
+                                            const DevFields& F, const double* s_z, const double* s_zz,
+                                            Drop& d, int istage, int it, Diag& o) {
+  const double dt = C.dt;
+  const double zs = C.zetas[istage - 1], gs = C.gama[istage - 1];
+  if (istage == 1) {
+    o.vold[0] = d.vx;
+    o.vold[1] = d.vy;
+    o.vold[2] = d.vz;
+    o.Tpold = d.Tp;
+    o.rold = d.r;
+  }
+  const Cell c = cell_of(g, s_z, s_zz, d.x, d.y, d.z);
+  Fluid f;
+  if (ILIN != 1) f = gather6_f64(g, F, s_z, s_zz, d.x, d.y, d.z, c);
+  else if (PREC == PREC_MIXED) f = gather_lin_mixed<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
+  else f = gather_lin_f64<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
+  double Tf = f.T;
+  const double rhoa = air_rho(C, fl, d.z, Tf);
+  if (it <= 1) {
+    d.vx = f.u[0];
+    d.vy = f.u[1];
+    d.vz = f.u[2];
+    d.Tp = Tf;
+  }
+  const double r = d.r;
+  double xtmp[3], vtmp[3];
+  xtmp[0] = d.x + dt * zs * o.xrhs[0];
+  xtmp[1] = d.y + dt * zs * o.xrhs[1];
+  xtmp[2] = d.z + dt * zs * o.xrhs[2];
+  vtmp[0] = d.vx + dt * zs * o.vrhs[0];
+  vtmp[1] = d.vy + dt * zs * o.vrhs[1];
+  vtmp[2] = d.vz + dt * zs * o.vrhs[2];
+  double Tptmp = d.Tp + dt * zs * o.Tprhs_s;
+  Tptmp = Tptmp + dt * zs * o.Tprhs_L;
+  const double radiustmp = r + dt * zs * o.radrhs;
+  o.xrhs[0] = d.vx;
+  o.xrhs[1] = d.vy;
+  o.xrhs[2] = d.vz;
+  if (PREC == PREC_MIXED) {
+    const float rf = (float)r;
+    const float a0 = (float)(d.vx - f.u[0]), a1 = (float)(d.vy - f.u[1]), a2 = (float)(d.vz - f.u[2]);
+    const float Rep = 2.0f * rf * sqrtf(a0 * a0 + a1 * a1 + a2 * a2) / (float)C.nuf;
+    const float Volp = (float)(C.pi2 * 2.0 / 3.0) * rf * rf * rf;
+    const float rhop = ((float)d.m_s + Volp * (float)C.rhow) / Volp;
+    const float taup_i = (float)(18.0 * rhoa * C.nuf) / rhop / ((2.0f * rf) * (2.0f * rf));
+    const float corrfac = 1.0f + 0.15f * __powf(Rep, 0.687f);
+    const float sRe = sqrtf(Rep);
+    const float Nup = 2.0f + 0.6f * sRe * (float)C.Pra13;
+    const float Shp = 2.0f + 0.6f * sRe * (float)C.Sc13;
+    const float einf = (float)mod_magnus(Tf);
+    const float Tp = (float)d.Tp;
+    const float E = (float)(C.Mw * C.Lv / C.Ru) * ((float)(d.Tp - Tf) / ((float)Tf * Tp)) +
+                    (float)(2.0 * C.Mw * C.Gam / (C.Ru * C.rhow)) / (rf * Tp) -
+                    (float)(d.kappa * d.m_s / C.rhos) / Volp;
+    const float qstar = (float)(C.Mw / C.Ru) * einf * __expf(E) / Tp / (float)rhoa;
+    o.qstar = qstar;
+    const float ct = corrfac * taup_i;
+    o.vrhs[0] = (double)(ct * (float)(f.u[0] - d.vx) + (float)C.g[0]);
+    o.vrhs[1] = (double)(ct * (float)(f.u[1] - d.vy) + (float)C.g[1]);
+    o.vrhs[2] = (double)(ct * (float)(f.u[2] - d.vz) + (float)C.g[2]);
+    const float radrhs = (fl.ievap == 1)
+                             ? Shp / 9.0f / (float)C.Sc * rhop / (float)C.rhow * rf * taup_i * (float)(f.q - (double)qstar)
+                             : 0.0f;
+    o.radrhs = radrhs;
+    o.Tprhs_s = (double)(-(Nup / 3.0f / (float)C.Pra * (float)C.CpaCpp * rhop / (float)C.rhow * taup_i * (float)(d.Tp - Tf)));
+    o.Tprhs_L = (double)((float)(3.0 * C.Lv / C.Cpp) / rf * radrhs);
+  } else {
+    const double a0 = d.vx - f.u[0], a1 = d.vy - f.u[1], a2 = d.vz - f.u[2];
+    const double diffnorm = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    const double Rep = 2.0 * r * diffnorm / C.nuf;
+    const double Volp = C.pi2 * 2.0 / 3.0 * (r * r * r);
+    const double rhop = (d.m_s + Volp * C.rhow) / Volp;
+    const double taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
+    const double corrfac = (1.0 + 0.15 * pow(Rep, 0.687));
+    const double sRe = sqrt(Rep);
+    const double Nup = 2.0 + 0.6 * sRe * C.Pra13;
+    const double Shp = 2.0 + 0.6 * sRe * C.Sc13;
+    const double einf = mod_magnus(Tf);
+    const double Eff_C = 2.0 * C.Mw * C.Gam / (C.Ru * C.rhow * r * d.Tp);
+    const double Eff_S = d.kappa * d.m_s * C.rhow / C.rhos / (Volp * rhop - d.m_s);
+    const double estar = einf * exp(C.Mw * C.Lv / C.Ru * (1.0 / Tf - 1.0 / d.Tp) + Eff_C - Eff_S);
+    const double qstar = C.Mw / C.Ru * estar / d.Tp / rhoa;
+    o.qstar = qstar;
+    o.vrhs[0] = corrfac * taup_i * (f.u[0] - d.vx) + C.g[0];
+    o.vrhs[1] = corrfac * taup_i * (f.u[1] - d.vy) + C.g[1];
+    o.vrhs[2] = corrfac * taup_i * (f.u[2] - d.vz) + C.g[2];
+    o.radrhs = (fl.ievap == 1) ? Shp / 9.0 / C.Sc * rhop / C.rhow * r * taup_i * (f.q - qstar) : 0.0;
+    o.Tprhs_s = -(Nup / 3.0 / C.Pra * C.CpaCpp * rhop / C.rhow * taup_i * (d.Tp - Tf));
+    o.Tprhs_L = 3.0 * C.Lv / C.Cpp / r * o.radrhs;
+  }
+  d.x = xtmp[0] + dt * gs * o.xrhs[0];
+  d.y = xtmp[1] + dt * gs * o.xrhs[1];
+  d.z = xtmp[2] + dt * gs * o.xrhs[2];
+  d.vx = vtmp[0] + dt * gs * o.vrhs[0];
+  d.vy = vtmp[1] + dt * gs * o.vrhs[1];
+  d.vz = vtmp[2] + dt * gs * o.vrhs[2];
+  d.Tp = Tptmp + dt * gs * o.Tprhs_s;
+  d.Tp = d.Tp + dt * gs * o.Tprhs_L;
+  d.r = radiustmp + dt * gs * o.radrhs;
+  if (istage == 3) d.res = d.res + dt;
+  o.uf[0] = f.u[0];
+  o.uf[1] = f.u[1];
+  o.uf[2] = f.u[2];
+  o.Tf = Tf;
+  o.qinf = f.q;
+  apply_bcs(g, s_z, d);
+}
+
+// -------------------------------------------------------- coupling amounts
+
+// Per-droplet source amounts of particle_coupling_update (1187-1264): the
+// node contribution is amount * wtt / dV.  a[0..2] momentum (icouple),
+// a[3] heat (iTcouple), a[4] vapour and a[5] enthalpy (iHcouple).
+__device__ __forceinline__ void coupling_amounts(const DevConsts& C, const DevFlags& fl, const Drop& d,
+                                                 double Tf, const double vold[3], double Tpold,
+                                                 double rold, int64_t mult, double a[6]) {
+  const double dt = C.dt;
+  const double vrhs0 = (d.vx - vold[0]) / dt, vrhs1 = (d.vy - vold[1]) / dt, vrhs2 = (d.vz - vold[2]) / dt;
+  const double r = d.r;
+  const double radrhs = (r - rold) / dt;
+  const double Tprhs_L = 3.0 * C.Lv / C.Cpp / r * radrhs;
+  const double Tprhs_s = (d.Tp - Tpold) / dt - Tprhs_L;
+  double rhoa;
+  if (fl.iexner == 1) rhoa = func_p_base(C, d.z) / C.Rd / Tf;
+  else rhoa = C.surf_rho;
+  const double V = C.pi2 * 2.0 / 3.0 * (r * r * r);
+  const double rhop = (d.m_s + V * C.rhow) / V;
+  const double partmass = rhop * 2.0 / 3.0 * C.pi2 * (r * r * r);
+  const double taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
+  const double rm = (double)mult;
+  const double vap = C.rhow / rhoa * C.pi2 * 2 * (r * r) * radrhs;
+  for (int k = 0; k < 6; ++k) a[k] = 0.0;
+  if (fl.icouple == 1) {
+    a[0] = -(partmass / rhoa * (vrhs0 - C.g[0]) + vap * d.vx) * rm;
+    a[1] = -(partmass / rhoa * (vrhs1 - C.g[1]) + vap * d.vy) * rm;
+    a[2] = -(partmass / rhoa * (vrhs2 - C.g[2]) + vap * d.vz) * rm;
+  }
+  if (fl.iTcouple == 1) a[3] = -(Tprhs_s * 6.0 * C.rhow / rhop / C.CpaCpp / taup_i * (C.pi2 / 2.0) * r * C.nuf) * rm;
+  if (fl.iHcouple == 1) {
+    a[4] = -vap * rm;
+    a[5] = (-vap * C.Cpv / C.Cpa * d.Tp + vap * C.Cpv / C.Cpa * Tf) * rm;
+  }
+}
+
+}  // namespace ntlp

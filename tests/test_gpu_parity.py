@@ -1,0 +1,264 @@
+"""Parity of the HIP droplet step with the oracle, through the C ABI.
+
+Bit-exact: cell indices, per-cell counts, the stable sort permutation, the
+critical radius table lookup.  FP64 outputs: bounded relative error against
+the oracle (the tolerances are per quantity in TOL64 below and absorb the
+last-ulp differences of exp/log/pow between the device math library and the
+host's).  Mixed precision: radius within 1e-5 relative of the FP64 oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from ntlp_amd import (COL, LAYOUT_CORE, LAYOUT_EXT, LAYOUT_XPENCIL, MODE_BE, MODE_RK3, PREC_FP64,
+                      PREC_FP64_AJ, PREC_MIXED, DropletStep, Flags)
+
+from .helpers import edge_positions, rel_err, small_case
+
+pytestmark = pytest.mark.gpu
+
+# relative error bound and magnitude floor per column, FP64 device vs oracle
+TOL64 = {
+    "x": (1e-13, 1.0), "y": (1e-13, 1.0), "z": (1e-13, 1.0),
+    "vx": (1e-10, 0.1), "vy": (1e-10, 0.1), "vz": (1e-10, 0.1),
+    "ufx": (1e-13, 0.1), "ufy": (1e-13, 0.1), "ufz": (1e-13, 0.1),
+    "Tf": (1e-14, 300.0), "qinf": (1e-13, 0.02),
+    "Tp": (1e-11, 300.0), "radius": (1e-9, 1e-6), "qstar": (1e-9, 0.02),
+    "res": (0.0, 1.0), "actres": (0.0, 1.0), "numact": (0.0, 1.0), "rc": (0.0, 1e-6),
+}
+
+
+def make_step(grid, consts, flags, fields, p, layout=LAYOUT_CORE):
+    s = DropletStep(grid, consts, flags)
+    s.set_fields(*fields, layout=layout)
+    s.put(p)
+    return s
+
+
+def compare(got, want, cols, tol=TOL64):
+    bad = {}
+    for c in cols:
+        r, floor = tol[c]
+        e = rel_err(got[c], want[c], floor)
+        if e > r:
+            bad[c] = e
+    return bad
+
+
+def test_cell_indices_counts_and_sort_bit_exact():
+    for kind in ("uniform", "stretched"):
+        grid, consts, flags, fields, p, ora = small_case(seed=1, n=4000, kind=kind)
+        edges = edge_positions(grid)
+        p.soa[COL["x"], :len(edges)] = edges[:, 0]
+        p.soa[COL["y"], :len(edges)] = edges[:, 1]
+        p.soa[COL["z"], :len(edges)] = edges[:, 2]
+        s = make_step(grid, consts, flags, fields, p)
+        cellidx, counts, offsets, perm = s.bin()
+        want = ora.cell_index(p["x"], p["y"], p["z"])
+        assert np.array_equal(cellidx, want[:, :4]), kind
+        keys, c_want, o_want, p_want = ora.bin(p["x"], p["y"], p["z"])
+        assert np.array_equal(counts, c_want), kind
+        assert np.array_equal(offsets, o_want), kind
+        assert np.array_equal(perm, p_want), kind
+        assert np.array_equal(s.wcell_counts(), ora.wcell_counts(p["x"], p["y"], p["z"])), kind
+        s.close()
+
+
+@pytest.mark.parametrize("ilin", [1, 0])
+def test_interpolation(ilin):
+    grid, consts, flags, fields, p, ora = small_case(seed=2, flags=Flags(ilin=ilin), kind="stretched")
+    s = make_step(grid, consts, flags, fields, p)
+    s.interp()
+    got = s.get()
+    want = p.copy()
+    ora.interp(want.soa)
+    bad = compare(got, want, ["ufx", "ufy", "ufz", "Tf", "qinf"])
+    assert not bad, bad
+    s.close()
+
+
+def test_field_layouts_agree():
+    grid, consts, flags, fields, p, ora = small_case(seed=3, n=500)
+    outs = []
+    ext = [ora.fill_ext(f) for f in fields]
+    # flow-solver x-pencil layout u(1:nnx+2, 1:nny, 0:nnz+1): numpy (nnz+2, nny, nnx+2)
+    xp = []
+    for f in fields:
+        a = np.zeros((grid.nnz + 2, grid.nny, grid.nnx + 2))
+        a[:, :, :grid.nnx] = np.transpose(f, (2, 1, 0))
+        xp.append(a)
+    for layout, data in ((LAYOUT_CORE, fields), (LAYOUT_EXT, ext), (LAYOUT_XPENCIL, xp)):
+        s = make_step(grid, consts, flags, data, p, layout=layout)
+        s.interp()
+        outs.append(s.get())
+        s.close()
+    for o in outs[1:]:
+        assert np.array_equal(o.soa, outs[0].soa)
+
+
+def test_crit_radius_bit_exact():
+    grid, consts, flags, fields, p, ora = small_case(seed=4)
+    q = p.copy()
+    q["rc"] = 0.0
+    s = make_step(grid, consts, flags, fields, q)
+    s.crit_radius()
+    assert np.array_equal(s.get()["rc"], p["rc"])
+    s.close()
+
+
+def test_put_get_and_records_roundtrip():
+    grid, consts, flags, fields, p, ora = small_case(seed=5, n=777)
+    s = make_step(grid, consts, flags, fields, p)
+    got = s.get()
+    assert np.array_equal(got.soa, p.soa) and np.array_equal(got.mult, p.mult)
+    assert np.array_equal(got.pidx, p.pidx)
+    rec = s.get_records()
+    s.put_records(rec)
+    again = s.get()
+    assert np.array_equal(again.soa, p.soa) and np.array_equal(again.mult, p.mult)
+    s.close()
+
+
+@pytest.mark.parametrize("kind", ["uniform", "stretched"])
+def test_particle_update_be_fp64(kind):
+    grid, consts, flags, fields, p, ora = small_case(seed=6, kind=kind)
+    s = make_step(grid, consts, flags, fields, p)
+    want = p.copy()
+    for it in (1, 2):
+        s.particle_update_BE(it=it)
+        cnt_o = ora.update_be(want.soa, it)
+    got = s.get()
+    bad = compare(got, want, ["x", "y", "z", "vx", "vy", "vz", "ufx", "ufy", "ufz", "Tf", "qinf", "Tp",
+                              "radius", "qstar", "res", "actres", "numact"])
+    assert not bad, bad
+    for c in ("vxold", "vyold", "vzold", "Tpold", "radiusold"):
+        assert rel_err(got[c], want[c], 1e-6) < 1e-9, c
+    s.close()
+
+
+def test_particle_update_rk3_fp64():
+    grid, consts, flags, fields, p, ora = small_case(seed=7, n=2000, dt=2e-5)
+    s = make_step(grid, consts, flags, fields, p)
+    want = p.copy()
+    for it in (1, 2):
+        for st in (1, 2, 3):
+            s.particle_update_rk3(st, it=it)
+            ora.update_rk3(want.soa, st, it)
+    got = s.get()
+    tol = dict(TOL64)
+    tol["Tp"] = (1e-12, 300.0)
+    tol["radius"] = (1e-11, 1e-6)
+    bad = compare(got, want, ["x", "y", "z", "vx", "vy", "vz", "Tp", "radius", "qstar", "res"], tol)
+    assert not bad, bad
+    # right-hand sides carry differences such as Tp - Tf, so they are compared
+    # on the scale of their column (1e-13 differences of Tp become 1e-8 of
+    # Tp - Tf)
+    for c in ("xrhsx", "vrhsz", "Tprhs_s", "Tprhs_L", "radrhs"):
+        assert rel_err(got[c], want[c], np.max(np.abs(want[c])) + 1e-30) < 1e-9, c
+    s.close()
+
+
+@pytest.mark.parametrize("deterministic", [False, True])
+@pytest.mark.parametrize("kind", ["uniform", "stretched"])
+def test_coupling_update(deterministic, kind):
+    """particle_coupling_update on the oracle's own post-step state: only the
+    deposit arithmetic and the summation order differ."""
+    grid, consts, flags, fields, p, ora = small_case(seed=8, kind=kind)
+    want = p.copy()
+    ora.update_be(want.soa, 2)
+    src_o = ora.coupling(want.soa, want.mult)
+    s = make_step(grid, consts, flags, fields, want)
+    src = s.particle_coupling_update(deterministic=deterministic)
+    for f in range(6):
+        scale = np.max(np.abs(src_o[f])) + 1e-300
+        # field 5 is the reference's -A*Tp + A*Tf (particles.f90:1258-1260): two
+        # terms ~Tp/(Tp-Tf) ~ 3000 times larger than their sum cancel
+        tol = 1e-12 if f < 5 else 1e-9
+        assert np.max(np.abs(src[f] - src_o[f])) <= tol * scale, f
+    if deterministic:
+        again = s.particle_coupling_update(deterministic=True)
+        assert np.array_equal(again, src)
+    s.close()
+
+
+def test_coupling_single_droplet_on_a_node_and_linearity():
+    """A droplet exactly on node (i, j, k) deposits on that node only; the
+    deposit is linear in the multiplicity."""
+    grid, consts, flags, fields, p, ora = small_case(seed=9, n=64)
+    want = p.copy()
+    ora.update_be(want.soa, 2)
+    one = want.copy()
+    i, j, k = 5, 4, 7
+    one.soa = one.soa[:, :1].copy()
+    one.mult, one.pidx, one.procidx = one.mult[:1].copy(), one.pidx[:1].copy(), one.procidx[:1].copy()
+    one["x"], one["y"], one["z"] = grid.dx * (i - 1), grid.dy * (j - 1), grid.zz[k]
+    s = make_step(grid, consts, flags, fields, one)
+    src = s.particle_coupling_update()
+    big = np.abs(src).max()
+    nz = np.argwhere(np.any(np.abs(src) > 1e-12 * big, axis=0))
+    assert nz.tolist() == [[i - 1, j - 1, k]]
+    one.mult[:] *= 3
+    s.put(one)
+    src3 = s.particle_coupling_update()
+    assert np.allclose(src3, 3 * src, rtol=1e-14, atol=0)
+    s.close()
+
+
+@pytest.mark.parametrize("mode", [MODE_BE, MODE_RK3])
+def test_fused_steps_match_single_steps(mode):
+    dt = 5e-4 if mode == MODE_BE else 2e-5
+    grid, consts, flags, fields, p, ora = small_case(seed=10, dt=dt)
+    s = make_step(grid, consts, flags, fields, p)
+    s.step(mode=mode, precision=PREC_FP64, nsteps=4, it0=2)
+    got = s.get()
+    src = s.sources()
+    want = p.copy()
+    for it in range(2, 6):
+        if mode == MODE_BE:
+            ora.update_be(want.soa, it)
+        else:
+            for st in (1, 2, 3):
+                ora.update_rk3(want.soa, st, it)
+    src_o = ora.coupling(want.soa, want.mult)
+    cols = ["x", "y", "z", "vx", "vy", "vz", "Tp", "radius", "res", "ufx", "Tf", "qinf", "qstar"]
+    tol = dict(TOL64)
+    tol["radius"] = (1e-8, 1e-6)
+    bad = compare(got, want, cols, tol)
+    assert not bad, bad
+    assert np.array_equal(got.mult, p.mult) and np.array_equal(got.pidx, p.pidx)
+    # after four steps Tp - Tpold and v - vold carry the 1e-13 state
+    # differences at 1e-7 of their own size
+    for f in range(6):
+        scale = np.max(np.abs(src_o[f])) + 1e-300
+        assert np.max(np.abs(src[f] - src_o[f])) <= 1e-5 * scale, f
+    s.close()
+
+
+@pytest.mark.parametrize("prec", [PREC_MIXED, PREC_FP64_AJ])
+def test_mixed_precision_radius_within_1e5(prec):
+    grid, consts, flags, fields, p, ora = small_case(seed=11, n=4000, kind="stretched")
+    s = make_step(grid, consts, flags, fields, p)
+    s.step(mode=MODE_BE, precision=prec, nsteps=10, it0=2)
+    got = s.get()
+    want = p.copy()
+    for it in range(2, 12):
+        ora.update_be(want.soa, it)
+    assert rel_err(got["radius"], want["radius"], 1e-7) <= 1e-5
+    assert rel_err(got["Tp"], want["Tp"], 300.0) <= 1e-6
+    assert rel_err(got["z"], want["z"], 1.0) <= 1e-9
+    s.close()
+
+
+def test_counters_count_solves():
+    grid, consts, flags, fields, p, ora = small_case(seed=12)
+    s = make_step(grid, consts, flags, fields, p)
+    s.counters(reset=True)
+    s.step(mode=MODE_BE, precision=PREC_FP64, nsteps=3, it0=2)
+    c = s.counters()
+    assert c["steps"] == 3
+    assert c["solves"] == 3 * p.n  # qinf > 0 and ievap = 1 for every droplet
+    assert sum(c["gn_hist"]) == c["solves"]
+    rows = s.step_counters()
+    assert rows.shape[0] == 3 and rows[:, 6].sum() == c["solves"]
+    s.close()
