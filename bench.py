@@ -1,0 +1,251 @@
+"""Droplet-step benchmark on one MI355X.
+
+Workload (default): 20M droplets (60% haze at activation) on the
+256x256x128 tanh-stretched grid, backward-Euler droplet step (the
+ipart_method=2 step every NTLP case runs once per LES RK3 step), two-way
+coupling on.  Each timed step = gather + advance with the Newton radius and
+temperature solve + boundary conditions + coupling scatter + counting sort.
+
+Reports droplet-updates/s for the mixed-precision step (value) beside the
+FP64 step, the roofline of the dominant kernel from HIP events on the
+library's stream, and the oracle timed on a bounded sample on the host.
+
+  python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--workload 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def step_kernel_bytes(n: int, grid, precision: int) -> float:
+    """Algorithmic HBM bytes of one step_be launch (not the last of a call):
+    droplet state in (14 FP64 + mult + home + perm) and out (14 FP64 + mult +
+    home), 6 FP64 coupling amounts and the u-cell key out, and one pass over
+    the field values the gather needs (u, v, w in FP32 for mixed precision,
+    FP64 otherwise; T, q FP64) over the ext array."""
+    per_drop = (14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4) + 6 * 8 + 4
+    ext = (grid.nnz + 2) * (grid.nny + 5) * (grid.nnx + 5)
+    per_cell = (3 * 4 + 2 * 8) if precision == 1 else 5 * 8
+    return float(n) * per_drop + float(ext) * per_cell
+
+
+def prepare(wl_id: int, n: int | None, seed: int):
+    from ntlp_amd import synth
+
+    t = time.time()
+    wl = synth.workload(wl_id, n=n, seed=seed)
+    log(f"[bench] workload {wl_id}: {wl.name}; inputs built in {time.time() - t:.1f}s")
+    return wl
+
+
+def init_device(wl):
+    from ntlp_amd import DropletStep
+
+    s = DropletStep(wl.grid, wl.consts, wl.flags)
+    s.set_fields(*wl.fields)
+    s.put(wl.droplets)
+    s.interp(set_vp=True, set_tp=True)  # vp = uf, Tp = Tf at creation
+    s.crit_radius()
+    return s
+
+
+def run_steps(s, init_state, mode, prec, steps, warmup, it0=2):
+    s.put(init_state)
+    s.counters(reset=True)
+    if warmup:
+        s.step(mode=mode, precision=prec, nsteps=warmup, it0=it0)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(mode=mode, precision=prec, nsteps=steps, it0=it0 + warmup)
+    s.sync()
+    dt = time.perf_counter() - t0
+    return dt
+
+
+def cpu_baseline(wl_id: int, seed: int, target_s: float = 12.0):
+    """The oracle (single-threaded FP64 C restatement of the reference loop)
+    on a bounded sample of the same workload: same grid and fields, the first
+    droplets of the same ensemble, one BE step + coupling deposit."""
+    from ntlp_amd import COL, synth
+    from oracle import Oracle
+
+    n_probe = 20000
+    wl = synth.workload(wl_id, n=n_probe, seed=seed)
+    ora = Oracle(wl.grid, wl.consts, wl.flags)
+    ora.set_fields(*wl.fields)
+    p = wl.droplets
+    ora.interp(p.soa)
+    for d in "xyz":
+        p["v" + d] = p["uf" + d]
+    p["Tp"] = p["Tf"]
+    p["rc"] = ora.crit_radius(p.soa, COL)
+    t0 = time.perf_counter()
+    ora.update_be(p.soa, 2)
+    ora.coupling(p.soa, p.mult)
+    t_probe = time.perf_counter() - t0
+    reps = max(1, int(target_s / max(t_probe, 1e-3)))
+    reps = min(reps, 200)
+    t0 = time.perf_counter()
+    for r in range(reps):
+        ora.update_be(p.soa, 3 + r)
+        ora.coupling(p.soa, p.mult)
+    t = time.perf_counter() - t0
+    rate = n_probe * reps / t
+    return {"value": rate, "unit": "droplet-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (C, FP64, -O2) BE step + coupling deposit on {n_probe} droplets of the same "
+                      f"ensemble and fields, {reps} consecutive steps, {t:.1f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", type=int, default=3)
+    ap.add_argument("--droplets", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": x.numel() * 8, "copy_probe_bytes_written": y.numel() * 8,
+                          "launches": 3}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the droplet step shards by "
+              "droplets with an all-reduce of the coupling fields; that path is not built).", file=sys.stderr)
+        sys.exit(2)
+
+    from ntlp_amd import MODE_BE, PREC_FP64, PREC_MIXED
+
+    wl = prepare(args.workload, args.droplets, args.seed)
+    s = init_device(wl)
+    init_state = s.get()
+    n = init_state.n
+    g = wl.grid
+    log(f"[bench] {n} droplets on {g.nnx}x{g.nny}x{g.nnz}; device ready")
+
+    # mixed precision: the headline
+    t_mixed = run_steps(s, init_state, MODE_BE, PREC_MIXED, args.steps, args.warmup)
+    cnt_mixed = s.counters()
+    rows = s.step_counters()
+    end_mixed = s.get()
+    log(f"[bench] mixed: {args.steps} steps in {t_mixed:.3f}s")
+
+    # same steps with per-kernel HIP-event timing for the roofline
+    s.put(init_state)
+    if args.warmup:
+        s.step(mode=MODE_BE, precision=PREC_MIXED, nsteps=args.warmup, it0=2)
+    s.timing(True)
+    s.step(mode=MODE_BE, precision=PREC_MIXED, nsteps=args.steps, it0=2 + args.warmup)
+    ktimes = s.kernel_times()
+    s.timing(False)
+    log("[bench] kernel times (mixed): " + ", ".join(f"{k}={v[1] / v[0]:.3f}ms x{v[0]}" for k, v in ktimes.items()))
+
+    fp64 = None
+    if not args.no_fp64:
+        t_fp64 = run_steps(s, init_state, MODE_BE, PREC_FP64, args.steps, args.warmup)
+        cnt_fp64 = s.counters()
+        end_fp64 = s.get()
+        log(f"[bench] fp64: {args.steps} steps in {t_fp64:.3f}s")
+        s.put(init_state)
+        if args.warmup:
+            s.step(mode=MODE_BE, precision=PREC_FP64, nsteps=args.warmup, it0=2)
+        s.timing(True)
+        s.step(mode=MODE_BE, precision=PREC_FP64, nsteps=args.steps, it0=2 + args.warmup)
+        ktimes64 = s.kernel_times()
+        s.timing(False)
+        log("[bench] kernel times (fp64): " + ", ".join(f"{k}={v[1] / v[0]:.3f}ms x{v[0]}" for k, v in ktimes64.items()))
+        rr = np.abs(end_mixed["radius"] / end_fp64["radius"] - 1.0)
+        fp64 = {
+            "value": n * args.steps / t_fp64, "ms_per_step": 1e3 * t_fp64 / args.steps,
+            "newton_iters_per_solve": cnt_fp64["gn_iters"] / max(cnt_fp64["solves"], 1),
+            "lm_fallbacks": cnt_fp64["num100"], "failed_solves": cnt_fp64["numimpos"],
+            "kernel_ms": {k: v[1] / v[0] for k, v in ktimes64.items()},
+        }
+        accuracy = {"radius_max_rel_err_mixed_vs_fp64": float(rr.max()),
+                    "radius_p99_rel_err_mixed_vs_fp64": float(np.quantile(rr, 0.99)),
+                    "Tp_max_abs_err_K": float(np.abs(end_mixed["Tp"] - end_fp64["Tp"]).max()),
+                    "z_max_abs_err_m": float(np.abs(end_mixed["z"] - end_fp64["z"]).max())}
+    else:
+        accuracy = None
+
+    # roofline of the dominant kernel (mixed precision)
+    dom = max(ktimes.items(), key=lambda kv: kv[1][1])
+    dom_name, (dom_launches, dom_ms) = dom
+    avg_ms = dom_ms / dom_launches
+    if dom_name.startswith("step_"):
+        traffic_alg = step_kernel_bytes(n, g, PREC_MIXED)
+    else:
+        traffic_alg = None
+    roofline = {"kernel": dom_name, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                "avg_launch_ms": avg_ms, "launches": dom_launches}
+    if traffic_alg:
+        ach = traffic_alg / (avg_ms * 1e-3) / 1e9
+        roofline.update({"achieved": ach, "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": traffic_alg})
+    roofline["traffic"] = None
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(args.workload, args.seed)
+        log(f"[bench] cpu baseline {cpu['value']:.3g} droplet-updates/s")
+
+    value = n * args.steps / t_mixed
+    per_step_iters = (rows[:, 4] / np.maximum(rows[:, 6], 1)).tolist() if len(rows) else []
+    out = {
+        "metric": "droplet-updates/s per RK3 step, FP64 vs mixed; serve p50/p99 latency",
+        "value": value,
+        "unit": "droplet-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mixed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "droplets": n, "grid": f"{g.nnx}x{g.nny}x{g.nnz}",
+                   "mode": "BE (ipart_method=2)", "precision": "mixed", "dt": wl.consts.dt,
+                   "coupling": "on", "parallelism": "1 GPU"},
+        "fp64": fp64,
+        "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
+        "accuracy": accuracy,
+        "newton": {"iters_per_solve": cnt_mixed["gn_iters"] / max(cnt_mixed["solves"], 1),
+                   "lm_fallbacks": cnt_mixed["num100"], "failed_solves": cnt_mixed["numimpos"],
+                   "activations": cnt_mixed["actnum"], "deactivations": cnt_mixed["denum"],
+                   "iters_per_solve_per_step": per_step_iters[-args.steps:]},
+        "kernel_ms": {k: v[1] / v[0] for k, v in ktimes.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

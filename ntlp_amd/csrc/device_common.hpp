@@ -27,6 +27,14 @@ struct DevConsts {
   double surf_rho, surf_p, tsfcc1, grav, radius_init, pi2;
   double g[3], zetas[3], gama[3];
   double Sc13, Pra13;  // Sc**(1./3.), Pra**(1./3.): loop invariants of ie_vrt_nd
+  // constant subexpressions, formed on the host in the reference's order
+  double LvMw_Ru;         // (Lv*Mw)/Ru
+  double Mw2Gam;          // (2*Mw)*Gam
+  double Ru_rhow;         // Ru*rhow
+  double Lv3;             // 3*Lv
+  double Mw_Ru;           // Mw/Ru
+  double Mw2Gam_Ru_rhow;  // 2 Mw Gam / (Ru rhow)
+  double isol;            // 1 / ((2/3) pi2 rhos)
 };
 
 struct DevFlags {
@@ -77,6 +85,33 @@ __device__ __forceinline__ Cell cell_of(const DevGrid& g, const double* s_z, con
   int pz;
   if (g.interleaved) {
     // z(m-2) < zz(m-1) <= x < zz(m) < z(m): the first z above x is m-1 or m.
+    pz = (pzz >= 1 && s_z[pzz - 1] > z) ? pzz - 1 : pzz;
+  } else {
+    pz = first_gt(s_z, len, g.nsteps, z);
+  }
+  c.kpt = (pzz == len) ? -2 : pzz - 1;
+  c.kwpt = (pz == len) ? -2 : pz - 1;
+  return c;
+}
+
+// The same indices when a nearby u level is known (the droplet's previous
+// cell): two comparisons confirm kpt = kg, i.e. zz(kg) <= z < zz(kg+1), and
+// only otherwise the binary search runs, so the result is always the
+// reference's minloc.
+__device__ __forceinline__ Cell cell_near(const DevGrid& g, const double* s_z, const double* s_zz,
+                                          double x, double y, double z, int kg) {
+  Cell c;
+  c.ipt = (int)floor(x / g.dx) + 1;
+  c.jpt = (int)floor(y / g.dy) + 1;
+  const int len = g.nnz + 2;
+  kg = clampi(kg, 0, g.nnz);
+  int pzz;
+  if (s_zz[kg] <= z && s_zz[kg + 1] > z) pzz = kg + 1;
+  else if (kg + 2 < len && s_zz[kg + 1] <= z && s_zz[kg + 2] > z) pzz = kg + 2;
+  else if (kg >= 1 && s_zz[kg - 1] <= z && s_zz[kg] > z) pzz = kg;
+  else pzz = first_gt(s_zz, len, g.nsteps, z);
+  int pz;
+  if (g.interleaved) {
     pz = (pzz >= 1 && s_z[pzz - 1] > z) ? pzz - 1 : pzz;
   } else {
     pz = first_gt(s_z, len, g.nsteps, z);

@@ -42,9 +42,12 @@ struct StepArgs {
   const int32_t* perm;   // sorted index -> src slot (nullptr: identity)
   double* cold;          // put-order cold columns
   double* amounts;       // [6][cap] coupling amounts in dst slot order
-  int32_t* keys;         // [cap] u-cell key of the new position (dst slot order)
+  const int32_t* keys_in; // u-cell key of the current position (src slot order), or null
+  int32_t* keys;          // [cap] u-cell key of the new position (dst slot order)
   int32_t* counts;       // u-cell histogram (must be zero on entry)
-  unsigned long long* tally;  // row of kTallyWidth counters
+  unsigned long long* tally;  // kTallySlots x kTallyWidth counter slots
+  double* hand;          // [kHand][cap] advance -> solve hand-over (slot order)
+  int32_t* hand_kpt;     // [cap] u level of the gather position
   int it;
   int istage;
   int write_cold;        // write the record diagnostics (final step of a fused call)
@@ -70,19 +73,54 @@ __device__ __forceinline__ long long wave_sum(long long v) {
   return v;
 }
 
-// Adds a thread's Newton tally to the counter row: wave reductions for the
-// scalar counters and an LDS histogram for the iteration counts.
-__device__ __forceinline__ void flush_tally(const Tally& tl, unsigned long long* row, int* s_hist) {
-  long long v[7] = {tl.num100, tl.numimpos, tl.actnum, tl.denum, tl.gn_iters, tl.lm_iters, tl.solves};
+// Newton tallies are summed per workgroup (ballots and 32-bit wave sums, then
+// LDS) and added to one of kTallySlots counter slots, each on its own cache
+// lines, chosen by workgroup; k_tally_close folds the slots into the step's
+// row.  (One shared row would put every workgroup's atomics on one L2 line.)
+constexpr int kTallySlots = 64;
+
+__device__ __forceinline__ int wave_sum_i32(int v) {
 #pragma unroll
-  for (int k = 0; k < 7; ++k) {
-    long long s = wave_sum(v[k]);
-    if (lane_id() == 0 && s != 0) atomicAdd(&row[k], (unsigned long long)s);
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// s_red: LDS scratch of kTallyWidth ints, zeroed (with the histogram part)
+// before the kernel's first __syncthreads.
+__device__ __forceinline__ void flush_tally(const Tally& tl, unsigned long long* slots, int* s_red) {
+  const int c_n100 = __popcll(__ballot(tl.num100 != 0));
+  const int c_imp = __popcll(__ballot(tl.numimpos != 0));
+  const int c_act = __popcll(__ballot(tl.actnum != 0));
+  const int c_de = __popcll(__ballot(tl.denum != 0));
+  const int c_sol = __popcll(__ballot(tl.solves != 0));
+  const int c_gn = wave_sum_i32(tl.gn_iters);
+  const int c_lm = __ballot(tl.lm_iters != 0) ? wave_sum_i32(tl.lm_iters) : 0;
+  if (lane_id() == 0) {
+    const int v[7] = {c_n100, c_imp, c_act, c_de, c_gn, c_lm, c_sol};
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      if (v[k]) atomicAdd(&s_red[k], v[k]);
   }
-  if (tl.hist >= 0) atomicAdd(&s_hist[tl.hist < 63 ? tl.hist : 63], 1);
+  if (tl.hist >= 0) atomicAdd(&s_red[kTHist0 + (tl.hist < 63 ? tl.hist : 63)], 1);
   __syncthreads();
-  if (threadIdx.x < 64 && s_hist[threadIdx.x] != 0)
-    atomicAdd(&row[kTHist0 + threadIdx.x], (unsigned long long)s_hist[threadIdx.x]);
+  if (threadIdx.x < kTallyWidth && s_red[threadIdx.x] != 0)
+    atomicAdd(&slots[(blockIdx.x % kTallySlots) * kTallyWidth + threadIdx.x], (unsigned long long)s_red[threadIdx.x]);
+}
+
+__device__ __forceinline__ void zero_tally_lds(int* s_red) {
+  for (int i = threadIdx.x; i < kTallyWidth; i += blockDim.x) s_red[i] = 0;
+}
+
+// Folds the slots into the step's row (and counts the step), clearing them.
+__global__ void k_tally_close(unsigned long long* slots, unsigned long long* row, int steps) {
+  for (int c = threadIdx.x; c < kTallyWidth; c += blockDim.x) {
+    unsigned long long s = 0;
+    for (int k = 0; k < kTallySlots; ++k) {
+      s += slots[k * kTallyWidth + c];
+      slots[k * kTallyWidth + c] = 0ull;
+    }
+    row[c] = s + (c == kTSteps ? (unsigned long long)steps : 0ull);
+  }
 }
 
 // ------------------------------------------------------------ load/store
@@ -204,9 +242,9 @@ __device__ __forceinline__ void count_key(int* counts, int key, bool valid) {
 template <int MODE, int PREC, int ILIN>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
-  __shared__ int s_hist[64];
+  __shared__ int s_red[kTallyWidth];
   stage_grid(a.g, s_z, s_zz);
-  if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
+  zero_tally_lds(s_red);
   __syncthreads();
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
   const bool valid = t < a.n;
@@ -218,16 +256,19 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     const int64_t mult = a.src.mult[s];
     const int32_t home = a.src.home[s];
     Diag o;
+    const int kg = a.keys_in ? a.keys_in[s] % (a.g.nnz + 1) : -1;
     if (MODE == 1) {  // three particle_update_rk3 stages; stage registers stay in VGPRs
       o.xrhs[0] = o.xrhs[1] = o.xrhs[2] = 0.0;  // zetas(1) = 0 discards the stale values
       o.vrhs[0] = o.vrhs[1] = o.vrhs[2] = 0.0;
       o.Tprhs_s = o.Tprhs_L = o.radrhs = 0.0;
-      for (int st = 1; st <= 3; ++st) rk3_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, st, a.it, o);
+      o.kpt = kg;
+      for (int st = 1; st <= 3; ++st)
+        rk3_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, st, a.it, o, o.kpt);
     } else {
-      be_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl);
+      be_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg);
     }
     double am[6];
-    coupling_amounts(a.C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
+    coupling_amounts<PREC == PREC_MIXED>(a.C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
 #pragma unroll
     for (int k = 0; k < 6; ++k) a.amounts[k * a.cap + t] = am[k];
     store_dynamic(a.dst.hot, a.cap, t, d);
@@ -236,7 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     a.dst.hot[kHRC * a.cap + t] = d.rc;
     a.dst.mult[t] = mult;
     a.dst.home[t] = home;
-    const Cell c = cell_of(a.g, s_z, s_zz, d.x, d.y, d.z);
+    const Cell c = cell_near(a.g, s_z, s_zz, d.x, d.y, d.z, o.kpt);
     key = ukey_of(a.g, c);
     a.keys[t] = key;
     if (a.write_cold) {
@@ -246,8 +287,104 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     }
   }
   count_key(a.counts, key, valid);
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.tally[kTSteps] = 1ull;
-  if (MODE == 2) flush_tally(tl, a.tally, s_hist);
+  if (MODE == 2) flush_tally(tl, a.tally, s_red);
+}
+
+// ------------------------------------------------ split backward-Euler step
+//
+// The BE step runs as two kernels so that each keeps the GPU busy on its own
+// bound: k_be_advance streams the droplets through the cell sort, gathers the
+// fluid and advances velocity and position (memory bound, few registers, high
+// occupancy); k_be_solve streams them again in slot order and runs the FP64
+// radius/temperature Newton solve, walls, coupling amounts and the new cell
+// key (ALU bound).  The hand-over between them is kHand FP64 values and the
+// gather's u level per droplet.
+enum { kHandTf = 0, kHandQ, kHandRho, kHandDiff, kHandRep, kHandVx, kHandVy, kHandVz, kHand };
+
+template <int PREC, int ILIN>
+__global__ void __launch_bounds__(kBlock) k_be_advance(StepArgs a) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  stage_grid(a.g, s_z, s_zz);
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  if (t >= a.n) return;
+  const int64_t s = a.perm ? (int64_t)a.perm[t] : t;
+  Drop d = load_drop(a.src.hot, a.cap, s);
+  const int64_t mult = a.src.mult[s];
+  const int32_t home = a.src.home[s];
+  const int kg = a.keys_in ? a.keys_in[s] % (a.g.nnz + 1) : -1;
+  Diag o;
+  const BeHand h = be_advance<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.it, o, kg);
+  apply_periodic(a.g, d);
+  store_drop(a.dst.hot, a.cap, t, d);
+  a.dst.mult[t] = mult;
+  a.dst.home[t] = home;
+  double* hd = a.hand;
+  const int64_t cap = a.cap;
+  hd[kHandTf * cap + t] = h.Tf;
+  hd[kHandQ * cap + t] = h.qinf;
+  hd[kHandRho * cap + t] = h.rhoa;
+  hd[kHandDiff * cap + t] = h.diffnorm;
+  hd[kHandRep * cap + t] = h.Rep;
+  hd[kHandVx * cap + t] = o.vold[0];
+  hd[kHandVy * cap + t] = o.vold[1];
+  hd[kHandVz * cap + t] = o.vold[2];
+  a.hand_kpt[t] = o.kpt;
+  if (a.write_cold) {
+    a.cold[kCUFX * cap + home] = o.uf[0];
+    a.cold[kCUFY * cap + home] = o.uf[1];
+    a.cold[kCUFZ * cap + home] = o.uf[2];
+    a.cold[kCTF * cap + home] = o.Tf;
+    a.cold[kCQINF * cap + home] = o.qinf;
+    store_old(a.cold, cap, home, o);
+  }
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(kBlock) k_be_solve(StepArgs a) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  __shared__ int s_red[kTallyWidth];
+  stage_grid(a.g, s_z, s_zz);
+  zero_tally_lds(s_red);
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  const bool valid = t < a.n;
+  Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
+  int key = -1;
+  if (valid) {
+    const int64_t cap = a.cap;
+    double* hot = a.dst.hot;
+    Drop d = load_drop(hot, cap, t);
+    const double* hd = a.hand;
+    BeHand h;
+    h.Tf = hd[kHandTf * cap + t];
+    h.qinf = hd[kHandQ * cap + t];
+    h.rhoa = hd[kHandRho * cap + t];
+    h.diffnorm = hd[kHandDiff * cap + t];
+    h.Rep = hd[kHandRep * cap + t];
+    const double vold[3] = {hd[kHandVx * cap + t], hd[kHandVy * cap + t], hd[kHandVz * cap + t]};
+    const int kpt = a.hand_kpt[t];
+    const double Tpold = d.Tp, rold = d.r;
+    double qstar;
+    be_finish<PREC>(a.g, a.C, a.fl, s_z, d, h, qstar, tl);
+    double am[6];
+    coupling_amounts<PREC == PREC_MIXED>(a.C, a.fl, d, h.Tf, vold, Tpold, rold, a.dst.mult[t], am);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.amounts[k * cap + t] = am[k];
+    hot[kHZ * cap + t] = d.z;
+    hot[kHVZ * cap + t] = d.vz;
+    hot[kHTP * cap + t] = d.Tp;
+    hot[kHR * cap + t] = d.r;
+    hot[kHRES * cap + t] = d.res;
+    hot[kHACTRES * cap + t] = d.actres;
+    hot[kHNUMACT * cap + t] = d.numact;
+    const Cell c = cell_near(a.g, s_z, s_zz, d.x, d.y, d.z, kpt);
+    key = ukey_of(a.g, c);
+    a.keys[t] = key;
+    if (a.write_cold) a.cold[kCQSTAR * cap + a.dst.home[t]] = qstar;
+  }
+  count_key(a.counts, key, valid);
+  flush_tally(tl, a.tally, s_red);
 }
 
 // ------------------------------------------------- reference entry points
@@ -257,9 +394,9 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
 template <int MODE, int PREC, int ILIN>
 __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
-  __shared__ int s_hist[64];
+  __shared__ int s_red[kTallyWidth];
   stage_grid(a.g, s_z, s_zz);
-  if (threadIdx.x < 64) s_hist[threadIdx.x] = 0;
+  zero_tally_lds(s_red);
   __syncthreads();
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
   Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
@@ -279,8 +416,7 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
     store_gathered(a.cold, a.cap, home, o);
     store_dynamic(a.src.hot, a.cap, t, d);
   }
-  if (MODE == 2 && blockIdx.x == 0 && threadIdx.x == 0) a.tally[kTSteps] = 1ull;
-  if (MODE == 2) flush_tally(tl, a.tally, s_hist);
+  if (MODE == 2) flush_tally(tl, a.tally, s_red);
 }
 
 // uf_interp_lin / uf_interp for every droplet, optionally vp=uf and Tp=Tf.
@@ -318,8 +454,8 @@ __global__ void __launch_bounds__(kBlock) k_amounts(StepArgs a) {
   const int32_t h = a.src.home[t];
   const double vold[3] = {a.cold[kCVXOLD * a.cap + h], a.cold[kCVYOLD * a.cap + h], a.cold[kCVZOLD * a.cap + h]};
   double am[6];
-  coupling_amounts(a.C, a.fl, d, a.cold[kCTF * a.cap + h], vold, a.cold[kCTPOLD * a.cap + h],
-                   a.cold[kCRADIUSOLD * a.cap + h], a.src.mult[t], am);
+  coupling_amounts<false>(a.C, a.fl, d, a.cold[kCTF * a.cap + h], vold, a.cold[kCTPOLD * a.cap + h],
+                          a.cold[kCRADIUSOLD * a.cap + h], a.src.mult[t], am);
 #pragma unroll
   for (int k = 0; k < 6; ++k) a.amounts[k * a.cap + t] = am[k];
 }
@@ -328,6 +464,7 @@ __global__ void __launch_bounds__(kBlock) k_amounts(StepArgs a) {
 
 struct ScatterArgs {
   DevGrid g;
+  const int32_t* keys;    // u-cell keys of the positions (slot order)
   const double* hot;      // positions (slot order)
   const double* amounts;  // [6][cap]
   int64_t n, cap;
@@ -337,67 +474,100 @@ struct ScatterArgs {
 };
 
 // Deposit geometry of one droplet (particles.f90:1179-1212): node offsets of
-// its 8 corners in the source arrays and the weights wtt/dV.
+// its 8 corners in the source arrays and the weights wtt/dV.  The nodes hang
+// off the u cell (ipt, jpt, kpt) of the droplet's sort key (indices clamped
+// into [1,nnx+1] x [1,nny+1] x [0,nnz], which only positions outside the
+// domain need); node nnx+2 (x == xl exactly) is the periodic image of 2.
 struct Deposit {
   int key;
   int64_t node[8];
   double w[8];
 };
 
-__device__ __forceinline__ Deposit deposit_of(const DevGrid& g, const double* s_z, const double* s_zz,
-                                              double x, double y, double z) {
-  Deposit dp;
-  const Cell c = cell_of(g, s_z, s_zz, x, y, z);
-  dp.key = ukey_of(g, c);
-  const int kpt = clampi(c.kpt, 0, g.nnz);
-  const double dzu1 = g.dzu[kpt + 1];
-  const double dV = g.dx * g.dy * dzu1;
-  double wtx[2], wty[2], wtz[2];
+__device__ __forceinline__ void key_nodes(const DevGrid& g, int key, int64_t node[8]) {
+  const int nz1 = g.nnz + 1, ny1 = g.nny + 1;
+  const int k0 = key % nz1;
+  const int j0 = (key / nz1) % ny1;
+  const int i0 = key / (nz1 * ny1);
+  const int64_t sj = g.nnz + 2, si = (int64_t)(g.nnz + 2) * (g.nny + 1);
   int ix[2], iy[2];
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
-    wtx[k] = 1.0 - fabs(x - g.dx * (double)(k + c.ipt - 1)) / g.dx;
-    wty[k] = 1.0 - fabs(y - g.dy * (double)(k + c.jpt - 1)) / g.dy;
-    wtz[k] = 1.0 - fabs(z - s_zz[kpt + k]) / dzu1;
-    int xi = c.ipt + k, yj = c.jpt + k;
-    // the +1 ghost column holds node nnx+1; nnx+2 (x == xl exactly) is its
-    // periodic image 2.  Out-of-domain indices are clamped.
+    int xi = i0 + 1 + k, yj = j0 + 1 + k;
     if (xi > g.nnx + 1) xi -= g.nnx;
-    if (xi < 1) xi += g.nnx;
     if (yj > g.nny + 1) yj -= g.nny;
-    if (yj < 1) yj += g.nny;
-    ix[k] = clampi(xi, 1, g.nnx + 1);
-    iy[k] = clampi(yj, 1, g.nny + 1);
+    ix[k] = xi;
+    iy[k] = yj;
   }
-  const int64_t sj = g.nnz + 2, si = (int64_t)(g.nnz + 2) * (g.nny + 1);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const int c8 = (i * 2 + j) * 2 + k;
-        dp.node[c8] = (int64_t)(kpt + k) + sj * (iy[j] - 1) + si * (ix[i] - 1);
-        dp.w[c8] = wtx[i] * wty[j] * wtz[k] / dV;
-      }
+      for (int k = 0; k < 2; ++k) node[(i * 2 + j) * 2 + k] = (int64_t)(k0 + k) + sj * (iy[j] - 1) + si * (ix[i] - 1);
+}
+
+// weight wtt/dV of a droplet of u cell (ipt, jpt, kpt) at corner (ci, cj, ck),
+// with the reference's expression (particles.f90:1197-1205)
+__device__ __forceinline__ double corner_weight(const DevGrid& g, const double* s_zz, int ipt, int jpt, int kpt,
+                                                int ci, int cj, int ck, double x, double y, double z) {
+  const double dzu1 = g.dzu[kpt + 1];
+  const double dV = g.dx * g.dy * dzu1;
+  const double wtx = 1.0 - fabs(x - g.dx * (double)(ci + ipt - 1)) / g.dx;
+  const double wty = 1.0 - fabs(y - g.dy * (double)(cj + jpt - 1)) / g.dy;
+  const double wtz = 1.0 - fabs(z - s_zz[kpt + ck]) / dzu1;
+  return wtx * wty * wtz / dV;
+}
+
+// The deposit of a droplet in u cell `key` (the clamped cell; every in-domain
+// droplet's own cell).
+__device__ __forceinline__ Deposit deposit_key(const DevGrid& g, const double* s_zz, int key, double x,
+                                               double y, double z) {
+  Deposit dp;
+  dp.key = key;
+  const int nz1 = g.nnz + 1, ny1 = g.nny + 1;
+  const int kpt = key % nz1;
+  const int jpt = (key / nz1) % ny1 + 1;
+  const int ipt = key / (nz1 * ny1) + 1;
+  key_nodes(g, dp.key, dp.node);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) dp.w[(i * 2 + j) * 2 + k] = corner_weight(g, s_zz, ipt, jpt, kpt, i, j, k, x, y, z);
   return dp;
 }
 
+__device__ __forceinline__ Deposit deposit_of(const DevGrid& g, const double* s_z, const double* s_zz,
+                                              double x, double y, double z) {
+  return deposit_key(g, s_zz, ukey_of(g, cell_of(g, s_z, s_zz, x, y, z)), x, y, z);
+}
+
 // Two-way coupling scatter: every droplet deposits amount*wtt/dV on its 8
-// nodes.  Within a wave the droplets are cell sorted, so runs of equal u-cell
-// keys share all 8 nodes: a segmented inclusive scan (shuffles, log2 of the
-// longest run steps) sums each run into its last lane, which issues the FP64
-// global atomics once for the whole run.
+// nodes.  The droplets are cell sorted, so within a wave runs of equal u-cell
+// keys share all 8 nodes:
+//  1. a segmented inclusive scan (log2 of the longest run shuffle steps) sums
+//     each run into its last lane;
+//  2. the run sums are compacted to lanes 0..nseg-1 in key order, so that one
+//     atomic instruction per (field, corner) covers consecutive cells of a z
+//     column, i.e. consecutive addresses of the k-fastest source arrays;
+//  3. a run whose cell sits directly above the previous run's cell shares its
+//     bottom nodes with that run's top nodes, which are folded in first.
+// Each wave then issues at most 48 FP64 global atomic instructions.
 __global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  __shared__ int s_tail[kBlock];
   stage_grid(a.g, s_z, s_zz);
   __syncthreads();
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
   const bool valid = t < a.n;
   const int lane = lane_id();
+  int* my_tail = s_tail + (threadIdx.x & ~63);
   Deposit dp;
   if (valid) {
-    dp = deposit_of(a.g, s_z, s_zz, a.hot[kHX * a.cap + t], a.hot[kHY * a.cap + t], a.hot[kHZ * a.cap + t]);
+    const double x = a.hot[kHX * a.cap + t], y = a.hot[kHY * a.cap + t], z = a.hot[kHZ * a.cap + t];
+    dp = a.keys ? deposit_key(a.g, s_zz, a.keys[t], x, y, z) : deposit_of(a.g, s_z, s_zz, x, y, z);
   } else {
     dp.key = -1 - lane;
 #pragma unroll
@@ -406,19 +576,33 @@ __global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
       dp.w[c] = 0.0;
     }
   }
-  // segment bookkeeping
   const int prev = __shfl_up(dp.key, 1, 64);
   const bool head = (lane == 0) || (prev != dp.key);
   const unsigned long long heads = __ballot(head);
   const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
   const int seg_start = 63 - __clzll((long long)(heads & upto));
   const bool tail = (lane == 63) || ((heads >> (lane + 1)) & 1ull);
-  int len = lane - seg_start + 1;
-  int maxlen = len;
+  const unsigned long long tails = __ballot(tail && valid);
+  const int nseg = __popcll(tails);
+  if (tail && valid) my_tail[__popcll(tails & ((1ull << lane) - 1ull))] = lane;
+  int maxlen = lane - seg_start + 1;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, o, 64));
   int nsteps = 0;
   while ((1 << nsteps) < maxlen) ++nsteps;
+  __syncthreads();
+  // compacted run s lives in lane s
+  const bool act = lane < nseg;
+  const int src_lane = act ? my_tail[lane] : 0;
+  const int skey = __shfl(dp.key, src_lane, 64);
+  const int pkey = __shfl_up(skey, 1, 64);
+  const int nkey = __shfl_down(skey, 1, 64);
+  const int nz1 = a.g.nnz + 1;
+  // the cell directly below / above in the same z column
+  const bool below = act && lane > 0 && pkey == skey - 1 && (skey % nz1) != 0;
+  const bool above = act && lane + 1 < nseg && nkey == skey + 1 && (nkey % nz1) != 0;
+  int64_t node[8];
+  key_nodes(a.g, act ? skey : 0, node);
   for (int f = 0; f < 6; ++f) {
     if (!((a.fmask >> f) & 1u)) continue;
     const double am = valid ? a.amounts[f * a.cap + t] : 0.0;
@@ -434,17 +618,24 @@ __global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
         if (take) v[c] += o;
       }
     }
-    if (tail && valid) {
-      double* fld = a.src + (int64_t)f * a.nsrc;
+    double* fld = a.src + (int64_t)f * a.nsrc;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) atomicAdd(&fld[dp.node[c]], v[c]);
+    for (int col = 0; col < 4; ++col) {
+      double bot = __shfl(v[2 * col], src_lane, 64);
+      const double top = __shfl(v[2 * col + 1], src_lane, 64);
+      const double top_below = __shfl_up(top, 1, 64);
+      if (below) bot += top_below;
+      if (act) atomicAdd(&fld[node[2 * col]], bot);
+      if (act && !above) atomicAdd(&fld[node[2 * col + 1]], top);
     }
   }
 }
 
-// Deterministic coupling: each source node sums, in a fixed order, the
-// deposits of the droplets of the (up to 12) u cells that touch it, walking
-// the cell-sorted permutation.  Same result on every run.
+// Deterministic coupling without atomics: each source node (one thread; the
+// threads of a wave are consecutive k of one node column) sums, in a fixed
+// order, the deposits of the droplets of the u cells that touch it, walking
+// the fresh cell sort (offsets + perm).  Every node is written exactly once,
+// so the fields need no clearing, and the result is the same on every run.
 struct PullArgs {
   DevGrid g;
   const double* hot;
@@ -454,11 +645,12 @@ struct PullArgs {
   int64_t cap;
   double* src;
   int64_t nsrc;
+  unsigned fmask;
 };
 
 __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
-  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
-  stage_grid(a.g, s_z, s_zz);
+  __shared__ double s_zz[kMaxLevels];
+  for (int k = threadIdx.x; k < a.g.nnz + 2; k += blockDim.x) s_zz[k] = a.g.zz[k];
   __syncthreads();
   const int64_t node = xcd_block() * kBlock + threadIdx.x;
   if (node >= a.nsrc) return;
@@ -468,35 +660,33 @@ __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
   const int iy = (int)(rest % (a.g.nny + 1)) + 1;
   const int ix = (int)(rest / (a.g.nny + 1)) + 1;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  // candidate cells along x: (i0, corner i) with i0 = ix-1-i; node 2 also
-  // receives the wrapped corner of cell nnx (x == xl exactly)
+  // cells along x touching node ix: cell i0 = ix-1-ci at corner ci; node 2
+  // also takes corner 1 of cell nnx (its periodic image nnx+2)
   int cx[3], ci[3], ncx = 0, cy[3], cj[3], ncy = 0;
   for (int i = 0; i < 2; ++i) {
-    int i0 = ix - 1 - i;
+    const int i0 = ix - 1 - i;
     if (i0 >= 0 && i0 <= a.g.nnx) { cx[ncx] = i0; ci[ncx] = i; ++ncx; }
   }
   if (ix == 2) { cx[ncx] = a.g.nnx; ci[ncx] = 1; ++ncx; }
   for (int j = 0; j < 2; ++j) {
-    int j0 = iy - 1 - j;
+    const int j0 = iy - 1 - j;
     if (j0 >= 0 && j0 <= a.g.nny) { cy[ncy] = j0; cj[ncy] = j; ++ncy; }
   }
   if (iy == 2) { cy[ncy] = a.g.nny; cj[ncy] = 1; ++ncy; }
   for (int xa = 0; xa < ncx; ++xa)
     for (int ya = 0; ya < ncy; ++ya)
-      for (int k = 0; k < 2; ++k) {
-        const int k0 = kz - k;
+      for (int ck = 0; ck < 2; ++ck) {
+        const int k0 = kz - ck;
         if (k0 < 0 || k0 > a.g.nnz) continue;
         const int key = k0 + (a.g.nnz + 1) * (cy[ya] + (a.g.nny + 1) * cx[xa]);
         const int b = a.offsets[key], e = a.offsets[key + 1];
-        const int corner = (ci[xa] * 2 + cj[ya]) * 2 + k;
         for (int p = b; p < e; ++p) {
           const int64_t s = a.perm[p];
-          const Deposit dp = deposit_of(a.g, s_z, s_zz, a.hot[kHX * a.cap + s], a.hot[kHY * a.cap + s],
-                                        a.hot[kHZ * a.cap + s]);
-          if (dp.node[corner] != node) continue;  // clamped out-of-domain droplets
-          const double w = dp.w[corner];
+          const double w = corner_weight(a.g, s_zz, cx[xa] + 1, cy[ya] + 1, k0, ci[xa], cj[ya], ck,
+                                         a.hot[kHX * a.cap + s], a.hot[kHY * a.cap + s], a.hot[kHZ * a.cap + s]);
 #pragma unroll
-          for (int f = 0; f < 6; ++f) acc[f] += a.amounts[f * a.cap + s] * w;
+          for (int f = 0; f < 6; ++f)
+            if ((a.fmask >> f) & 1u) acc[f] += a.amounts[f * a.cap + s] * w;
         }
       }
 #pragma unroll

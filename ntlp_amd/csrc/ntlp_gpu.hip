@@ -70,9 +70,12 @@ struct Ctx {
   int32_t* pidx = nullptr;
   int32_t* procidx = nullptr;
   double* amounts = nullptr;
-  int32_t* keys = nullptr;
+  double* hand = nullptr;
+  int32_t* hand_kpt = nullptr;
+  int32_t* keys[2] = {};
   int32_t* perm = nullptr;
-  bool perm_valid = false;
+  bool perm_valid = false;  // perm sorts the current slots by u cell
+  bool keys_valid = false;  // keys[cur] are the cells of the current positions
   double* col_tmp = nullptr;
   // bins
   int32_t* counts = nullptr;
@@ -83,7 +86,8 @@ struct Ctx {
   double* src = nullptr;
   int64_t nsrc = 0;
   // counters
-  unsigned long long* tally = nullptr;
+  unsigned long long* tally = nullptr;        // kTallyRows per-step rows
+  unsigned long long* tally_slots = nullptr;  // kTallySlots x kTallyWidth
   int64_t step_row = 0;
   // timing
   bool timing = false;
@@ -238,7 +242,10 @@ void release_all() {
   dfree(ctx.pidx);
   dfree(ctx.procidx);
   dfree(ctx.amounts);
-  dfree(ctx.keys);
+  dfree(ctx.hand);
+  dfree(ctx.hand_kpt);
+  dfree(ctx.keys[0]);
+  dfree(ctx.keys[1]);
   dfree(ctx.perm);
   dfree(ctx.col_tmp);
   dfree(ctx.counts);
@@ -246,6 +253,7 @@ void release_all() {
   dfree(ctx.bsum);
   dfree(ctx.src);
   dfree(ctx.tally);
+  dfree(ctx.tally_slots);
   dfree(ctx.d_grid);
   for (auto& t : ctx.pending) {
     (void)hipEventDestroy(t.a);
@@ -286,9 +294,12 @@ StepArgs base_args() {
   a.perm = nullptr;
   a.cold = ctx.cold;
   a.amounts = ctx.amounts;
-  a.keys = ctx.keys;
+  a.keys_in = nullptr;
+  a.keys = ctx.keys[ctx.cur];
   a.counts = ctx.counts;
-  a.tally = ctx.tally + (ctx.step_row % kTallyRows) * kTallyWidth;
+  a.tally = ctx.tally_slots;
+  a.hand = ctx.hand;
+  a.hand_kpt = ctx.hand_kpt;
   return a;
 }
 
@@ -308,7 +319,7 @@ int scan_counts() {
 int finish_bin(const int32_t* tie) {
   CHK(scan_counts());
   CHK(launch("bin_scatter", [&] {
-    k_bin_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(ctx.keys, ctx.n, ctx.offsets, ctx.counts, ctx.perm);
+    k_bin_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(ctx.keys[ctx.cur], ctx.n, ctx.offsets, ctx.counts, ctx.perm);
   }));
   CHK(launch("bin_cellsort", [&] {
     k_cell_sort<<<grid_flat(ctx.ncell), kBlock, 0, ctx.stream>>>(ctx.offsets, ctx.ncell, ctx.perm, tie);
@@ -324,10 +335,11 @@ int bin_current(int32_t* cellidx_dev, const int32_t* tie) {
   b.hot = ctx.hot[ctx.cur];
   b.n = ctx.n;
   b.cap = ctx.cap;
-  b.keys = ctx.keys;
+  b.keys = ctx.keys[ctx.cur];
   b.counts = ctx.counts;
   b.cellidx = cellidx_dev;
   CHK(launch("bin_keys", [&] { k_keys<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(b); }));
+  ctx.keys_valid = true;
   return finish_bin(tie);
 }
 
@@ -343,6 +355,7 @@ int scatter_atomic() {
   CHK(fill_f64(ctx.src, 6 * ctx.nsrc, 0.0));
   ScatterArgs s;
   s.g = ctx.g;
+  s.keys = ctx.keys_valid ? ctx.keys[ctx.cur] : nullptr;
   s.hot = ctx.hot[ctx.cur];
   s.amounts = ctx.amounts;
   s.n = ctx.n;
@@ -353,8 +366,9 @@ int scatter_atomic() {
   return launch("scatter", [&] { k_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(s); });
 }
 
-int scatter_pull() {
-  CHK(bin_current(nullptr, nullptr));
+// atomic-free deterministic scatter over the current cell sort (perm and
+// offsets must describe the current slots)
+int pull_sorted() {
   PullArgs p;
   p.g = ctx.g;
   p.hot = ctx.hot[ctx.cur];
@@ -364,7 +378,13 @@ int scatter_pull() {
   p.cap = ctx.cap;
   p.src = ctx.src;
   p.nsrc = ctx.nsrc;
+  p.fmask = src_mask();
   return launch("scatter_pull", [&] { k_scatter_pull<<<grid8(ctx.nsrc), kBlock, 0, ctx.stream>>>(p); });
+}
+
+int scatter_pull() {
+  CHK(bin_current(nullptr, nullptr));
+  return pull_sorted();
 }
 
 template <int MODE, int PREC, int ILIN>
@@ -388,6 +408,30 @@ int dispatch_update(const StepArgs& a, int prec) {
   });
 }
 
+template <int PREC, int ILIN>
+void launch_be_t(const StepArgs& a, unsigned grid) {
+  k_be_advance<PREC, ILIN><<<grid, kBlock, 0, ctx.stream>>>(a);
+}
+template <int PREC>
+void launch_be_solve_t(const StepArgs& a, unsigned grid) {
+  k_be_solve<PREC><<<grid, kBlock, 0, ctx.stream>>>(a);
+}
+
+int dispatch_be_split(const StepArgs& a, int prec) {
+  const unsigned gr = grid8(a.n);
+  const bool lin = ctx.fl.ilin == 1;
+  CHK(launch("be_advance", [&] {
+    if (prec == PREC_FP64) lin ? launch_be_t<0, 1>(a, gr) : launch_be_t<0, 0>(a, gr);
+    else if (prec == PREC_MIXED) lin ? launch_be_t<1, 1>(a, gr) : launch_be_t<1, 0>(a, gr);
+    else lin ? launch_be_t<2, 1>(a, gr) : launch_be_t<2, 0>(a, gr);
+  }));
+  return launch("be_solve", [&] {
+    if (prec == PREC_FP64) launch_be_solve_t<0>(a, gr);
+    else if (prec == PREC_MIXED) launch_be_solve_t<1>(a, gr);
+    else launch_be_solve_t<2>(a, gr);
+  });
+}
+
 template <int MODE>
 int dispatch_step(const StepArgs& a, int prec) {
   const unsigned gr = grid8(a.n);
@@ -400,11 +444,12 @@ int dispatch_step(const StepArgs& a, int prec) {
   });
 }
 
-int bump_step_row() {
-  // count the step in its row and open the next row (cleared before use)
+// closes the current step: slots -> a fresh per-step row
+int close_step() {
   ctx.step_row++;
   unsigned long long* row = ctx.tally + (ctx.step_row % kTallyRows) * kTallyWidth;
-  return launch("fill", [&] { k_fill_u64<<<1, kBlock, 0, ctx.stream>>>(row, kTallyWidth); });
+  unsigned long long* slots = ctx.tally_slots;
+  return launch("tally", [&] { k_tally_close<<<1, 128, 0, ctx.stream>>>(slots, row, 1); });
 }
 
 }  // namespace
@@ -487,6 +532,13 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   }
   C.Sc13 = std::pow(c->Sc, 1. / 3.);
   C.Pra13 = std::pow(c->Pra, 1. / 3.);
+  C.LvMw_Ru = c->Lv * c->Mw / c->Ru;
+  C.Mw2Gam = 2. * c->Mw * c->Gam;
+  C.Ru_rhow = c->Ru * c->rhow;
+  C.Lv3 = 3. * c->Lv;
+  C.Mw_Ru = c->Mw / c->Ru;
+  C.Mw2Gam_Ru_rhow = 2. * c->Mw * c->Gam / (c->Ru * c->rhow);
+  C.isol = 1.0 / ((2.0 / 3.0) * c->pi2 * c->rhos);
   DevFlags& fl = ctx.fl;
   fl.ilin = f->ilin;
   fl.icouple = f->icouple;
@@ -503,11 +555,13 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   ctx.nsrc = (int64_t)(nz + 2) * (grid->nny + 1) * (grid->nnx + 1);
   CHK(dalloc(ctx.src, 6 * ctx.nsrc, "coupling sources"));
   CHK(dalloc(ctx.tally, kTallyRows * kTallyWidth, "counters"));
+  CHK(dalloc(ctx.tally_slots, kTallySlots * kTallyWidth, "counter slots"));
   ctx.ext_elems = (int64_t)(nz + 2) * (grid->nny + 5) * (grid->nnx + 5);
   ctx.init = true;
   CHK(fill_i32(ctx.counts, ctx.ncell, 0));
   CHK(fill_f64(ctx.src, 6 * ctx.nsrc, 0.0));
   CHK(launch("fill", [&] { k_fill_u64<<<64, kBlock, 0, ctx.stream>>>(ctx.tally, kTallyRows * kTallyWidth); }));
+  CHK(launch("fill", [&] { k_fill_u64<<<8, kBlock, 0, ctx.stream>>>(ctx.tally_slots, kTallySlots * kTallyWidth); }));
   HIPCHK(hipStreamSynchronize(ctx.stream));
   return NTLP_OK;
 }
@@ -584,7 +638,10 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
     CHK(dalloc(ctx.pidx, n, "pidx"));
     CHK(dalloc(ctx.procidx, n, "procidx"));
     CHK(dalloc(ctx.amounts, 6 * n, "coupling amounts"));
-    CHK(dalloc(ctx.keys, n, "keys"));
+    CHK(dalloc(ctx.hand, kHand * n, "advance hand-over"));
+    CHK(dalloc(ctx.hand_kpt, n, "advance hand-over"));
+    CHK(dalloc(ctx.keys[0], n, "keys"));
+    CHK(dalloc(ctx.keys[1], n, "keys"));
     CHK(dalloc(ctx.perm, n, "perm"));
     CHK(dalloc(ctx.col_tmp, n, "column staging"));
     ctx.cap = n;
@@ -593,6 +650,7 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
   ctx.cur = 0;
   ctx.home_identity = true;
   ctx.perm_valid = false;
+  ctx.keys_valid = false;
   if (n == 0) return NTLP_OK;
   for (int c = 0; c < NTLP_NREAL; ++c) {
     double* dst = hot_of(c) >= 0 ? ctx.hot[0] + (int64_t)hot_of(c) * ctx.cap : ctx.cold + (int64_t)cold_of(c) * ctx.cap;
@@ -738,6 +796,7 @@ int ntlp_gpu_particle_update_rk3_prec(int32_t istage, int32_t it, int32_t prec) 
   CHK(need_drops());
   CHK(check_prec(prec));
   if (istage < 1 || istage > 3) return fail(NTLP_E_INVALID, "istage must be 1, 2 or 3");
+  ctx.keys_valid = false;
   StepArgs a = base_args();
   a.istage = istage;
   a.it = it;
@@ -753,10 +812,11 @@ int ntlp_gpu_particle_update_rk3(int32_t istage, int32_t it) {
 int ntlp_gpu_particle_update_be_prec(int32_t it, int32_t prec) {
   CHK(need_drops());
   CHK(check_prec(prec));
-  CHK(bump_step_row());
+  ctx.keys_valid = false;
   StepArgs a = base_args();
   a.it = it;
   CHK(dispatch_update<2>(a, prec));
+  CHK(close_step());
   HIPCHK(hipStreamSynchronize(ctx.stream));
   return NTLP_OK;
 }
@@ -863,20 +923,21 @@ int ntlp_gpu_step(int32_t mode, int32_t prec, int32_t nsteps, int32_t it0) {
   CHK(check_prec(prec));
   if (mode != NTLP_MODE_RK3 && mode != NTLP_MODE_BE) return fail(NTLP_E_INVALID, "unknown mode");
   if (nsteps < 1) return fail(NTLP_E_INVALID, "nsteps must be >= 1");
-  if (!ctx.perm_valid) {
-    CHK(launch("iota", [&] { k_iota<<<grid_flat(ctx.n), kBlock, 0, ctx.stream>>>(ctx.perm, ctx.n); }));
-    ctx.perm_valid = true;
-  }
+  // the first step of a call reads the droplets cell sorted too
+  if (!ctx.perm_valid || !ctx.keys_valid) CHK(bin_current(nullptr, nullptr));
   for (int k = 0; k < nsteps; ++k) {
-    CHK(bump_step_row());
     StepArgs a = base_args();
     a.perm = ctx.perm;
+    a.keys_in = ctx.keys[ctx.cur];
+    a.keys = ctx.keys[1 - ctx.cur];
     a.it = it0 + k;
     a.write_cold = (k == nsteps - 1) ? 1 : 0;
     if (mode == NTLP_MODE_RK3) CHK(dispatch_step<1>(a, prec));
-    else CHK(dispatch_step<2>(a, prec));
+    else CHK(dispatch_be_split(a, prec));
     ctx.cur = 1 - ctx.cur;
     ctx.home_identity = false;
+    ctx.keys_valid = true;
+    CHK(close_step());
     CHK(scatter_atomic());
     CHK(finish_bin(nullptr));
   }
@@ -906,6 +967,7 @@ int ntlp_gpu_counters(ntlp_counters* out, int32_t reset) {
   }
   if (reset) {
     CHK(launch("fill", [&] { k_fill_u64<<<64, kBlock, 0, ctx.stream>>>(ctx.tally, kTallyRows * kTallyWidth); }));
+    CHK(launch("fill", [&] { k_fill_u64<<<8, kBlock, 0, ctx.stream>>>(ctx.tally_slots, kTallySlots * kTallyWidth); }));
     HIPCHK(hipStreamSynchronize(ctx.stream));
     ctx.step_row = 0;
   }

@@ -26,6 +26,7 @@ struct Diag {
   double uf[3], Tf, qinf, qstar;
   double vold[3], Tpold, rold;
   double xrhs[3], vrhs[3], Tprhs_s, Tprhs_L, radrhs;
+  int kpt;  // u level of the gather position
 };
 
 // Per-thread Newton bookkeeping (particles.f90:3035-3039, 3128, 3147-3174).
@@ -75,11 +76,24 @@ __device__ __forceinline__ void apply_bcs(const DevGrid& g, const double* s_z, D
 
 // ----------------------------------------------------------- Newton solve
 
+// FP64 reciprocal: hardware estimate + two Newton-Raphson refinements.
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Inputs of ie_vrt_nd that stay fixed during one solve.
 struct NewtonIn {
   double r, Tp, Tf, qinf, m_s, kappa, rhoa, esa, taup0, h, diffnorm;
+  // reference-order invariants: 18*rhoa*nuf, (kappa*m_s*rhow)/rhos, taup0/r,
+  // r/taup0, taup0/Tp, 1/Tf
+  double c18, ksol, t0_r, r_t0, t0_Tp, iTf;
 };
 
-// ie_vrt_nd (particles.f90:4263-4327), reference operation order.
+// ie_vrt_nd (particles.f90:4263-4327) in the reference's operation order.
 __device__ __forceinline__ void resid_ref(const DevConsts& C, const NewtonIn& P, double tempr,
                                           double tempt, double F[2]) {
   const double rnext = tempr * P.r;
@@ -88,115 +102,115 @@ __device__ __forceinline__ void resid_ref(const DevConsts& C, const NewtonIn& P,
   const double VolP = (2. / 3.) * C.pi2 * (rnext * rnext * rnext);
   const double rhop = (P.m_s + VolP * C.rhow) / VolP;
   const double Rep = dnext * P.diffnorm / C.nuf;
-  const double taup = (rhop * (dnext * dnext)) / (18.0 * P.rhoa * C.nuf);
+  const double taup = (rhop * (dnext * dnext)) / P.c18;
   const double qstr = (C.Mw / (C.Ru * Tnext * P.rhoa)) * P.esa *
-                      exp(((C.Lv * C.Mw / C.Ru) * ((1. / P.Tf) - (1. / Tnext))) +
-                          ((2. * C.Mw * C.Gam) / (C.Ru * C.rhow * rnext * Tnext)) -
-                          ((P.kappa * P.m_s * C.rhow / C.rhos) / (VolP * rhop - P.m_s)));
+                      exp((C.LvMw_Ru * (P.iTf - (1. / Tnext))) + (C.Mw2Gam / (C.Ru_rhow * rnext * Tnext)) -
+                          (P.ksol / (VolP * rhop - P.m_s)));
   const double sRe = sqrt(Rep);
   const double Shp = 2. + 0.6 * sRe * C.Sc13;
   double rprime = (1. / 9.) * (Shp / C.Sc) * (rhop / C.rhow) * (rnext / taup) * (P.qinf - qstr);
-  rprime = rprime * (P.taup0 / P.r);
+  rprime = rprime * P.t0_r;
   const double Nup = 2. + 0.6 * sRe * C.Pra13;
   double Tprime = -((1. / 3.) * (Nup / C.Pra) * C.CpaCpp * (rhop / C.rhow) * (1. / taup) * (Tnext - P.Tf)) +
-                  3. * C.Lv * (1. / (rnext * C.Cpp)) * rprime * (P.r / P.taup0);
-  Tprime = Tprime * (P.taup0 / P.Tp);
+                  C.Lv3 * (1. / (rnext * C.Cpp)) * rprime * P.r_t0;
+  Tprime = Tprime * P.t0_Tp;
   F[0] = rnext / P.r - 1.0 - P.h * rprime;
   F[1] = Tnext / P.Tp - 1.0 - P.h * Tprime;
 }
 
-// jacob_approx_2d (4224-4252): forward differences with step 1e-12; f0 is the
-// residual at (tr, tt), which the reference evaluates as well.
-__device__ __forceinline__ void jacob_fd(const DevConsts& C, const NewtonIn& P, double tr, double tt,
-                                         const double f0[2], double J[2][2]) {
-  const double diff = 1e-12;
-  double fx[2];
-  resid_ref(C, P, tr + diff, tt, fx);
-  J[0][0] = (fx[0] - f0[0]) / diff;
-  J[1][0] = (fx[1] - f0[1]) / diff;
-  resid_ref(C, P, tr, tt + diff, fx);
-  J[0][1] = (fx[0] - f0[0]) / diff;
-  J[1][1] = (fx[1] - f0[1]) / diff;
-}
+// Reference residual + jacob_approx_2d (4224-4252): forward differences with
+// step 1e-12 around the residual, which the reference evaluates twice.
+struct RefEval {
+  const DevConsts& C;
+  const NewtonIn& P;
+  static constexpr bool kSquare = false;  // reference (J^T J)^-1 J^T step
+  __device__ __forceinline__ void f(double tr, double tt, double F[2]) const { resid_ref(C, P, tr, tt, F); }
+  __device__ __forceinline__ void fj(double tr, double tt, double F[2], double J[2][2]) const {
+    const double diff = 1e-12;
+    double fx[2];
+    resid_ref(C, P, tr, tt, F);
+    resid_ref(C, P, tr + diff, tt, fx);
+    J[0][0] = (fx[0] - F[0]) / diff;
+    J[1][0] = (fx[1] - F[1]) / diff;
+    resid_ref(C, P, tr, tt + diff, fx);
+    J[0][1] = (fx[0] - F[0]) / diff;
+    J[1][1] = (fx[1] - F[1]) / diff;
+  }
+};
 
-// Residual and exact Jacobian of the same equations.  With rn = tr*r,
-// Tn = tt*Tp the reference's groupings reduce to
-//   rprime = A*Sh*(qinf - q*)/rn,          A = ρa ν taup0 / (2 Sc ρw r)
-//   Tprime = -B*Nu*(Tn - Tf)/rn^2 + Cc*rprime/rn,
-//            B = 3/2 (Cpa/Cpp) ρa ν taup0 / (Pr ρw Tp),  Cc = 3 Lv r / (Cpp Tp)
-//   q*     = Mw esa / (Ru ρa Tn) * exp(E),
-//   E      = Lv Mw/Ru (1/Tf - 1/Tn) + 2 Mw Γ/(Ru ρw rn Tn) - κ m_s/(ρs V(rn))
-// (using ρp V - m_s = ρw V), differentiated in rn and Tn.
-__device__ __forceinline__ void resid_jac(const DevConsts& C, const NewtonIn& P, double tr, double tt,
-                                          double F[2], double J[2][2]) {
-  const double rn = tr * P.r, Tn = tt * P.Tp;
-  const double irn = 1.0 / rn, iTn = 1.0 / Tn;
-  const double V = (2. / 3.) * C.pi2 * (rn * rn * rn);
-  const double curv = (2. * C.Mw * C.Gam) / (C.Ru * C.rhow) * irn * iTn;
-  const double solu = (P.kappa * P.m_s / C.rhos) / V;
-  const double LMR = C.Lv * C.Mw / C.Ru;
-  const double E = LMR * ((Tn - P.Tf) / (P.Tf * Tn)) + curv - solu;
-  const double qs = (C.Mw / (C.Ru * P.rhoa)) * P.esa * iTn * exp(E);
-  const double dE_dr = -curv * irn + 3.0 * solu * irn;
-  const double dE_dT = LMR * iTn * iTn - curv * iTn;
-  const double dq_dr = qs * dE_dr;
-  const double dq_dT = qs * (dE_dT - iTn);
-  const double sRe = sqrt(2.0 * rn * P.diffnorm / C.nuf);
-  const double Sh = 2. + 0.6 * sRe * C.Sc13;
-  const double Nu = 2. + 0.6 * sRe * C.Pra13;
-  const double dSh_dr = 0.3 * sRe * C.Sc13 * irn;
-  const double dNu_dr = 0.3 * sRe * C.Pra13 * irn;
-  const double A = P.rhoa * C.nuf * P.taup0 / (2.0 * C.Sc * C.rhow * P.r);
-  const double B = 1.5 * C.CpaCpp * P.rhoa * C.nuf * P.taup0 / (C.Pra * C.rhow * P.Tp);
-  const double Cc = 3.0 * C.Lv * P.r / (C.Cpp * P.Tp);
-  const double dq = P.qinf - qs;
-  const double rp = A * Sh * dq * irn;
-  const double drp_dr = A * (dSh_dr * dq * irn - Sh * dq_dr * irn - Sh * dq * irn * irn);
-  const double drp_dT = -A * Sh * dq_dT * irn;
-  const double dT = Tn - P.Tf;
-  const double irn2 = irn * irn;
-  const double Tpr = -B * Nu * dT * irn2 + Cc * rp * irn;
-  const double dTp_dr = -B * (dNu_dr * dT * irn2 - 2.0 * Nu * dT * irn2 * irn) + Cc * (drp_dr * irn - rp * irn2);
-  const double dTp_dT = -B * Nu * irn2 + Cc * drp_dT * irn;
-  F[0] = tr - 1.0 - P.h * rp;
-  F[1] = tt - 1.0 - P.h * Tpr;
-  J[0][0] = 1.0 - P.h * P.r * drp_dr;
-  J[0][1] = -P.h * P.Tp * drp_dT;
-  J[1][0] = -P.h * P.r * dTp_dr;
-  J[1][1] = 1.0 - P.h * P.Tp * dTp_dT;
-}
+// The same equations in closed form with their exact Jacobian.  With
+// rn = tr*r, Tn = tt*Tp the reference's groupings reduce to
+//   h*rprime = X = hA*Sh*(qinf - q*)/rn,       hA = dt ρa ν / (2 Sc ρw r)
+//   h*Tprime = Y = -hB*Nu*(Tn - Tf)/rn^2 + Cc*X/rn,
+//              hB = 3/2 dt (Cpa/Cpp) ρa ν / (Pr ρw Tp),  Cc = 3 Lv r / (Cpp Tp)
+//   q* = K/Tn * exp(E),  K = Mw esa / (Ru ρa),
+//   E  = Lv Mw/Ru (Tn - Tf)/(Tf Tn) + 2 Mw Γ/(Ru ρw rn Tn) - κ m_s/(ρs V(rn))
+// (using ρp V - m_s = ρw V; taup0 cancels), F = (tr - 1 - X, tt - 1 - Y).
+struct AJIn {
+  double r, Tp, Tf, qinf, K, csol, LMRiTf, sq, hA, hB, Cc, sc6, pr6;
+};
 
-__device__ __forceinline__ void resid_fast(const DevConsts& C, const NewtonIn& P, double tr, double tt,
-                                           double F[2]) {
-  double J[2][2];
-  resid_jac(C, P, tr, tt, F, J);
-}
+struct AJEval {
+  const DevConsts& C;
+  const AJIn& P;
+  static constexpr bool kSquare = true;  // J^-1 F, equal to the reference step
+  __device__ __forceinline__ void eval(double tr, double tt, double F[2], double J[2][2], bool jac) const {
+    const double rn = tr * P.r, Tn = tt * P.Tp;
+    const double irn = rcp64(rn), iTn = rcp64(Tn);
+    const double irn2 = irn * irn;
+    const double s = P.sq * sqrt(rn);  // sqrt(Re)
+    const double Sh = 2.0 + P.sc6 * s, Nu = 2.0 + P.pr6 * s;
+    const double curv = C.Mw2Gam_Ru_rhow * irn * iTn;
+    const double solu = P.csol * irn2 * irn;
+    const double dT = Tn - P.Tf;
+    const double E = P.LMRiTf * dT * iTn + curv - solu;
+    const double qs = P.K * iTn * exp(E);
+    const double dq = P.qinf - qs;
+    const double X = P.hA * Sh * dq * irn;
+    const double Y = -P.hB * Nu * dT * irn2 + P.Cc * X * irn;
+    F[0] = tr - 1.0 - X;
+    F[1] = tt - 1.0 - Y;
+    if (!jac) return;
+    const double dE_r = (3.0 * solu - curv) * irn;
+    const double dE_T = (C.LvMw_Ru * iTn - curv) * iTn;
+    const double dqs_r = qs * dE_r, dqs_T = qs * (dE_T - iTn);
+    const double dSh = 0.5 * (Sh - 2.0) * irn, dNu = 0.5 * (Nu - 2.0) * irn;
+    const double dX_r = P.hA * irn * (dSh * dq - Sh * dqs_r - Sh * dq * irn);
+    const double dX_T = -P.hA * Sh * dqs_T * irn;
+    const double dY_r = -P.hB * irn2 * (dNu * dT - 2.0 * Nu * dT * irn) + P.Cc * irn * (dX_r - X * irn);
+    const double dY_T = -P.hB * Nu * irn2 + P.Cc * dX_T * irn;
+    J[0][0] = 1.0 - P.r * dX_r;
+    J[0][1] = -P.Tp * dX_T;
+    J[1][0] = -P.r * dY_r;
+    J[1][1] = 1.0 - P.Tp * dY_T;
+  }
+  __device__ __forceinline__ void f(double tr, double tt, double F[2]) const {
+    double J[2][2];
+    eval(tr, tt, F, J, false);
+  }
+  __device__ __forceinline__ void fj(double tr, double tt, double F[2], double J[2][2]) const {
+    eval(tr, tt, F, J, true);
+  }
+};
 
-// gauss_newton_2d (4071-4142).  AJ selects the analytic Jacobian.
-template <bool AJ>
-__device__ __noinline__ int gauss_newton(const DevConsts& C, const NewtonIn& P, const double vec1[2],
-                                         double vec2[2], int& iters) {
+// gauss_newton_2d (4071-4142): damped Gauss-Newton, backtracking while a
+// component is negative or NaN, failure below 0.01 or on NaN, convergence on
+// the undamped correction norm < 1e-8, at most 50 iterations.
+template <class Ev>
+__device__ __forceinline__ int gauss_newton(const Ev& ev, const double vec1[2], double vec2[2], int& iters) {
   int iterations = 0, flag = 0;
-  const double error = 1.0e-8;
   double v1[2] = {vec1[0], vec1[1]};
-  double correct[2] = {0.0, 0.0};
   vec2[0] = v1[0];
   vec2[1] = v1[1];
   while (iterations < 50) {
     iterations++;
-    double fv1[2], J[2][2];
-    if (AJ) {
-      resid_jac(C, P, v1[0], v1[1], fv1, J);
-    } else {
-      resid_ref(C, P, v1[0], v1[1], fv1);
-      jacob_fd(C, P, v1[0], v1[1], fv1, J);
-    }
-    double fJ[2][2];
-    if (AJ) {
-      // (J^T J)^-1 J^T = J^-1 for the square system; det(J^T J) = det(J)^2.
+    double fv1[2], J[2][2], fJ[2][2];
+    ev.fj(v1[0], v1[1], fv1, J);
+    if constexpr (Ev::kSquare) {
+      // (J^T J)^-1 J^T = J^-1 for the square system; det(J^T J) = det(J)^2
       const double detJ = J[0][0] * J[1][1] - J[0][1] * J[1][0];
       if (fabs(detJ * detJ) < 1.0e-10) { flag = 1; break; }
-      const double s = 1.0 / detJ;
+      const double s = rcp64(detJ);
       fJ[0][0] = s * J[1][1];
       fJ[0][1] = -s * J[0][1];
       fJ[1][0] = -s * J[1][0];
@@ -217,21 +231,21 @@ __device__ __noinline__ int gauss_newton(const DevConsts& C, const NewtonIn& P, 
       fJ[1][0] = inv10 * J[0][0] + inv11 * J[0][1];
       fJ[1][1] = inv10 * J[1][0] + inv11 * J[1][1];
     }
-    correct[0] = fJ[0][0] * fv1[0] + fJ[0][1] * fv1[1];
-    correct[1] = fJ[1][0] * fv1[0] + fJ[1][1] * fv1[1];
-    vec2[0] = v1[0] - correct[0];
-    vec2[1] = v1[1] - correct[1];
+    const double c0 = fJ[0][0] * fv1[0] + fJ[0][1] * fv1[1];
+    const double c1 = fJ[1][0] * fv1[0] + fJ[1][1] * fv1[1];
+    vec2[0] = v1[0] - c0;
+    vec2[1] = v1[1] - c1;
     int counts = 0;
     double relax = 1.0;
     while ((vec2[0] < 0) || (vec2[1] < 0) || isnan(vec2[0])) {
       counts++;
       relax = relax * 0.5;
-      vec2[0] = v1[0] - correct[0] * relax;
-      vec2[1] = v1[1] - correct[1] * relax;
+      vec2[0] = v1[0] - c0 * relax;
+      vec2[1] = v1[1] - c1 * relax;
       if (counts > 10) break;
     }
     if (vec2[0] < 0.01 || vec2[1] < 0.01 || isnan(vec2[0]) || isnan(vec2[1])) { flag = 1; break; }
-    if (sqrt(correct[0] * correct[0] + correct[1] * correct[1]) < error) break;
+    if (Ev::kSquare ? (c0 * c0 + c1 * c1 < 1.0e-16) : (sqrt(c0 * c0 + c1 * c1) < 1.0e-8)) break;
     v1[0] = vec2[0];
     v1[1] = vec2[1];
   }
@@ -241,25 +255,18 @@ __device__ __noinline__ int gauss_newton(const DevConsts& C, const NewtonIn& P, 
   return flag;
 }
 
-// LV_solver (4143-4223).  vec2 keeps what the failed Gauss-Newton left, as the
-// shared actual argument rt_zeroes does in the reference.
-template <bool AJ>
-__device__ __noinline__ int lv_solver(const DevConsts& C, const NewtonIn& P, const double vec1[2],
-                                      double vec2[2], int& iters) {
+// LV_solver (4143-4223): Levenberg-Marquardt, lambda 1e-3 halved on success
+// and doubled otherwise, merit |(F1^2/2, F2^2/2)|.  vec2 keeps what the failed
+// Gauss-Newton left, as the shared actual argument rt_zeroes does.
+template <class Ev>
+__device__ __forceinline__ int lv_solver(const Ev& ev, const double vec1[2], double vec2[2], int& iters) {
   int iterations = 0, flag = 0;
-  const double error = 1.0e-8;
   double v1[2] = {vec1[0], vec1[1]};
   double lambda = 0.001;
-  const double lup = 2.0, ldown = 2.0;
   while (iterations < 1000) {
     iterations++;
     double J[2][2], fv1[2];
-    if (AJ) {
-      resid_jac(C, P, v1[0], v1[1], fv1, J);
-    } else {
-      resid_ref(C, P, v1[0], v1[1], fv1);
-      jacob_fd(C, P, v1[0], v1[1], fv1, J);
-    }
+    ev.fj(v1[0], v1[1], fv1, J);
     double gm[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -276,18 +283,17 @@ __device__ __noinline__ int lv_solver(const DevConsts& C, const NewtonIn& P, con
     const double c1 = i10 * gradC0 + i11 * gradC1;
     vec2[0] = v1[0] - c0;
     vec2[1] = v1[1] - c1;
-    if (sqrt(c0 * c0 + c1 * c1) < error) break;
+    if (sqrt(c0 * c0 + c1 * c1) < 1.0e-8) break;
     if (vec2[0] < 0.0 || vec2[1] < 0.0 || isnan(vec2[0]) || isnan(vec2[1])) { flag = 1; break; }
     double fv2[2];
-    if (AJ) resid_fast(C, P, vec2[0], vec2[1], fv2);
-    else resid_ref(C, P, vec2[0], vec2[1], fv2);
+    ev.f(vec2[0], vec2[1], fv2);
     const double n0 = 0.5 * fv2[0] * fv2[0], n1 = 0.5 * fv2[1] * fv2[1];
     if (sqrt(n0 * n0 + n1 * n1) < sqrt(C0 * C0 + C1 * C1)) {
       v1[0] = vec2[0];
       v1[1] = vec2[1];
-      lambda = lambda / ldown;
+      lambda = lambda / 2.0;
     } else {
-      lambda = lambda * lup;
+      lambda = lambda * 2.0;
     }
   }
   if (iterations == 1000) flag = 1;
@@ -296,15 +302,14 @@ __device__ __noinline__ int lv_solver(const DevConsts& C, const NewtonIn& P, con
   return flag;
 }
 
-// rad_solver2 (4328-4371): Koehler equilibrium radius, Cardano on
-// r^3 + a r^2 + c = 0.
-__device__ __forceinline__ double rad_solver2(const DevConsts& C, const NewtonIn& P, double esa,
-                                              int& mflag) {
+// rad_solver2 (4328-4371): Koehler equilibrium radius from Cardano on
+// r^3 + a r^2 + c = 0, in FP64.
+__device__ __forceinline__ double rad_solver2(const DevConsts& C, double Tf, double rhoa, double qinf,
+                                              double kappa, double m_s, double r, double esa, int& mflag) {
   mflag = 0;
-  const double Tf = P.Tf;
-  const double L = log((C.Ru * Tf * P.rhoa * P.qinf) / (C.Mw * esa));
+  const double L = log((C.Ru * Tf * rhoa * qinf) / (C.Mw * esa));
   const double a = -((2 * C.Mw * C.Gam) / (C.Ru * C.rhow * Tf) / L);
-  const double c = (P.kappa * P.m_s) / ((2.0 / 3.0) * C.pi2 * C.rhos) / L;
+  const double c = (kappa * m_s) / ((2.0 / 3.0) * C.pi2 * C.rhos) / L;
   const double Q = (a * a) / 9.0;
   const double R = (2.0 * (a * a * a) + 27.0 * c) / 54.0;
   const double Q3 = Q * Q * Q;
@@ -319,45 +324,62 @@ __device__ __forceinline__ double rad_solver2(const DevConsts& C, const NewtonIn
     const double T = Q / S;
     guess = S + T - a / 3.0;
     if (guess < 0) {
-      guess = P.r;
+      guess = r;
       mflag = 1;
     }
   }
   return guess;
 }
 
+// The same initial guess in FP32 (mixed precision; the guess only seeds the
+// FP64 solve).  Radii in micrometres keep a^3, c and R^2 inside the FP32
+// range; ln S is formed as log1p(S - 1) from the FP64 difference.
+__device__ __forceinline__ double rad_solver2_f32(const DevConsts& C, double Tf, double rhoa, double qinf,
+                                                  double kappa, double m_s, double r, double esa, int& mflag) {
+  mflag = 0;
+  const double Sm1 = (C.Ru * Tf * rhoa * qinf) / (C.Mw * esa) - 1.0;
+  const float L = log1pf((float)Sm1);
+  const float a = -(float)(C.Mw2Gam_Ru_rhow * 1e6 / Tf) / L;
+  const float c = (float)(kappa * m_s * C.isol * 1e18) / L;
+  const float Q = (a * a) * (1.0f / 9.0f);
+  const float R = (2.0f * (a * a * a) + 27.0f * c) * (1.0f / 54.0f);
+  const float Q3 = Q * Q * Q;
+  const float M = R * R - Q3;
+  const float pi2 = (float)C.pi2;
+  float guess;
+  if (M < 0.0f) {
+    const float theta = acosf(R / sqrtf(Q3));
+    const float sQ = sqrtf(Q);
+    guess = -(2.0f * sQ * cosf((theta - pi2) * (1.0f / 3.0f))) - a * (1.0f / 3.0f);
+    if (guess < 0.0f) guess = -(2.0f * sQ * cosf((theta + pi2) * (1.0f / 3.0f))) - a * (1.0f / 3.0f);
+  } else {
+    const float S = -copysignf(1.0f, R) * cbrtf(fabsf(R) + sqrtf(M));
+    guess = S + Q / S - a * (1.0f / 3.0f);
+    if (guess < 0.0f) {
+      mflag = 1;
+      return r;
+    }
+  }
+  return (double)guess * 1e-6;
+}
+
 // Radius/temperature block of particle_update_BE (3103-3178) on a droplet
 // whose velocity is already the new one.  Updates d.r, d.Tp, activation
-// bookkeeping and the tally.  AJ: analytic Jacobian.  WARM (mixed precision
-// only): start from whichever of the Koehler guess and the unchanged state
-// (1, Tf/Tp) has the smaller residual.
-template <bool AJ, bool WARM>
+// bookkeeping and the tally.  PREC_FP64: reference residual and FD
+// Jacobian.  Otherwise the closed-form residual with its exact Jacobian; in
+// mixed precision the Koehler guess is formed in FP32 and the solve starts
+// from whichever of that guess and the unchanged state (1, Tf/Tp) has the
+// smaller residual.
+template <int PREC>
 __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFlags& fl, Drop& d,
-                                                double Tf, double qinf, double rhoa, double ufx,
-                                                double ufy, double ufz, double Rep, double rhop,
-                                                double taup_i, double esa, Tally& tl) {
+                                                double Tf, double qinf, double rhoa, double diffnorm,
+                                                double Rep, double esa, Tally& tl) {
   const double dt = C.dt;
-  const double ri = C.radius_init;
   if (fl.ievap == 1 && qinf > 0.0) {
-    NewtonIn P;
-    P.r = d.r;
-    P.Tp = d.Tp;
-    P.Tf = Tf;
-    P.qinf = qinf;
-    P.m_s = d.m_s;
-    P.kappa = d.kappa;
-    P.rhoa = rhoa;
-    P.esa = esa;
-    P.taup0 = (((d.m_s) / ((2. / 3.) * C.pi2 * (ri * ri * ri)) + C.rhow) * ((ri * 2) * (ri * 2))) /
-              (18 * rhoa * C.nuf);
-    P.h = dt / P.taup0;
-    {
-      const double a0 = ufx - d.vx, a1 = ufy - d.vy, a2 = ufz - d.vz;
-      P.diffnorm = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
-    }
-    int mflag;
-    const double guess = rad_solver2(C, P, esa, mflag);
+    int mflag, flag, it1 = 0, it2 = 0;
     double rt_start[2], rt_zeroes[2];
+    double guess = (PREC == PREC_MIXED) ? rad_solver2_f32(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag)
+                                        : rad_solver2(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
     if (mflag == 0) {
       rt_start[0] = guess / d.r;
       rt_start[1] = Tf / d.Tp;
@@ -365,27 +387,74 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
       rt_start[0] = 1.0;
       rt_start[1] = 1.0;
     }
-    if (WARM) {
-      const double alt[2] = {1.0, Tf / d.Tp};
-      double fa[2], fb[2];
-      resid_fast(C, P, rt_start[0], rt_start[1], fa);
-      resid_fast(C, P, alt[0], alt[1], fb);
-      const double na = fa[0] * fa[0] + fa[1] * fa[1], nb = fb[0] * fb[0] + fb[1] * fb[1];
-      if (!(na <= nb)) {
-        rt_start[0] = alt[0];
-        rt_start[1] = alt[1];
+    if (PREC == PREC_FP64) {
+      const double ri = C.radius_init;
+      NewtonIn P;
+      P.r = d.r;
+      P.Tp = d.Tp;
+      P.Tf = Tf;
+      P.qinf = qinf;
+      P.m_s = d.m_s;
+      P.kappa = d.kappa;
+      P.rhoa = rhoa;
+      P.esa = esa;
+      P.taup0 = (((d.m_s) / ((2. / 3.) * C.pi2 * (ri * ri * ri)) + C.rhow) * ((ri * 2) * (ri * 2))) /
+                (18 * rhoa * C.nuf);
+      P.h = dt / P.taup0;
+      P.diffnorm = diffnorm;
+      P.c18 = 18.0 * rhoa * C.nuf;
+      P.ksol = d.kappa * d.m_s * C.rhow / C.rhos;
+      P.t0_r = P.taup0 / d.r;
+      P.r_t0 = d.r / P.taup0;
+      P.t0_Tp = P.taup0 / d.Tp;
+      P.iTf = 1. / Tf;
+      const RefEval ev{C, P};
+      flag = gauss_newton(ev, rt_start, rt_zeroes, it1);
+      tl.gn_iters += it1;
+      tl.solves += 1;
+      tl.hist = it1;
+      if (flag == 1) {
+        tl.num100 += 1;
+        tl.hist = 0;
+        flag = lv_solver(ev, rt_start, rt_zeroes, it2);
+        tl.lm_iters += it2;
       }
-    }
-    int it1 = 0, it2 = 0;
-    int flag = gauss_newton<AJ>(C, P, rt_start, rt_zeroes, it1);
-    tl.gn_iters += it1;
-    tl.solves += 1;
-    tl.hist = it1;
-    if (flag == 1) {
-      tl.num100 += 1;
-      tl.hist = 0;
-      flag = lv_solver<AJ>(C, P, rt_start, rt_zeroes, it2);
-      tl.lm_iters += it2;
+    } else {
+      AJIn P;
+      P.r = d.r;
+      P.Tp = d.Tp;
+      P.Tf = Tf;
+      P.qinf = qinf;
+      P.K = C.Mw_Ru * esa / rhoa;
+      P.csol = d.kappa * d.m_s * C.isol;
+      P.LMRiTf = C.LvMw_Ru / Tf;
+      P.sq = sqrt(2.0 * diffnorm / C.nuf);
+      P.hA = 0.5 * dt * rhoa * C.nuf / (C.Sc * C.rhow * d.r);
+      P.hB = 1.5 * dt * C.CpaCpp * rhoa * C.nuf / (C.Pra * C.rhow * d.Tp);
+      P.Cc = C.Lv3 * d.r / (C.Cpp * d.Tp);
+      P.sc6 = 0.6 * C.Sc13;
+      P.pr6 = 0.6 * C.Pra13;
+      const AJEval ev{C, P};
+      if (PREC == PREC_MIXED) {
+        const double alt[2] = {1.0, Tf / d.Tp};
+        double fa[2], fb[2];
+        ev.f(rt_start[0], rt_start[1], fa);
+        ev.f(alt[0], alt[1], fb);
+        if (!(fa[0] * fa[0] + fa[1] * fa[1] <= fb[0] * fb[0] + fb[1] * fb[1])) {
+          rt_start[0] = alt[0];
+          rt_start[1] = alt[1];
+        }
+      }
+      flag = gauss_newton(ev, rt_start, rt_zeroes, it1);
+      tl.gn_iters += it1;
+      tl.solves += 1;
+      tl.hist = it1;
+      if (flag == 1) {
+        tl.num100 += 1;
+        tl.hist = 0;
+        flag = lv_solver(ev, rt_start, rt_zeroes, it2);
+        tl.lm_iters += it2;
+      }
     }
     if ((flag == 1) || isnan(rt_zeroes[0]) || (rt_zeroes[0] * d.r < 0) || isnan(rt_zeroes[1]) ||
         (rt_zeroes[1] < 0) || (rt_zeroes[0] * d.r > 1.0e-2)) {
@@ -403,6 +472,10 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
     d.r = rt_zeroes[0] * d.r;
     d.Tp = rt_zeroes[1] * d.Tp;
   } else {
+    const double r = d.r;
+    const double Volp = C.pi2 * 2.0 / 3.0 * (r * r * r);
+    const double rhop = (d.m_s + Volp * C.rhow) / Volp;
+    const double taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
     const double Nup = 2.0 + 0.6 * sqrt(Rep) * C.Pra13;
     const double tmp_coeff = Nup / 3.0 / C.Pra * C.CpaCpp * rhop / C.rhow * taup_i;
     d.Tp = (d.Tp + tmp_coeff * dt * Tf) / (1 + dt * tmp_coeff);
@@ -434,19 +507,38 @@ __device__ __forceinline__ double qstar_f32(const DevConsts& C, const Drop& d, d
 
 // ---------------------------------------------------------- BE droplet
 
-// One droplet of particle_update_BE (3043-3206) plus its boundary
-// conditions; PREC chooses the arithmetic (see the header comment).
+// particle_bcs_periodic alone (2814-2843); it does not depend on the radius,
+// so it may run before the wall reflection of the same step.
+__device__ __forceinline__ void apply_periodic(const DevGrid& g, Drop& d) {
+  if (d.x > g.xl) d.x = d.x - g.xl;
+  else if (d.x < 0) d.x = g.xl + d.x;
+  if (d.y > g.yl) d.y = d.y - g.yl;
+  else if (d.y < 0) d.y = g.yl + d.y;
+}
+
+// What the radius/temperature solve of one droplet needs from the advance.
+struct BeHand {
+  double Tf, qinf, rhoa;
+  double diffnorm;  // |u_f - v| with the new velocity (ie_vrt_nd's Reynolds number)
+  double Rep;       // Reynolds number of the pre-step velocity (no-evaporation branch)
+};
+
+// First half of one droplet of particle_update_BE (3047-3100): interpolation,
+// it<=1 velocity set-up, air density, then the implicit velocity and the
+// position advanced with the old velocity.  PREC_MIXED: drag and the velocity
+// update in FP32, positions in FP64.
 template <int PREC, int ILIN>
-__device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
-                                           const DevFields& F, const double* s_z, const double* s_zz,
-                                           Drop& d, int it, Diag& o, Tally& tl) {
+__device__ __forceinline__ BeHand be_advance(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
+                                             const DevFields& F, const double* s_z, const double* s_zz,
+                                             Drop& d, int it, Diag& o, int kguess) {
   const double dt = C.dt;
   o.vold[0] = d.vx;
   o.vold[1] = d.vy;
   o.vold[2] = d.vz;
   o.Tpold = d.Tp;
   o.rold = d.r;
-  const Cell c = cell_of(g, s_z, s_zz, d.x, d.y, d.z);
+  const Cell c = kguess >= 0 ? cell_near(g, s_z, s_zz, d.x, d.y, d.z, kguess) : cell_of(g, s_z, s_zz, d.x, d.y, d.z);
+  o.kpt = c.kpt;
   Fluid f;
   if (ILIN != 1) f = gather6_f64(g, F, s_z, s_zz, d.x, d.y, d.z, c);
   else if (PREC == PREC_MIXED) f = gather_lin_mixed<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
@@ -456,34 +548,38 @@ __device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C,
     d.vy = f.u[1];
     d.vz = f.u[2];
   }
+  BeHand h;
   double Tf = f.T;
-  const double rhoa = air_rho(C, fl, d.z, Tf);
+  h.rhoa = air_rho(C, fl, d.z, Tf);
+  h.Tf = Tf;
+  h.qinf = f.q;
   const double r = d.r;
-  double Rep, rhop, taup_i;
   if (PREC == PREC_MIXED) {
-    // drag and the implicit velocity update in FP32, position in FP64
     const float rf = (float)r;
     const float a0 = (float)(d.vx - f.u[0]), a1 = (float)(d.vy - f.u[1]), a2 = (float)(d.vz - f.u[2]);
     const float Volp = (float)(C.pi2 * 2.0 / 3.0) * rf * rf * rf;
     const float rhopf = ((float)d.m_s + Volp * (float)C.rhow) / Volp;
-    const float tpi = (float)(18.0 * rhoa * C.nuf) / rhopf / ((2.0f * rf) * (2.0f * rf));
-    Rep = (double)(2.0f * rf * sqrtf(a0 * a0 + a1 * a1 + a2 * a2) / (float)C.nuf);
-    rhop = rhopf;
-    taup_i = tpi;
+    const float tpi = (float)(18.0 * h.rhoa * C.nuf) / rhopf / ((2.0f * rf) * (2.0f * rf));
+    h.Rep = (double)(2.0f * rf * sqrtf(a0 * a0 + a1 * a1 + a2 * a2) / (float)C.nuf);
     d.x = d.x + dt * d.vx;
     d.y = d.y + dt * d.vy;
     d.z = d.z + dt * d.vz;
     const float dtf = (float)dt, den = 1.0f + dtf * tpi;
-    d.vx = (double)(((float)d.vx + tpi * dtf * (float)f.u[0] + dtf * (float)C.g[0]) / den);
-    d.vy = (double)(((float)d.vy + tpi * dtf * (float)f.u[1] + dtf * (float)C.g[1]) / den);
-    d.vz = (double)(((float)d.vz + tpi * dtf * (float)f.u[2] + dtf * (float)C.g[2]) / den);
+    const float nx = ((float)d.vx + tpi * dtf * (float)f.u[0] + dtf * (float)C.g[0]) / den;
+    const float ny = ((float)d.vy + tpi * dtf * (float)f.u[1] + dtf * (float)C.g[1]) / den;
+    const float nz = ((float)d.vz + tpi * dtf * (float)f.u[2] + dtf * (float)C.g[2]) / den;
+    d.vx = nx;
+    d.vy = ny;
+    d.vz = nz;
+    const float b0 = (float)f.u[0] - nx, b1 = (float)f.u[1] - ny, b2 = (float)f.u[2] - nz;
+    h.diffnorm = (double)sqrtf(b0 * b0 + b1 * b1 + b2 * b2);
   } else {
     const double a0 = d.vx - f.u[0], a1 = d.vy - f.u[1], a2 = d.vz - f.u[2];
     const double diffnorm = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
     const double Volp = C.pi2 * 2.0 / 3.0 * (r * r * r);
-    rhop = (d.m_s + Volp * C.rhow) / Volp;
-    taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
-    Rep = 2.0 * r * diffnorm / C.nuf;
+    const double rhop = (d.m_s + Volp * C.rhow) / Volp;
+    const double taup_i = 18.0 * h.rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
+    h.Rep = 2.0 * r * diffnorm / C.nuf;
     d.x = d.x + dt * d.vx;
     d.y = d.y + dt * d.vy;
     d.z = d.z + dt * d.vz;
@@ -491,23 +587,39 @@ __device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C,
     d.vx = (d.vx + taup_i * dt * 1.0 * f.u[0] + dt * C.g[0]) / den;
     d.vy = (d.vy + taup_i * dt * 1.0 * f.u[1] + dt * C.g[1]) / den;
     d.vz = (d.vz + taup_i * dt * 1.0 * f.u[2] + dt * C.g[2]) / den;
+    const double b0 = f.u[0] - d.vx, b1 = f.u[1] - d.vy, b2 = f.u[2] - d.vz;
+    h.diffnorm = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
   }
-  const double esa = mod_magnus(Tf);
-  if (PREC == PREC_FP64)
-    be_microphysics<false, false>(C, fl, d, Tf, f.q, rhoa, f.u[0], f.u[1], f.u[2], Rep, rhop, taup_i, esa, tl);
-  else if (PREC == PREC_FP64_AJ)
-    be_microphysics<true, false>(C, fl, d, Tf, f.q, rhoa, f.u[0], f.u[1], f.u[2], Rep, rhop, taup_i, esa, tl);
-  else
-    be_microphysics<true, true>(C, fl, d, Tf, f.q, rhoa, f.u[0], f.u[1], f.u[2], Rep, rhop, taup_i, esa, tl);
-  o.qstar = (PREC == PREC_MIXED) ? qstar_f32(C, d, Tf, rhoa, esa) : qstar_f64(C, d, Tf, rhoa, esa);
-  d.res = d.res + dt;
-  d.actres = d.actres + dt;
   o.uf[0] = f.u[0];
   o.uf[1] = f.u[1];
   o.uf[2] = f.u[2];
   o.Tf = Tf;
   o.qinf = f.q;
+  return h;
+}
+
+// Second half (3103-3206 and the wall reflection of particle_bcs_nonperiodic):
+// radius/temperature solve, q*, residence times, then walls at the new radius.
+template <int PREC>
+__device__ __forceinline__ void be_finish(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
+                                          const double* s_z, Drop& d, const BeHand& h, double& qstar,
+                                          Tally& tl) {
+  const double dt = C.dt;
+  const double esa = mod_magnus(h.Tf);
+  be_microphysics<PREC>(C, fl, d, h.Tf, h.qinf, h.rhoa, h.diffnorm, h.Rep, esa, tl);
+  qstar = (PREC == PREC_MIXED) ? qstar_f32(C, d, h.Tf, h.rhoa, esa) : qstar_f64(C, d, h.Tf, h.rhoa, esa);
+  d.res = d.res + dt;
+  d.actres = d.actres + dt;
   apply_bcs(g, s_z, d);
+}
+
+// One droplet of particle_update_BE (3043-3206) plus its boundary conditions.
+template <int PREC, int ILIN>
+__device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
+                                           const DevFields& F, const double* s_z, const double* s_zz,
+                                           Drop& d, int it, Diag& o, Tally& tl, int kguess = -1) {
+  const BeHand h = be_advance<PREC, ILIN>(g, C, fl, F, s_z, s_zz, d, it, o, kguess);
+  be_finish<PREC>(g, C, fl, s_z, d, h, o.qstar, tl);
 }
 
 // ---------------------------------------------------------- RK3 droplet
@@ -520,7 +632,7 @@ __device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C,
 template <int PREC, int ILIN>
 __device__ __forceinline__ void rk3_droplet(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
                                             const DevFields& F, const double* s_z, const double* s_zz,
-                                            Drop& d, int istage, int it, Diag& o) {
+                                            Drop& d, int istage, int it, Diag& o, int kguess = -1) {
   const double dt = C.dt;
   const double zs = C.zetas[istage - 1], gs = C.gama[istage - 1];
   if (istage == 1) {
@@ -530,7 +642,8 @@ __device__ __forceinline__ void rk3_droplet(const DevGrid& g, const DevConsts& C
     o.Tpold = d.Tp;
     o.rold = d.r;
   }
-  const Cell c = cell_of(g, s_z, s_zz, d.x, d.y, d.z);
+  const Cell c = kguess >= 0 ? cell_near(g, s_z, s_zz, d.x, d.y, d.z, kguess) : cell_of(g, s_z, s_zz, d.x, d.y, d.z);
+  o.kpt = c.kpt;
   Fluid f;
   if (ILIN != 1) f = gather6_f64(g, F, s_z, s_zz, d.x, d.y, d.z, c);
   else if (PREC == PREC_MIXED) f = gather_lin_mixed<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
@@ -631,32 +744,54 @@ __device__ __forceinline__ void rk3_droplet(const DevGrid& g, const DevConsts& C
 
 // Per-droplet source amounts of particle_coupling_update (1187-1264): the
 // node contribution is amount * wtt / dV.  a[0..2] momentum (icouple),
-// a[3] heat (iTcouple), a[4] vapour and a[5] enthalpy (iHcouple).
+// a[3] heat (iTcouple), a[4] vapour and a[5] enthalpy (iHcouple).  FAST
+// (mixed precision) replaces the divisions by refined reciprocals; the sums
+// stay FP64.
+template <bool FAST>
 __device__ __forceinline__ void coupling_amounts(const DevConsts& C, const DevFlags& fl, const Drop& d,
                                                  double Tf, const double vold[3], double Tpold,
                                                  double rold, int64_t mult, double a[6]) {
   const double dt = C.dt;
-  const double vrhs0 = (d.vx - vold[0]) / dt, vrhs1 = (d.vy - vold[1]) / dt, vrhs2 = (d.vz - vold[2]) / dt;
+  const double idt = FAST ? rcp64(dt) : 0.0;
   const double r = d.r;
-  const double radrhs = (r - rold) / dt;
-  const double Tprhs_L = 3.0 * C.Lv / C.Cpp / r * radrhs;
-  const double Tprhs_s = (d.Tp - Tpold) / dt - Tprhs_L;
-  double rhoa;
-  if (fl.iexner == 1) rhoa = func_p_base(C, d.z) / C.Rd / Tf;
-  else rhoa = C.surf_rho;
+  double vrhs0, vrhs1, vrhs2, radrhs, Tprhs_L, Tprhs_s, rhoa, rhop, taup_i;
   const double V = C.pi2 * 2.0 / 3.0 * (r * r * r);
-  const double rhop = (d.m_s + V * C.rhow) / V;
+  if (FAST) {
+    vrhs0 = (d.vx - vold[0]) * idt;
+    vrhs1 = (d.vy - vold[1]) * idt;
+    vrhs2 = (d.vz - vold[2]) * idt;
+    radrhs = (r - rold) * idt;
+    const double ir = rcp64(r);
+    Tprhs_L = C.Lv3 / C.Cpp * ir * radrhs;
+    Tprhs_s = (d.Tp - Tpold) * idt - Tprhs_L;
+    rhoa = (fl.iexner == 1) ? func_p_base(C, d.z) / C.Rd / Tf : C.surf_rho;
+    rhop = d.m_s * rcp64(V) + C.rhow;
+    taup_i = 4.5 * rhoa * C.nuf * rcp64(rhop) * ir * ir;
+  } else {
+    vrhs0 = (d.vx - vold[0]) / dt;
+    vrhs1 = (d.vy - vold[1]) / dt;
+    vrhs2 = (d.vz - vold[2]) / dt;
+    radrhs = (r - rold) / dt;
+    Tprhs_L = 3.0 * C.Lv / C.Cpp / r * radrhs;
+    Tprhs_s = (d.Tp - Tpold) / dt - Tprhs_L;
+    rhoa = (fl.iexner == 1) ? func_p_base(C, d.z) / C.Rd / Tf : C.surf_rho;
+    rhop = (d.m_s + V * C.rhow) / V;
+    taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
+  }
+  const double irhoa = FAST ? rcp64(rhoa) : 1.0 / rhoa;
   const double partmass = rhop * 2.0 / 3.0 * C.pi2 * (r * r * r);
-  const double taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
   const double rm = (double)mult;
-  const double vap = C.rhow / rhoa * C.pi2 * 2 * (r * r) * radrhs;
+  const double vap = C.rhow * irhoa * C.pi2 * 2 * (r * r) * radrhs;
   for (int k = 0; k < 6; ++k) a[k] = 0.0;
   if (fl.icouple == 1) {
-    a[0] = -(partmass / rhoa * (vrhs0 - C.g[0]) + vap * d.vx) * rm;
-    a[1] = -(partmass / rhoa * (vrhs1 - C.g[1]) + vap * d.vy) * rm;
-    a[2] = -(partmass / rhoa * (vrhs2 - C.g[2]) + vap * d.vz) * rm;
+    a[0] = -(partmass * irhoa * (vrhs0 - C.g[0]) + vap * d.vx) * rm;
+    a[1] = -(partmass * irhoa * (vrhs1 - C.g[1]) + vap * d.vy) * rm;
+    a[2] = -(partmass * irhoa * (vrhs2 - C.g[2]) + vap * d.vz) * rm;
   }
-  if (fl.iTcouple == 1) a[3] = -(Tprhs_s * 6.0 * C.rhow / rhop / C.CpaCpp / taup_i * (C.pi2 / 2.0) * r * C.nuf) * rm;
+  if (fl.iTcouple == 1) {
+    const double itp = FAST ? rcp64(rhop * C.CpaCpp * taup_i) : 1.0 / (rhop * C.CpaCpp * taup_i);
+    a[3] = -(Tprhs_s * 6.0 * C.rhow * itp * (C.pi2 / 2.0) * r * C.nuf) * rm;
+  }
   if (fl.iHcouple == 1) {
     a[4] = -vap * rm;
     a[5] = (-vap * C.Cpv / C.Cpa * d.Tp + vap * C.Cpv / C.Cpa * Tf) * rm;

@@ -739,7 +739,11 @@ void or_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const 
     load_rec(soa, n, ip, p);
     int32_t ci[5];
     or_cell_index(g, p[OR_X], p[OR_Y], p[OR_Z], ci);
-    int ipt = ci[0], jpt = ci[1], kpt = clampi(ci[2], 0, g->nnz);
+    /* the deposit hangs off the droplet's u cell clamped into
+       [1,nnx+1] x [1,nny+1] x [0,nnz], which only changes positions outside
+       the domain (where the reference writes outside its arrays) */
+    int ipt = clampi(ci[0], 1, g->nnx + 1), jpt = clampi(ci[1], 1, g->nny + 1), kpt = clampi(ci[2], 0, g->nnz);
+    int ipn = ipt, jpn = jpt;
     double vrhs[3];
     for (int d = 0; d < 3; ++d) vrhs[d] = (p[OR_VX + d] - p[OR_VXOLD + d]) / dt;
     double r = p[OR_RADIUS];
@@ -767,11 +771,9 @@ void or_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const 
                         (c->pi2 * 2.0 / 3.0 * (r * r * r));
           double partmass = rhop * 2.0 / 3.0 * c->pi2 * (r * r * r);
           double taup_i = 18.0 * rhoa * c->nuf / rhop / ((2.0 * r) * (2.0 * r));
-          int ix = ipt + i, iy = jpt + j, iz = kpt + k;
+          int ix = ipn + i, iy = jpn + j, iz = kpt + k;
           if (ix > g->nnx + 1) ix -= g->nnx;
-          if (ix < 1) ix += g->nnx;
           if (iy > g->nny + 1) iy -= g->nny;
-          if (iy < 1) iy += g->nny;
           int64_t o = src_off(g, iz, iy, ix);
           if (f->icouple == 1) {
             for (int d = 0; d < 3; ++d) {

@@ -1,0 +1,334 @@
+"""CPU pins of the oracle (no GPU): each check compares the oracle with
+something other than its own formula -- numpy's searchsorted/argsort,
+polynomial exactness, closed forms, an mpmath root of the BE equations
+(tests/golden/be_roots.json), conservation and hand-worked boundary cases."""
+from __future__ import annotations
+
+import json
+import os
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from ntlp_amd import COL, NREAL, Consts, Flags, Particles
+from ntlp_amd.grid import Grid
+from oracle import Oracle
+
+from .helpers import edge_positions, small_grid
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def ora_for(grid, consts=None, flags=None):
+    return Oracle(grid, consts or Consts(), flags or Flags())
+
+
+# ------------------------------------------------------------ cell indices
+
+@pytest.mark.parametrize("kind", ["uniform", "stretched"])
+def test_cell_index_matches_searchsorted(kind):
+    g = small_grid(kind)
+    rng = np.random.default_rng(0)
+    pts = np.concatenate([edge_positions(g),
+                          np.column_stack([rng.uniform(0, g.xl, 500), rng.uniform(0, g.yl, 500),
+                                           rng.uniform(-0.1 * g.zl, 1.2 * g.zl, 500)])])
+    ci = ora_for(g).cell_index(pts[:, 0], pts[:, 1], pts[:, 2])
+    len_ = g.nnz + 2
+    # first index with zz > z == number of zz <= z
+    pzz = np.searchsorted(g.zz, pts[:, 2], side="right")
+    pz = np.searchsorted(g.z, pts[:, 2], side="right")
+    assert np.array_equal(ci[:, 0], np.floor(pts[:, 0] / g.dx).astype(int) + 1)
+    assert np.array_equal(ci[:, 1], np.floor(pts[:, 1] / g.dy).astype(int) + 1)
+    assert np.array_equal(ci[:, 2], np.where(pzz == len_, -2, pzz - 1))
+    assert np.array_equal(ci[:, 3], np.where(pz == len_, -2, pz - 1))
+    assert np.array_equal(ci[:, 4], np.where(pz == len_, -1, pz))
+
+
+def test_counting_sort_is_a_stable_argsort():
+    g = small_grid("stretched")
+    rng = np.random.default_rng(1)
+    n = 5000
+    x, y, z = rng.uniform(0, g.xl, n), rng.uniform(0, g.yl, n), rng.uniform(0, g.zl, n)
+    keys, counts, offsets, perm = ora_for(g).bin(x, y, z)
+    assert np.array_equal(perm, np.argsort(keys, kind="stable"))
+    assert np.array_equal(counts, np.bincount(keys, minlength=len(counts)))
+    assert offsets[0] == 0 and offsets[-1] == n and np.array_equal(np.diff(offsets), counts)
+
+
+# ----------------------------------------------------------- interpolation
+
+def node_coords(g):
+    """x, y of ext nodes (i = -1..nnx+3: x = dx (i-1)) and the z levels."""
+    xi = g.dx * (np.arange(-1, g.nnx + 4) - 1.0)
+    yj = g.dy * (np.arange(-1, g.nny + 4) - 1.0)
+    return xi, yj
+
+
+def ext_from(g, fn, levels):
+    xi, yj = node_coords(g)
+    X, Y, Z = np.meshgrid(xi, yj, levels, indexing="ij")
+    return fn(X, Y, Z)
+
+
+def interp_points(o, g, pts):
+    p = Particles.empty(len(pts))
+    p["x"], p["y"], p["z"] = pts[:, 0], pts[:, 1], pts[:, 2]
+    o.interp(p.soa)
+    return p
+
+
+@pytest.mark.parametrize("kind", ["uniform", "stretched"])
+def test_trilinear_is_exact_for_affine_fields(kind):
+    g = small_grid(kind)
+    aff = lambda X, Y, Z: 0.3 + 1.7 * X - 0.9 * Y + 2.3 * Z
+    o = ora_for(g)
+    ext_uv = ext_from(g, aff, g.zz)
+    ext_w = ext_from(g, aff, g.z)
+    o.set_fields(ext_uv, ext_uv, ext_w, ext_uv, ext_uv, ext=True)
+    rng = np.random.default_rng(2)
+    n = 400
+    # interior in z: between zz(1) and zz(nnz), where no wall override applies
+    pts = np.column_stack([rng.uniform(0, g.xl, n), rng.uniform(0, g.yl, n), rng.uniform(g.zz[1], g.zz[g.nnz], n)])
+    p = interp_points(o, g, pts)
+    want = aff(pts[:, 0], pts[:, 1], pts[:, 2])
+    for c in ("ufx", "ufy", "ufz", "Tf", "qinf"):
+        assert np.max(np.abs(p[c] - want)) < 1e-12, c
+
+
+def test_trilinear_wall_override_takes_node_ipt1_jpt1():
+    """particles.f90:831-842: between the last u level and the wall (and below
+    the first), u, v, T and q are the values of node (ipt+1, jpt+1) at level
+    kpt (nnz) or 1; w keeps its trilinear value."""
+    g = small_grid("uniform")
+    o = ora_for(g)
+    rng = np.random.default_rng(3)
+    exts = [rng.standard_normal((g.nnx + 5, g.nny + 5, g.nnz + 2)) for _ in range(5)]
+    o.set_fields(*exts, ext=True)
+    xq, yq = 4.3 * g.dx, 6.6 * g.dy
+    ipt, jpt = 5, 7
+    for zq, level in ((0.5 * (g.zz[g.nnz] + g.z[g.nnz]), g.nnz), (0.5 * g.zz[1], 1)):
+        p = interp_points(o, g, np.array([[xq, yq, zq]]))
+        ix, iy = ipt + 1 + 1, jpt + 1 + 1  # ext index offset: i = -1 is row 0
+        assert p["ufx"][0] == exts[0][ix, iy, level]
+        assert p["ufy"][0] == exts[1][ix, iy, level]
+        assert p["Tf"][0] == exts[3][ix, iy, level]
+        assert p["qinf"][0] == exts[4][ix, iy, level]
+        assert p["ufz"][0] != exts[2][ix, iy, level]
+
+
+def test_sixth_order_is_exact_for_quintics_in_the_interior():
+    g = small_grid("stretched")
+    o = ora_for(g, flags=Flags(ilin=0))
+    poly = lambda X, Y, Z: 1 + X ** 5 - 3 * X ** 2 * Y ** 3 + Y ** 4 * Z + 7 * Z ** 5 - X * Y * Z ** 3
+    ext_uv = ext_from(g, poly, g.zz)
+    ext_w = ext_from(g, poly, g.z)
+    o.set_fields(ext_uv, ext_uv, ext_w, ext_uv, ext_uv, ext=True)
+    rng = np.random.default_rng(4)
+    n = 200
+    pts = np.column_stack([rng.uniform(0, g.xl, n), rng.uniform(0, g.yl, n),
+                           rng.uniform(g.z[4], g.z[g.nnz - 4], n)])
+    p = interp_points(o, g, pts)
+    want = poly(pts[:, 0], pts[:, 1], pts[:, 2])
+    for c in ("ufx", "ufz", "Tf"):
+        assert np.max(np.abs(p[c] - want)) < 1e-9 * np.max(np.abs(want)), c
+
+
+# ----------------------------------------------------------- thermodynamics
+
+def test_magnus_at_freezing_point():
+    from oracle import lib
+    assert lib().or_mod_magnus(273.15) == 610.94
+
+
+def test_crit_radius_within_one_table_step_of_closed_form():
+    """r_c = sqrt(3B/A) maximises A/r - B/r^3 (Koehler without the exponent)."""
+    c = Consts()
+    o = ora_for(small_grid())
+    for ms in (1e-19, 5e-18, 1e-17, 1e-15):
+        p = Particles.empty(1)
+        p["m_s"], p["kappa_s"], p["Tp"] = ms, 1.2, 295.0
+        rc = o.crit_radius(p.soa, COL)[0]
+        A = 2 * c.Mw * c.Gam / (c.Ru * c.rhow * 295.0)
+        B = 1.2 * ms / (c.rhos * 4.0 / 3.0 * np.pi)
+        assert abs(np.log10(rc / np.sqrt(3 * B / A))) <= 5.0 / 1024 + 1e-12
+
+
+def test_koehler_guess_solves_the_equilibrium_cubic():
+    """rad_solver2's guess r satisfies ln S = A/r - B/r^3 (checked at 50 digits)."""
+    c = Consts(surf_rho=1.2)
+    o = ora_for(small_grid(), c)
+    mp.mp.dps = 50
+    # below the critical supersaturation the haze branch has an equilibrium;
+    # 1.0003 is above S_c = 1 + sqrt(4A^3/(27B)) ~ 1.000035 for m_s = 1e-15
+    # kg, where the reference falls back to the current radius (mflag = 1)
+    for rh, ms, has_eq in ((0.999, 5e-18, True), (0.9995, 1e-17, True), (0.99, 1e-18, True),
+                           (1.00001, 1e-15, True), (1.0003, 1e-15, False)):
+        Tf = 290.0
+        es = o.L.or_mod_magnus(Tf)
+        q = rh * c.Mw / c.Ru / Tf * es / c.surf_rho
+        rec = np.zeros(NREAL)
+        rec[COL["Tf"]], rec[COL["qinf"]], rec[COL["m_s"]], rec[COL["kappa_s"]], rec[COL["radius"]] = Tf, q, ms, 1.2, 1e-6
+        g, mflag = o.rad_solver2(rec, c.surf_rho)
+        if not has_eq:
+            assert mflag == 1 and g == 1e-6
+            continue
+        assert mflag == 0 and g > 0, (rh, ms)
+        A = mp.mpf(2 * c.Mw * c.Gam) / (c.Ru * c.rhow * Tf)
+        B = mp.mpf(1.2 * ms) / (mp.mpf(4) / 3 * mp.pi * c.rhos)
+        lnS = mp.log(mp.mpf(c.Ru) * Tf * c.surf_rho * q / (c.Mw * es))
+        resid = A / g - B / mp.mpf(g) ** 3 - lnS
+        assert abs(resid) <= 1e-9 * (abs(A / g) + abs(B / mp.mpf(g) ** 3)), (rh, ms, resid)
+
+
+# ------------------------------------------------------- Newton vs mpmath
+
+def golden_roots():
+    with open(os.path.join(HERE, "golden", "be_roots.json")) as fh:
+        return json.load(fh)["roots"]
+
+
+@pytest.mark.parametrize("radius_init", [1.5e-6, 1e-5])
+def test_newton_matches_mpmath_roots(radius_init):
+    """The oracle's Koehler guess + Gauss-Newton (+ LM) lands on the
+    50-digit root of the BE equations, for any radius_init (taup0 cancels)."""
+    c = Consts(surf_rho=1.2, radius_init=radius_init)
+    o = ora_for(small_grid(), c)
+    ri = radius_init
+    for s in golden_roots():
+        rec = np.zeros(NREAL)
+        rec[COL["radius"]], rec[COL["Tp"]], rec[COL["Tf"]], rec[COL["qinf"]] = s["r"], s["Tp"], s["Tf"], s["qinf"]
+        rec[COL["m_s"]], rec[COL["kappa_s"]], rec[COL["ufx"]] = s["m_s"], s["kappa"], s["slip"]
+        taup0 = ((s["m_s"] / ((2.0 / 3.0) * c.pi2 * ri ** 3) + c.rhow) * (ri * 2) ** 2) / (18 * c.surf_rho * c.nuf)
+        h = s["dt"] / taup0
+        g, mflag = o.rad_solver2(rec, c.surf_rho)
+        start = (g / s["r"], s["Tf"] / s["Tp"]) if mflag == 0 else (1.0, 1.0)
+        v2, flag, _ = o.gauss_newton(rec, np.zeros(3), h, c.surf_rho, start)
+        if flag:
+            v2, flag, _ = o.lv_solver(rec, np.zeros(3), h, c.surf_rho, start)
+        assert flag == 0, s
+        rn, Tn = v2[0] * s["r"], v2[1] * s["Tp"]
+        assert abs(rn / float(s["r_new"]) - 1) < 1e-9, (s, rn)
+        assert abs(Tn / float(s["Tp_new"]) - 1) < 1e-11, (s, Tn)
+        F = o.ie_vrt_nd(rec, c.surf_rho, np.zeros(3), float(s["r_new"]) / s["r"], float(s["Tp_new"]) / s["Tp"], h)
+        assert np.max(np.abs(F)) < 1e-12
+
+
+def be_case(dt=5e-4, rh=1.0, r=1e-5, ms=1e-18, n=1):
+    g = Grid.uniform(8, 8, 8, 0.02, 0.02, 0.02)
+    c = Consts(dt=dt, surf_rho=1.2, radius_init=r)
+    o = ora_for(g, c)
+    shape = (g.nnx, g.nny, g.nnz + 2)
+    T = np.full(shape, 300.0)
+    q = np.full(shape, rh * c.Mw / c.Ru / 300.0 * o.L.or_mod_magnus(300.0) / c.surf_rho)
+    zero = np.zeros(shape)
+    o.set_fields(zero, zero, zero, T, q)
+    p = Particles.empty(n)
+    p["x"], p["y"], p["z"] = 0.05, 0.05, 0.08
+    p["radius"], p["m_s"], p["kappa_s"], p["Tp"], p["Tf"] = r, ms, 1.2, 300.0, 300.0
+    return g, c, o, p
+
+
+def test_growth_sign_follows_supersaturation():
+    for rh, sign in ((1.01, 1), (0.95, -1)):
+        g, c, o, p = be_case(rh=rh)
+        r0 = p["radius"][0]
+        o.update_be(p.soa, 2)
+        assert np.sign(p["radius"][0] - r0) == sign
+
+
+def test_stokes_terminal_settling():
+    """Still air: the BE velocity update's fixed point is v = g tau_p
+    (Stokes settling, corrfac = 1 in the BE step)."""
+    g, c, o, p = be_case(dt=1e-3, rh=1.0, r=3e-6)
+    p["vz"] = 0.0
+    o.f.ievap = 0
+    for it in range(2, 200):
+        p["z"] = 0.08  # keep it inside the domain
+        o.update_be(p.soa, it)
+    r = p["radius"][0]
+    V = 4.0 / 3.0 * np.pi * r ** 3
+    rhop = c.rhow + p["m_s"][0] / V
+    tau = rhop * (2 * r) ** 2 / (18 * c.surf_rho * c.nuf)
+    assert abs(p["vz"][0] / (c.part_grav[2] * tau) - 1) < 1e-10
+
+
+def test_rk3_velocity_matches_third_order_taylor():
+    """Linear drag (tiny slip: corrfac -> 1), no gravity: one low-storage RK3
+    step multiplies the slip by 1 + z + z^2/2 + z^3/6, z = -dt/tau_p."""
+    g, c, o, p = be_case(dt=2e-5, r=4e-6)
+    c2 = Consts(dt=2e-5, surf_rho=1.2, radius_init=4e-6, part_grav=(0.0, 0.0, 0.0))
+    o.set_consts(c2)
+    o.f.ievap = 0
+    slip = 1e-9
+    p["vx"] = slip
+    for st in (1, 2, 3):
+        p["x"], p["y"], p["z"] = 0.05, 0.05, 0.08
+        o.update_rk3(p.soa, st, 2)
+    r = p["radius"][0]
+    V = 4.0 / 3.0 * np.pi * r ** 3
+    tau = (c.rhow + p["m_s"][0] / V) * (2 * r) ** 2 / (18 * c.surf_rho * c.nuf)
+    z = -c2.dt / tau
+    assert abs(p["vx"][0] / (slip * (1 + z + z * z / 2 + z ** 3 / 6)) - 1) < 1e-6
+
+
+# ------------------------------------------------------------- coupling
+
+def test_coupling_conserves_and_is_linear():
+    """sum(src * dV) over nodes equals the deposit of the same droplet placed on
+    a node (the trilinear weights sum to one); deposits are linear in mult."""
+    g = Grid.uniform(10, 9, 12, 0.02, 0.02, 0.02)
+    c = Consts(dt=5e-4, radius_init=8e-6)
+    o = ora_for(g, c)
+    rng = np.random.default_rng(5)
+    p = Particles.empty(1)
+    for k in ("vx", "vy", "vz", "vxold", "vyold", "vzold"):
+        p[k] = rng.normal(0, 0.1)
+    p["radius"], p["radiusold"], p["Tp"], p["Tpold"], p["Tf"] = 8e-6, 7.99e-6, 300.02, 300.0, 300.01
+    p["m_s"], p["kappa_s"] = 1e-17, 1.2
+    p.mult[:] = 7
+    dV = g.dx * g.dy * (g.zz[5] - g.zz[4])
+    p["x"], p["y"], p["z"] = 3 * g.dx, 4 * g.dy, g.zz[4]
+    on_node = o.coupling(p.soa, p.mult)
+    p["x"], p["y"], p["z"] = 3.37 * g.dx, 4.81 * g.dy, g.zz[4] + 0.4 * (g.zz[5] - g.zz[4])
+    spread = o.coupling(p.soa, p.mult)
+    for f in range(6):
+        tol = 1e-12 if f < 5 else 1e-9  # field 5 is a difference of two ~Tp-sized terms
+        assert abs(spread[f].sum() * dV - on_node[f].sum() * dV) <= tol * abs(on_node[f].sum() * dV) + 1e-300
+        big = np.abs(on_node[f]).max()
+        assert np.count_nonzero(np.abs(on_node[f]) > 1e-12 * big) <= 1
+        assert np.count_nonzero(np.abs(spread[f]) > 1e-12 * big) == (8 if big > 0 else 0)
+    p.mult[:] = 21
+    triple = o.coupling(p.soa, p.mult)
+    for f in range(6):
+        tol = 1e-14 if f < 5 else 1e-9
+        assert np.max(np.abs(triple[f] - 3 * spread[f])) <= tol * np.max(np.abs(3 * spread[f])), f
+
+
+# ----------------------------------------------------------------- walls
+
+def test_boundary_conditions_hand_cases():
+    g = Grid.uniform(8, 8, 8, 0.02, 0.02, 0.02)
+    c = Consts(dt=1e-3, radius_init=1e-5)
+    o = ora_for(g, c, Flags(ievap=0))
+    zero = np.zeros((g.nnx, g.nny, g.nnz + 2))
+    o.set_fields(zero, zero, zero, zero + 300.0, zero + 0.01)
+    p = Particles.empty(4)
+    r = 1e-5
+    p["radius"], p["m_s"], p["Tp"] = r, 1e-18, 300.0
+    p["x"] = [0.05, g.xl - 1e-4, 1e-4, g.xl]
+    p["y"] = [0.05, 0.05, 0.05, 0.05]
+    p["z"] = [g.z[g.nnz] - 2 * r, 1.5 * r, 0.08, 0.08]
+    p["vx"] = [0.0, 1.0, -1.0, 0.0]
+    p["vz"] = [3.0, -3.0, 0.0, 0.0]
+    o.f.icouple = 0
+    o.update_be(p.soa, 2)
+    top = g.z[g.nnz] - r
+    # droplet 0 crossed the top wall: mirrored and vz negated
+    assert p["z"][0] < top and p["vz"][0] < 0
+    # droplet 1 crossed the bottom wall (z < r): mirrored about z = r
+    assert p["z"][1] > r and p["vz"][1] > 0
+    # droplets 1 and 2 wrapped through x = xl and x = 0
+    assert 0 <= p["x"][1] < g.xl / 2 and g.xl / 2 < p["x"][2] <= g.xl
+    # x == xl is not wrapped (strict >, particles.f90:2826) when it stays put
+    assert p["x"][3] == g.xl
