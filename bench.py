@@ -32,16 +32,47 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def step_kernel_bytes(n: int, grid, precision: int) -> float:
-    """Algorithmic HBM bytes of one step_be launch (not the last of a call):
-    droplet state in (14 FP64 + mult + home + perm) and out (14 FP64 + mult +
-    home), 6 FP64 coupling amounts and the u-cell key out, and one pass over
-    the field values the gather needs (u, v, w in FP32 for mixed precision,
-    FP64 otherwise; T, q FP64) over the ext array."""
-    per_drop = (14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4) + 6 * 8 + 4
+def kernel_bytes(name: str, n: int, grid, precision: int) -> float | None:
+    """Algorithmic HBM bytes of one launch of a step kernel over n droplets
+    (a step that is not the last of a call, so no cold-column write-back).
+
+    be_advance: reads perm (4), the 14 FP64 hot columns, mult (8), home (4)
+      and the previous u-cell key (4) of each droplet; writes the 14 hot
+      columns, mult and home in slot order plus the hand-over to be_solve
+      (8 FP64 + the u level, 68); reads once the gathered fields over the ext
+      array (u, v, w in FP32 for mixed precision, else FP64; T, q FP64).
+    be_solve: reads the 14 hot columns, mult and the hand-over; writes 7 hot
+      columns, the 6 FP64 coupling amounts and the new u-cell key.
+    scatter: reads x, y, z, the key and the 6 amounts of each droplet; reads
+      and writes each of the 6 FP64 source fields once.
+    """
     ext = (grid.nnz + 2) * (grid.nny + 5) * (grid.nnx + 5)
+    nodes = (grid.nnz + 2) * (grid.nny + 1) * (grid.nnx + 1)
     per_cell = (3 * 4 + 2 * 8) if precision == 1 else 5 * 8
-    return float(n) * per_drop + float(ext) * per_cell
+    if name == "be_advance":
+        return float(n) * ((4 + 14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4 + 8 * 8 + 4)) + float(ext) * per_cell
+    if name == "be_solve":
+        return float(n) * ((14 * 8 + 8 + 8 * 8 + 4) + (7 * 8 + 6 * 8 + 4))
+    if name == "scatter":
+        return float(n) * (3 * 8 + 4 + 6 * 8) + 2.0 * 6 * 8 * nodes
+    if name == "step_be":
+        return float(n) * ((4 + 14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4 + 6 * 8 + 4)) + float(ext) * per_cell
+    return None
+
+
+def pmc_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the factors a plain copy of
+    known size gives), when that summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        pm = json.load(fh)
+    if pm.get("workload") != workload or kernel not in pm.get("kernels", {}):
+        return None
+    k = pm["kernels"][kernel]
+    return k["fetch_kib"] * 1024 * pm["fetch_scale"] + k["write_kib"] * 1024 * pm["write_scale"]
 
 
 def prepare(wl_id: int, n: int | None, seed: int):
@@ -194,20 +225,24 @@ def main():
     else:
         accuracy = None
 
-    # roofline of the dominant kernel (mixed precision)
-    dom = max(ktimes.items(), key=lambda kv: kv[1][1])
-    dom_name, (dom_launches, dom_ms) = dom
+    # roofline of the dominant kernel (mixed precision): algorithmic bytes per
+    # launch over its average HIP-event duration on the library's stream
+    dom_name, (dom_launches, dom_ms) = max(ktimes.items(), key=lambda kv: kv[1][1])
     avg_ms = dom_ms / dom_launches
-    if dom_name.startswith("step_"):
-        traffic_alg = step_kernel_bytes(n, g, PREC_MIXED)
-    else:
-        traffic_alg = None
+    per_kernel = {}
+    for k, (nl, tot) in ktimes.items():
+        b = kernel_bytes(k, n, g, PREC_MIXED)
+        if b is not None:
+            per_kernel[k] = {"avg_ms": tot / nl, "algorithmic_bytes": b, "GBps": b / (tot / nl * 1e-3) / 1e9}
+    traffic_alg = kernel_bytes(dom_name, n, g, PREC_MIXED)
     roofline = {"kernel": dom_name, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
-                "avg_launch_ms": avg_ms, "launches": dom_launches}
+                "avg_launch_ms": avg_ms, "launches": dom_launches, "achieved": None, "frac": None,
+                "algorithmic_bytes": traffic_alg}
     if traffic_alg:
         ach = traffic_alg / (avg_ms * 1e-3) / 1e9
-        roofline.update({"achieved": ach, "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes": traffic_alg})
-    roofline["traffic"] = None
+        roofline.update({"achieved": ach, "frac": ach / HBM_PEAK_GBS})
+    roofline["traffic"] = pmc_traffic(dom_name, wl.name)
+    roofline["kernels"] = per_kernel
 
     cpu = None
     if not args.no_cpu:

@@ -46,8 +46,6 @@ struct StepArgs {
   int32_t* keys;          // [cap] u-cell key of the new position (dst slot order)
   int32_t* counts;       // u-cell histogram (must be zero on entry)
   unsigned long long* tally;  // kTallySlots x kTallyWidth counter slots
-  double* hand;          // [kHand][cap] advance -> solve hand-over (slot order)
-  int32_t* hand_kpt;     // [cap] u level of the gather position
   int it;
   int istage;
   int write_cold;        // write the record diagnostics (final step of a fused call)
@@ -288,103 +286,6 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   }
   count_key(a.counts, key, valid);
   if (MODE == 2) flush_tally(tl, a.tally, s_red);
-}
-
-// ------------------------------------------------ split backward-Euler step
-//
-// The BE step runs as two kernels so that each keeps the GPU busy on its own
-// bound: k_be_advance streams the droplets through the cell sort, gathers the
-// fluid and advances velocity and position (memory bound, few registers, high
-// occupancy); k_be_solve streams them again in slot order and runs the FP64
-// radius/temperature Newton solve, walls, coupling amounts and the new cell
-// key (ALU bound).  The hand-over between them is kHand FP64 values and the
-// gather's u level per droplet.
-enum { kHandTf = 0, kHandQ, kHandRho, kHandDiff, kHandRep, kHandVx, kHandVy, kHandVz, kHand };
-
-template <int PREC, int ILIN>
-__global__ void __launch_bounds__(kBlock) k_be_advance(StepArgs a) {
-  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
-  stage_grid(a.g, s_z, s_zz);
-  __syncthreads();
-  const int64_t t = xcd_block() * kBlock + threadIdx.x;
-  if (t >= a.n) return;
-  const int64_t s = a.perm ? (int64_t)a.perm[t] : t;
-  Drop d = load_drop(a.src.hot, a.cap, s);
-  const int64_t mult = a.src.mult[s];
-  const int32_t home = a.src.home[s];
-  const int kg = a.keys_in ? a.keys_in[s] % (a.g.nnz + 1) : -1;
-  Diag o;
-  const BeHand h = be_advance<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.it, o, kg);
-  apply_periodic(a.g, d);
-  store_drop(a.dst.hot, a.cap, t, d);
-  a.dst.mult[t] = mult;
-  a.dst.home[t] = home;
-  double* hd = a.hand;
-  const int64_t cap = a.cap;
-  hd[kHandTf * cap + t] = h.Tf;
-  hd[kHandQ * cap + t] = h.qinf;
-  hd[kHandRho * cap + t] = h.rhoa;
-  hd[kHandDiff * cap + t] = h.diffnorm;
-  hd[kHandRep * cap + t] = h.Rep;
-  hd[kHandVx * cap + t] = o.vold[0];
-  hd[kHandVy * cap + t] = o.vold[1];
-  hd[kHandVz * cap + t] = o.vold[2];
-  a.hand_kpt[t] = o.kpt;
-  if (a.write_cold) {
-    a.cold[kCUFX * cap + home] = o.uf[0];
-    a.cold[kCUFY * cap + home] = o.uf[1];
-    a.cold[kCUFZ * cap + home] = o.uf[2];
-    a.cold[kCTF * cap + home] = o.Tf;
-    a.cold[kCQINF * cap + home] = o.qinf;
-    store_old(a.cold, cap, home, o);
-  }
-}
-
-template <int PREC>
-__global__ void __launch_bounds__(kBlock) k_be_solve(StepArgs a) {
-  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
-  __shared__ int s_red[kTallyWidth];
-  stage_grid(a.g, s_z, s_zz);
-  zero_tally_lds(s_red);
-  __syncthreads();
-  const int64_t t = xcd_block() * kBlock + threadIdx.x;
-  const bool valid = t < a.n;
-  Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
-  int key = -1;
-  if (valid) {
-    const int64_t cap = a.cap;
-    double* hot = a.dst.hot;
-    Drop d = load_drop(hot, cap, t);
-    const double* hd = a.hand;
-    BeHand h;
-    h.Tf = hd[kHandTf * cap + t];
-    h.qinf = hd[kHandQ * cap + t];
-    h.rhoa = hd[kHandRho * cap + t];
-    h.diffnorm = hd[kHandDiff * cap + t];
-    h.Rep = hd[kHandRep * cap + t];
-    const double vold[3] = {hd[kHandVx * cap + t], hd[kHandVy * cap + t], hd[kHandVz * cap + t]};
-    const int kpt = a.hand_kpt[t];
-    const double Tpold = d.Tp, rold = d.r;
-    double qstar;
-    be_finish<PREC>(a.g, a.C, a.fl, s_z, d, h, qstar, tl);
-    double am[6];
-    coupling_amounts<PREC == PREC_MIXED>(a.C, a.fl, d, h.Tf, vold, Tpold, rold, a.dst.mult[t], am);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) a.amounts[k * cap + t] = am[k];
-    hot[kHZ * cap + t] = d.z;
-    hot[kHVZ * cap + t] = d.vz;
-    hot[kHTP * cap + t] = d.Tp;
-    hot[kHR * cap + t] = d.r;
-    hot[kHRES * cap + t] = d.res;
-    hot[kHACTRES * cap + t] = d.actres;
-    hot[kHNUMACT * cap + t] = d.numact;
-    const Cell c = cell_near(a.g, s_z, s_zz, d.x, d.y, d.z, kpt);
-    key = ukey_of(a.g, c);
-    a.keys[t] = key;
-    if (a.write_cold) a.cold[kCQSTAR * cap + a.dst.home[t]] = qstar;
-  }
-  count_key(a.counts, key, valid);
-  flush_tally(tl, a.tally, s_red);
 }
 
 // ------------------------------------------------- reference entry points
@@ -691,6 +592,140 @@ __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
       }
 #pragma unroll
   for (int f = 0; f < 6; ++f) a.src[(int64_t)f * a.nsrc + node] = acc[f];
+}
+
+// Tiled coupling scatter without global atomics.  Each workgroup owns a box
+// of TI x TJ x TK source nodes (node columns ix, iy; levels kz) and keeps its
+// 6 fields in LDS.  It walks the droplets of every u cell whose 8 nodes can
+// touch the box -- cells ix-2..ix-1 along x (plus cell nnx for the periodic
+// image of node 2), likewise along y, and kz-1..kz along z -- through the
+// fresh cell sort (offsets + perm), adds each corner that falls inside the
+// box into LDS, and finally stores the box with plain coalesced writes.
+// Every node belongs to exactly one box, so the fields need no clearing.  A
+// droplet near a box face is read by the neighbouring boxes too: the read
+// amplification is (TI+1)(TJ+1)(TK+1)/(TI TJ TK).
+struct TileArgs {
+  DevGrid g;
+  const double* hot;       // positions (slot order)
+  const double* amounts;   // [6][cap]
+  const int32_t* keys;     // u-cell keys (slot order)
+  const int32_t* perm;     // sorted index -> slot
+  const int32_t* offsets;  // u-cell offsets of the sort
+  int64_t cap;
+  double* src;
+  int64_t nsrc;
+  unsigned fmask;
+  int ntx, nty, ntz;  // boxes along x, y, z
+};
+
+template <int TI, int TJ, int TK>
+__global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
+  constexpr int kNodes = TI * TJ * TK;
+  constexpr int kMaxSeg = (TI + 2) * (TJ + 2);
+  __shared__ double s_acc[6 * kNodes];
+  __shared__ double s_zz[kMaxLevels];
+  __shared__ int s_beg[kMaxSeg], s_pre[kMaxSeg + 1];
+  __shared__ int s_cx[TI + 2], s_cy[TJ + 2];
+  __shared__ int s_ncx, s_ncy;
+  const int64_t b = xcd_block();
+  if (b >= (int64_t)a.ntx * a.nty * a.ntz) return;  // whole block
+  const int kt = (int)(b % a.ntz);
+  const int jt = (int)((b / a.ntz) % a.nty);
+  const int it = (int)(b / ((int64_t)a.ntz * a.nty));
+  const int nnx = a.g.nnx, nny = a.g.nny, nnz = a.g.nnz;
+  const int X0 = 1 + it * TI, X1 = min(X0 + TI - 1, nnx + 1);
+  const int Y0 = 1 + jt * TJ, Y1 = min(Y0 + TJ - 1, nny + 1);
+  const int K0 = kt * TK, K1 = min(K0 + TK - 1, nnz + 1);
+  for (int k = threadIdx.x; k < nnz + 2; k += blockDim.x) s_zz[k] = a.g.zz[k];
+  for (int l = threadIdx.x; l < 6 * kNodes; l += blockDim.x) s_acc[l] = 0.0;
+  if (threadIdx.x == 0) {
+    // x cells whose nodes reach [X0, X1]: node ix takes cells ix-1 and ix-2;
+    // node 2 also takes cell nnx through the periodic image nnx+2
+    int n = 0;
+    const int lo = max(0, X0 - 2), hi = min(nnx, X1 - 1);
+    for (int c = lo; c <= hi; ++c) s_cx[n++] = c;
+    if (X0 <= 2 && X1 >= 2 && hi < nnx) s_cx[n++] = nnx;
+    s_ncx = n;
+    n = 0;
+    const int lo_y = max(0, Y0 - 2), hi_y = min(nny, Y1 - 1);
+    for (int c = lo_y; c <= hi_y; ++c) s_cy[n++] = c;
+    if (Y0 <= 2 && Y1 >= 2 && hi_y < nny) s_cy[n++] = nny;
+    s_ncy = n;
+  }
+  __syncthreads();
+  const int ncx = s_ncx, ncy = s_ncy, nseg = ncx * ncy;
+  const int klo = max(0, K0 - 1), khi = min(nnz, K1);
+  const int nz1 = nnz + 1, ny1 = nny + 1;
+  if (threadIdx.x < nseg) {
+    const int cx = s_cx[threadIdx.x / ncy], cy = s_cy[threadIdx.x % ncy];
+    const int base = nz1 * (cy + ny1 * cx);
+    const int beg = a.offsets[base + klo], end = a.offsets[base + khi + 1];
+    s_beg[threadIdx.x] = beg;
+    s_pre[threadIdx.x + 1] = end - beg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_pre[0] = 0;
+    for (int s = 1; s <= nseg; ++s) s_pre[s] += s_pre[s - 1];
+  }
+  __syncthreads();
+  const int total = s_pre[nseg];
+  const double dx = a.g.dx, dy = a.g.dy;
+  int s = 0;  // segment of p; p only grows
+  for (int p = threadIdx.x; p < total; p += blockDim.x) {
+    while (s + 1 < nseg && s_pre[s + 1] <= p) ++s;
+    const int64_t slot = a.perm[s_beg[s] + (p - s_pre[s])];
+    const int key = a.keys[slot];
+    const double x = a.hot[kHX * a.cap + slot], y = a.hot[kHY * a.cap + slot], z = a.hot[kHZ * a.cap + slot];
+    const int kpt = key % nz1;
+    const int j0 = (key / nz1) % ny1;
+    const int i0 = key / (nz1 * ny1);
+    // the same expression as corner_weight (particles.f90:1197-1205)
+    const double dzu1 = a.g.dzu[kpt + 1];
+    const double dV = dx * dy * dzu1;
+    double wtx[2], wty[2], wtz[2];
+    int lx[2], ly[2], lz[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      wtx[c] = 1.0 - fabs(x - dx * (double)(c + i0)) / dx;
+      wty[c] = 1.0 - fabs(y - dy * (double)(c + j0)) / dy;
+      wtz[c] = 1.0 - fabs(z - s_zz[kpt + c]) / dzu1;
+      int xi = i0 + 1 + c, yj = j0 + 1 + c;
+      if (xi > nnx + 1) xi -= nnx;
+      if (yj > nny + 1) yj -= nny;
+      lx[c] = (xi >= X0 && xi <= X1) ? xi - X0 : -1;
+      ly[c] = (yj >= Y0 && yj <= Y1) ? yj - Y0 : -1;
+      const int kz = kpt + c;
+      lz[c] = (kz >= K0 && kz <= K1) ? kz - K0 : -1;
+    }
+    double am[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) am[f] = ((a.fmask >> f) & 1u) ? a.amounts[f * a.cap + slot] : 0.0;
+#pragma unroll
+    for (int ci = 0; ci < 2; ++ci)
+#pragma unroll
+      for (int cj = 0; cj < 2; ++cj)
+#pragma unroll
+        for (int ck = 0; ck < 2; ++ck) {
+          if (lx[ci] < 0 || ly[cj] < 0 || lz[ck] < 0) continue;
+          const double w = wtx[ci] * wty[cj] * wtz[ck] / dV;
+          const int l = (lx[ci] * TJ + ly[cj]) * TK + lz[ck];
+#pragma unroll
+          for (int f = 0; f < 6; ++f)
+            if ((a.fmask >> f) & 1u) atomicAdd(&s_acc[f * kNodes + l], am[f] * w);
+        }
+  }
+  __syncthreads();
+  // store the box: z fastest, so consecutive threads write consecutive levels
+  const int nk = K1 - K0 + 1, nj = Y1 - Y0 + 1, ni = X1 - X0 + 1;
+  const int64_t sj = nnz + 2, si = (int64_t)(nnz + 2) * (nny + 1);
+  for (int l = threadIdx.x; l < ni * nj * nk; l += blockDim.x) {
+    const int lk = l % nk, lj = (l / nk) % nj, li = l / (nk * nj);
+    const int64_t node = (int64_t)(K0 + lk) + sj * (Y0 + lj - 1) + si * (X0 + li - 1);
+    const int ll = (li * TJ + lj) * TK + lk;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) a.src[(int64_t)f * a.nsrc + node] = s_acc[f * kNodes + ll];
+  }
 }
 
 // ---------------------------------------------------------------- binning

@@ -70,8 +70,6 @@ struct Ctx {
   int32_t* pidx = nullptr;
   int32_t* procidx = nullptr;
   double* amounts = nullptr;
-  double* hand = nullptr;
-  int32_t* hand_kpt = nullptr;
   int32_t* keys[2] = {};
   int32_t* perm = nullptr;
   bool perm_valid = false;  // perm sorts the current slots by u cell
@@ -242,8 +240,6 @@ void release_all() {
   dfree(ctx.pidx);
   dfree(ctx.procidx);
   dfree(ctx.amounts);
-  dfree(ctx.hand);
-  dfree(ctx.hand_kpt);
   dfree(ctx.keys[0]);
   dfree(ctx.keys[1]);
   dfree(ctx.perm);
@@ -298,8 +294,6 @@ StepArgs base_args() {
   a.keys = ctx.keys[ctx.cur];
   a.counts = ctx.counts;
   a.tally = ctx.tally_slots;
-  a.hand = ctx.hand;
-  a.hand_kpt = ctx.hand_kpt;
   return a;
 }
 
@@ -387,6 +381,36 @@ int scatter_pull() {
   return pull_sorted();
 }
 
+// node boxes of the tiled scatter
+constexpr int kTileI = 4, kTileJ = 4, kTileK = 32;
+
+// tiled scatter over the current cell sort (keys, perm and offsets must
+// describe the current slots); writes every node of the 6 fields
+int tile_sorted() {
+  TileArgs t;
+  t.g = ctx.g;
+  t.hot = ctx.hot[ctx.cur];
+  t.amounts = ctx.amounts;
+  t.keys = ctx.keys[ctx.cur];
+  t.perm = ctx.perm;
+  t.offsets = ctx.offsets;
+  t.cap = ctx.cap;
+  t.src = ctx.src;
+  t.nsrc = ctx.nsrc;
+  t.fmask = src_mask();
+  t.ntx = (ctx.g.nnx + 1 + kTileI - 1) / kTileI;
+  t.nty = (ctx.g.nny + 1 + kTileJ - 1) / kTileJ;
+  t.ntz = (ctx.g.nnz + 2 + kTileK - 1) / kTileK;
+  const int64_t nt = (int64_t)t.ntx * t.nty * t.ntz;
+  const unsigned grid = (unsigned)((nt + 7) / 8 * 8);
+  return launch("scatter", [&] { k_scatter_tile<kTileI, kTileJ, kTileK><<<grid, kBlock, 0, ctx.stream>>>(t); });
+}
+
+int scatter_tile() {
+  CHK(bin_current(nullptr, nullptr));
+  return tile_sorted();
+}
+
 template <int MODE, int PREC, int ILIN>
 void launch_update_t(const StepArgs& a, unsigned grid) {
   k_update<MODE, PREC, ILIN><<<grid, kBlock, 0, ctx.stream>>>(a);
@@ -405,30 +429,6 @@ int dispatch_update(const StepArgs& a, int prec) {
     if (prec == PREC_FP64) lin ? launch_update_t<MODE, 0, 1>(a, gr) : launch_update_t<MODE, 0, 0>(a, gr);
     else if (prec == PREC_MIXED) lin ? launch_update_t<MODE, 1, 1>(a, gr) : launch_update_t<MODE, 1, 0>(a, gr);
     else lin ? launch_update_t<MODE, 2, 1>(a, gr) : launch_update_t<MODE, 2, 0>(a, gr);
-  });
-}
-
-template <int PREC, int ILIN>
-void launch_be_t(const StepArgs& a, unsigned grid) {
-  k_be_advance<PREC, ILIN><<<grid, kBlock, 0, ctx.stream>>>(a);
-}
-template <int PREC>
-void launch_be_solve_t(const StepArgs& a, unsigned grid) {
-  k_be_solve<PREC><<<grid, kBlock, 0, ctx.stream>>>(a);
-}
-
-int dispatch_be_split(const StepArgs& a, int prec) {
-  const unsigned gr = grid8(a.n);
-  const bool lin = ctx.fl.ilin == 1;
-  CHK(launch("be_advance", [&] {
-    if (prec == PREC_FP64) lin ? launch_be_t<0, 1>(a, gr) : launch_be_t<0, 0>(a, gr);
-    else if (prec == PREC_MIXED) lin ? launch_be_t<1, 1>(a, gr) : launch_be_t<1, 0>(a, gr);
-    else lin ? launch_be_t<2, 1>(a, gr) : launch_be_t<2, 0>(a, gr);
-  }));
-  return launch("be_solve", [&] {
-    if (prec == PREC_FP64) launch_be_solve_t<0>(a, gr);
-    else if (prec == PREC_MIXED) launch_be_solve_t<1>(a, gr);
-    else launch_be_solve_t<2>(a, gr);
   });
 }
 
@@ -638,8 +638,6 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
     CHK(dalloc(ctx.pidx, n, "pidx"));
     CHK(dalloc(ctx.procidx, n, "procidx"));
     CHK(dalloc(ctx.amounts, 6 * n, "coupling amounts"));
-    CHK(dalloc(ctx.hand, kHand * n, "advance hand-over"));
-    CHK(dalloc(ctx.hand_kpt, n, "advance hand-over"));
     CHK(dalloc(ctx.keys[0], n, "keys"));
     CHK(dalloc(ctx.keys[1], n, "keys"));
     CHK(dalloc(ctx.perm, n, "perm"));
@@ -828,7 +826,7 @@ int ntlp_gpu_particle_coupling_update(double* src_out, int32_t deterministic) {
   StepArgs a = base_args();
   CHK(launch("amounts", [&] { k_amounts<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(a); }));
   if (deterministic) CHK(scatter_pull());
-  else CHK(scatter_atomic());
+  else CHK(scatter_tile());
   if (src_out)
     HIPCHK(hipMemcpyAsync(src_out, ctx.src, sizeof(double) * 6 * ctx.nsrc, hipMemcpyDeviceToHost, ctx.stream));
   HIPCHK(hipStreamSynchronize(ctx.stream));
@@ -933,13 +931,13 @@ int ntlp_gpu_step(int32_t mode, int32_t prec, int32_t nsteps, int32_t it0) {
     a.it = it0 + k;
     a.write_cold = (k == nsteps - 1) ? 1 : 0;
     if (mode == NTLP_MODE_RK3) CHK(dispatch_step<1>(a, prec));
-    else CHK(dispatch_be_split(a, prec));
+    else CHK(dispatch_step<2>(a, prec));
     ctx.cur = 1 - ctx.cur;
     ctx.home_identity = false;
     ctx.keys_valid = true;
     CHK(close_step());
-    CHK(scatter_atomic());
     CHK(finish_bin(nullptr));
+    CHK(tile_sorted());
   }
   HIPCHK(hipStreamSynchronize(ctx.stream));
   return NTLP_OK;

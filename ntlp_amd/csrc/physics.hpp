@@ -7,8 +7,9 @@
 // residual of ie_vrt_nd and the backward evaluations of jacob_approx_2d.
 //
 // The analytic-Jacobian Newton (NTLP_PREC_MIXED, NTLP_PREC_FP64_AJ) solves
-// the same backward-Euler equations (App. of particle_update_BE,
-// particles.f90:4263-4327) with the exact 2x2 Jacobian from the chain rule,
+// the same backward-Euler equations (the residual ie_vrt_nd of
+// particle_update_BE, particles.f90:4263-4327) with the exact 2x2 Jacobian
+// from the chain rule,
 // the same damping, guards, tolerance and Levenberg-Marquardt fallback.
 #pragma once
 #include "device_common.hpp"

@@ -61,6 +61,8 @@ def lib() -> ctypes.CDLL:
         vp, i64, i32, dp = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
         for name, res, args in [
             ("or_mod_magnus", dp, [dp]),
+            ("or_exner", dp, [dp, dp, vp]),
+            ("or_func_p_base", dp, [dp, dp, dp, vp]),
             ("or_cell_index_many", None, [vp, vp, vp, vp, i64, vp]),
             ("or_num_ucells", i64, [vp]),
             ("or_bin", None, [vp, vp, vp, vp, i64, vp, vp, vp, vp]),
@@ -110,6 +112,12 @@ class Oracle:
         c.zetas = (ctypes.c_double * 3)(*consts.zetas)
         c.gama = (ctypes.c_double * 3)(*consts.gama)
         self.c = c
+
+    def exner(self, p0: float, p: float) -> float:
+        return self.L.or_exner(p0, p, ctypes.byref(self.c))
+
+    def p_base(self, p_surf: float, T_surf: float, z: float) -> float:
+        return self.L.or_func_p_base(p_surf, T_surf, z, ctypes.byref(self.c))
 
     # fields --------------------------------------------------------------
     def fill_ext(self, core: np.ndarray) -> np.ndarray:

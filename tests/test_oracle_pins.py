@@ -56,6 +56,36 @@ def test_counting_sort_is_a_stable_argsort():
     assert offsets[0] == 0 and offsets[-1] == n and np.array_equal(np.diff(offsets), counts)
 
 
+def test_wcell_counts_are_a_bincount_of_searchsorted_cells():
+    """calc_tauc_min's num_array (particles.f90:3320-3339): droplets counted
+    at (ipt, jpt, kc) with kc = first w level above z, 1..nnz."""
+    g = small_grid("stretched")
+    rng = np.random.default_rng(4)
+    n = 4000
+    x, y, z = rng.uniform(0, g.xl, n), rng.uniform(0, g.yl, n), rng.uniform(0, g.zl, n)
+    num = ora_for(g).wcell_counts(x, y, z)  # (nnz, nny, nnx)
+    ix = np.floor(x / g.dx).astype(int)
+    iy = np.floor(y / g.dy).astype(int)
+    kc = np.searchsorted(g.z, z, side="right")  # first z(k) > z, k counted from 0
+    ok = (ix < g.nnx) & (iy < g.nny) & (kc >= 1) & (kc <= g.nnz)
+    want = np.zeros((g.nnz, g.nny, g.nnx), dtype=np.int64)
+    np.add.at(want, (kc[ok] - 1, iy[ok], ix[ok]), 1)
+    assert np.array_equal(num, want)
+
+
+# ------------------------------------------------------------------ fields
+
+def test_fill_ext_is_a_periodic_wrap():
+    """fill_ext on one domain: ext(0:nnz+1, -1:nny+3, -1:nnx+3) holds the
+    core (1:nnx, 1:nny) and its periodic images."""
+    g = small_grid("uniform")
+    core = np.random.default_rng(5).standard_normal((g.nnx, g.nny, g.nnz + 2))
+    ext = ora_for(g).fill_ext(core)
+    ii = (np.arange(-1, g.nnx + 4) - 1) % g.nnx
+    jj = (np.arange(-1, g.nny + 4) - 1) % g.nny
+    assert np.array_equal(ext, core[ii][:, jj])
+
+
 # ----------------------------------------------------------- interpolation
 
 def node_coords(g):
@@ -141,6 +171,23 @@ def test_magnus_at_freezing_point():
     assert lib().or_mod_magnus(273.15) == 610.94
 
 
+def test_base_pressure_is_the_dry_adiabat_and_exner_inverts_it():
+    """func_p_base (les.F:10787-10795) is the hydrostatic pressure of a dry
+    adiabatic column: dp/dz = -p g / (Rd T(z)) with T(z) = Ts - g z / Cpa;
+    exner (les.F:10775-10784) of that pressure is T(z)/Ts."""
+    c = Consts()
+    o = ora_for(small_grid(), c)
+    ps, Ts = 101325.0, 300.0
+    assert o.p_base(ps, Ts, 0.0) == ps
+    assert o.exner(ps, ps) == 1.0
+    for z in (10.0, 250.0, 1500.0):
+        h = 1e-3
+        dpdz = (o.p_base(ps, Ts, z + h) - o.p_base(ps, Ts, z - h)) / (2 * h)
+        Tz = Ts - c.grav * z / c.Cpa
+        assert abs(dpdz / (-o.p_base(ps, Ts, z) * c.grav / (c.Rd * Tz)) - 1) < 1e-7
+        assert abs(o.exner(ps, o.p_base(ps, Ts, z)) - Tz / Ts) < 1e-13
+
+
 def test_crit_radius_within_one_table_step_of_closed_form():
     """r_c = sqrt(3B/A) maximises A/r - B/r^3 (Koehler without the exponent)."""
     c = Consts()
@@ -212,6 +259,27 @@ def test_newton_matches_mpmath_roots(radius_init):
         assert abs(Tn / float(s["Tp_new"]) - 1) < 1e-11, (s, Tn)
         F = o.ie_vrt_nd(rec, c.surf_rho, np.zeros(3), float(s["r_new"]) / s["r"], float(s["Tp_new"]) / s["Tp"], h)
         assert np.max(np.abs(F)) < 1e-12
+
+
+def test_levenberg_marquardt_alone_reaches_the_mpmath_roots():
+    """LV_solver (particles.f90:4143-4223) started from the Koehler guess
+    converges to the same roots on its own (its stop test is the same 1e-8
+    correction norm, so it lands within the Newton tolerance)."""
+    c = Consts(surf_rho=1.2, radius_init=1e-5)
+    o = ora_for(small_grid(), c)
+    ri = c.radius_init
+    for s in golden_roots():
+        rec = np.zeros(NREAL)
+        rec[COL["radius"]], rec[COL["Tp"]], rec[COL["Tf"]], rec[COL["qinf"]] = s["r"], s["Tp"], s["Tf"], s["qinf"]
+        rec[COL["m_s"]], rec[COL["kappa_s"]], rec[COL["ufx"]] = s["m_s"], s["kappa"], s["slip"]
+        taup0 = ((s["m_s"] / ((2.0 / 3.0) * c.pi2 * ri ** 3) + c.rhow) * (ri * 2) ** 2) / (18 * c.surf_rho * c.nuf)
+        h = s["dt"] / taup0
+        g, mflag = o.rad_solver2(rec, c.surf_rho)
+        start = (g / s["r"], s["Tf"] / s["Tp"]) if mflag == 0 else (1.0, 1.0)
+        v2, flag, iters = o.lv_solver(rec, np.zeros(3), h, c.surf_rho, start)
+        assert flag == 0 and iters < 1000, s
+        assert abs(v2[0] * s["r"] / float(s["r_new"]) - 1) < 1e-8, s
+        assert abs(v2[1] * s["Tp"] / float(s["Tp_new"]) - 1) < 1e-10, s
 
 
 def be_case(dt=5e-4, rh=1.0, r=1e-5, ms=1e-18, n=1):
