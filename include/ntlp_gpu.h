@@ -209,6 +209,60 @@ int ntlp_gpu_kernel_times(char *names, int64_t *launches, double *total_ms, int3
 /* Waits for all work of the context. */
 int ntlp_gpu_sync(void);
 
+/* ------------------------------------------------------- serving loop */
+
+/* A continuous-batching serving loop over the resident fields; it has no
+ * counterpart in the reference (NTLP advances one droplet population per
+ * run).  A request is a droplet ensemble that is created the way
+ * create_particle does it (vp = uf, Tp = Tf, rc = crit_radius) and advanced
+ * `steps` backward-Euler steps (particle_update_BE, particles.f90:3000-3280,
+ * without the coupling scatter) against the fields of the last
+ * ntlp_gpu_set_fields.  Requests are admitted FIFO into a device pool whose
+ * capacity is hbm_budget_bytes / NTLP_SERVE_BYTES_PER_DROPLET; every
+ * iteration steps all pooled droplets together (one captured hipGraph per
+ * pool size class when use_graph=1) and releases the requests that finish.
+ * A request's result does not depend on what else is in the pool.
+ *
+ * Request input  in[6*n]: column blocks x, y, z, radius, m_s, kappa_s (each n
+ *                values), in_mult[n].
+ * Request output out[NTLP_SERVE_COLS*n]: column blocks x, y, z, vx, vy, vz,
+ *                Tp, radius, m_s, kappa_s, rc, res, actres, numact;
+ *                out_mult[n].
+ * The four buffers are read and written asynchronously by the serving thread
+ * and must stay valid until ntlp_serve_wait returns for the ticket (pinned
+ * host memory makes the copies asynchronous DMA).
+ * While the loop is open, ntlp_gpu_init, ntlp_gpu_finalize and
+ * ntlp_gpu_set_fields return NTLP_E_STATE. */
+#define NTLP_SERVE_COLS 14
+#define NTLP_SERVE_BYTES_PER_DROPLET 436
+typedef struct ntlp_serve_config {
+  int64_t hbm_budget_bytes; /* device memory of the pool (sets its capacity)     */
+  int32_t steps;            /* steps per request                                 */
+  int32_t precision;        /* NTLP_PREC_*                                       */
+  int32_t use_graph;        /* 1: replay captured hipGraphs; 0: launch directly   */
+  int32_t reserved;
+} ntlp_serve_config;
+typedef struct ntlp_serve_totals {
+  int64_t submitted, completed, queued;
+  int64_t iterations, droplet_steps, max_pool, graph_launches, graphs, capacity;
+  ntlp_counters newton; /* Newton bookkeeping over every served droplet-step */
+} ntlp_serve_totals;
+int ntlp_serve_open(const ntlp_serve_config *cfg, int64_t *capacity);
+/* Queues a request (1 <= n <= capacity, n <= 2^23) and returns at once. */
+int ntlp_serve_submit(const double *in, const int64_t *in_mult, int64_t n, double *out, int64_t *out_mult,
+                      int64_t *ticket);
+/* 1 when the ticket's result is in place, 0 while it is queued or running. */
+int ntlp_serve_poll(int64_t ticket);
+/* Blocks until the result is in place; times_ms (may be NULL) receives the
+ * latency from submission to result and the queueing delay before admission.
+ * The ticket is released. */
+int ntlp_serve_wait(int64_t ticket, double *times_ms);
+/* Submitted/completed/queued counts so far (the rest is filled at close). */
+int ntlp_serve_stats(ntlp_serve_totals *out);
+/* Serves every queued request to the end, stops the loop and releases the
+ * pool; final_stats (may be NULL) receives the loop's totals. */
+int ntlp_serve_close(ntlp_serve_totals *final_stats);
+
 #ifdef __cplusplus
 }
 #endif

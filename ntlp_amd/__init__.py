@@ -21,7 +21,7 @@ import numpy as np
 from . import grid as _grid  # noqa: F401  (re-exported helpers)
 from .grid import Grid
 
-__all__ = ["Grid", "Consts", "Flags", "Particles", "DropletStep", "NtlpError", "load_library",
+__all__ = ["Grid", "Consts", "Flags", "Particles", "DropletStep", "ServeLoop", "NtlpError", "load_library",
            "NREAL", "COL", "MODE_RK3", "MODE_BE", "PREC_FP64", "PREC_MIXED", "PREC_FP64_AJ",
            "LAYOUT_EXT", "LAYOUT_CORE", "LAYOUT_XPENCIL", "LIB_PATH"]
 
@@ -172,6 +172,17 @@ class Particles:
                          np.arange(1, n + 1, dtype=np.int32), np.zeros(n, dtype=np.int32))
 
 
+class _CServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("steps", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class _CServeTotals(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "completed", "queued", "iterations", "droplet_steps",
+                                              "max_pool", "graph_launches", "graphs", "capacity")] + \
+               [("newton", _CCounters)]
+
+
 _lib_handle = None
 
 
@@ -214,6 +225,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_timing": (ctypes.c_int, [i32]),
         "ntlp_gpu_kernel_times": (ctypes.c_int, [vp, vp, vp, i32]),
         "ntlp_gpu_sync": (ctypes.c_int, []),
+        "ntlp_serve_open": (ctypes.c_int, [ctypes.POINTER(_CServeConfig), ctypes.POINTER(i64)]),
+        "ntlp_serve_submit": (ctypes.c_int, [vp, vp, i64, vp, vp, ctypes.POINTER(i64)]),
+        "ntlp_serve_poll": (ctypes.c_int, [i64]),
+        "ntlp_serve_wait": (ctypes.c_int, [i64, vp]),
+        "ntlp_serve_stats": (ctypes.c_int, [ctypes.POINTER(_CServeTotals)]),
+        "ntlp_serve_close": (ctypes.c_int, [ctypes.POINTER(_CServeTotals)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -367,3 +384,70 @@ class DropletStep:
 
     def sync(self) -> None:
         self._check(self.lib.ntlp_gpu_sync())
+
+
+# serving request columns (in) and result columns (out), include/ntlp_gpu.h
+SERVE_IN = ["x", "y", "z", "radius", "m_s", "kappa_s"]
+SERVE_OUT = ["x", "y", "z", "vx", "vy", "vz", "Tp", "radius", "m_s", "kappa_s", "rc", "res", "actres", "numact"]
+SERVE_BYTES_PER_DROPLET = 436
+
+
+class ServeLoop:
+    """The continuous-batching serving loop (ntlp_serve_*) over the fields
+    resident in an open :class:`DropletStep`.
+
+    ``submit(inp, mult, out, out_mult)`` queues one droplet ensemble --
+    ``inp`` is (6, n) FP64 in ``SERVE_IN`` order, ``out`` (14, n) FP64 receives
+    ``SERVE_OUT`` -- and returns a ticket at once; the arrays must stay alive
+    until ``wait(ticket)`` returns (latency ms, queueing ms).  Pinned host
+    arrays make the copies asynchronous DMA."""
+
+    def __init__(self, step: DropletStep, hbm_budget_bytes: int, steps: int = 10, precision: int = PREC_MIXED,
+                 use_graph: bool = True):
+        self.step = step
+        self.lib = step.lib
+        cfg = _CServeConfig(int(hbm_budget_bytes), int(steps), int(precision), int(use_graph), 0)
+        cap = ctypes.c_int64()
+        self._check(self.lib.ntlp_serve_open(ctypes.byref(cfg), ctypes.byref(cap)))
+        self.capacity = int(cap.value)
+        self._held = {}
+
+    def _check(self, rc: int) -> int:
+        return self.step._check(rc)
+
+    def submit(self, inp: np.ndarray, mult: np.ndarray, out: np.ndarray, out_mult: np.ndarray) -> int:
+        n = int(inp.shape[1])
+        for a, shape, dt in ((inp, (6, n), np.float64), (mult, (n,), np.int64), (out, (14, n), np.float64),
+                             (out_mult, (n,), np.int64)):
+            if a.shape != shape or a.dtype != dt or not a.flags.c_contiguous:
+                raise ValueError(f"serving buffers must be C-contiguous {dt.__name__} of shape {shape}")
+        t = ctypes.c_int64()
+        self._check(self.lib.ntlp_serve_submit(_ptr(inp), _ptr(mult), n, _ptr(out), _ptr(out_mult),
+                                               ctypes.byref(t)))
+        self._held[t.value] = (inp, mult, out, out_mult)
+        return int(t.value)
+
+    def poll(self, ticket: int) -> bool:
+        return self._check(self.lib.ntlp_serve_poll(int(ticket))) == 1
+
+    def wait(self, ticket: int) -> tuple[float, float]:
+        times = np.zeros(2)
+        self._check(self.lib.ntlp_serve_wait(int(ticket), _ptr(times)))
+        self._held.pop(ticket, None)
+        return float(times[0]), float(times[1])
+
+    def stats(self) -> dict:
+        t = _CServeTotals()
+        self._check(self.lib.ntlp_serve_stats(ctypes.byref(t)))
+        return {"submitted": int(t.submitted), "completed": int(t.completed), "queued": int(t.queued)}
+
+    def close(self) -> dict:
+        t = _CServeTotals()
+        self._check(self.lib.ntlp_serve_close(ctypes.byref(t)))
+        self._held.clear()
+        d = {n: int(getattr(t, n)) for n in ("submitted", "completed", "queued", "iterations", "droplet_steps",
+                                             "max_pool", "graph_launches", "graphs", "capacity")}
+        c = t.newton
+        d["newton"] = {n: int(getattr(c, n)) for n in
+                       ("num100", "numimpos", "actnum", "denum", "gn_iters", "lm_iters", "solves", "steps")}
+        return d

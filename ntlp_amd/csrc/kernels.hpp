@@ -32,6 +32,21 @@ struct DropletBufs {
   int32_t* home;
 };
 
+// Serving pool: the slot of a request (home >> kServeIdxBits) and the index
+// of the droplet in it (home & kServeIdxMask) travel in `home`.
+constexpr int kServeIdxBits = 23;
+constexpr int32_t kServeIdxMask = (1 << kServeIdxBits) - 1;
+constexpr int kServeSlots = 256;
+
+// Per-iteration control block of a serving pool (device resident, written by
+// the host before each step so that one captured graph serves every pool size).
+struct ServeCtl {
+  int64_t hw;      // slots in use in the current buffer (staying and leaving)
+  int64_t n_live;  // droplets stepped in this iteration
+  int64_t out_off[kServeSlots];  // first output row of each request slot
+  int32_t last[kServeSlots];     // 1: this iteration is the request's last step
+};
+
 struct StepArgs {
   DevGrid g;
   DevConsts C;
@@ -46,6 +61,10 @@ struct StepArgs {
   int32_t* keys;          // [cap] u-cell key of the new position (dst slot order)
   int32_t* counts;       // u-cell histogram (must be zero on entry)
   unsigned long long* tally;  // kTallySlots x kTallyWidth counter slots
+  const ServeCtl* serve;      // serving pool control (SERVE kernels only)
+  double* out;                // serving output rows: [kNHot][out_cap]
+  int64_t* out_mult;
+  int64_t out_cap;
   int it;
   int istage;
   int write_cold;        // write the record diagnostics (final step of a fused call)
@@ -217,16 +236,18 @@ __device__ __forceinline__ void store_rhs(double* cold, int64_t cap, int64_t h, 
 
 // Wave-aggregated histogram increment for runs of equal keys in lane order
 // (the droplets are cell sorted, so runs are long): one atomic per run.
+// Histogram of the u-cell keys of a wave: one atomic per run of equal keys.
+// A negative key (a droplet leaving the pool) is not counted.
 __device__ __forceinline__ void count_key(int* counts, int key, bool valid) {
   const int lane = lane_id();
-  int prev = __shfl_up(key, 1, 64);
-  bool head = valid && (lane == 0 || prev != key);
-  unsigned long long heads = __ballot(head);
-  unsigned long long vmask = __ballot(valid);
+  const bool live = valid && key >= 0;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = live && (lane == 0 || prev != key);
+  // a run ends before the next head or the next lane that is not counted
+  const unsigned long long brk = __ballot(head || !live);
   if (head) {
-    // run ends before the next head (or at the last valid lane)
-    unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1)) << (lane + 1);
-    int end = after ? (__ffsll((long long)after) - 1) : (64 - __clzll((long long)vmask));
+    const unsigned long long after = (lane == 63) ? 0ull : (brk >> (lane + 1)) << (lane + 1);
+    const int end = after ? (__ffsll((long long)after) - 1) : 64;
     atomicAdd(&counts[key], end - lane);
   }
 }
@@ -237,7 +258,12 @@ __device__ __forceinline__ void count_key(int* counts, int key, bool valid) {
 // conditions + coupling amounts + new u-cell key and histogram.  Reads the
 // droplet through perm from the source buffers and writes it to slot t of
 // the destination buffers, so that the destination is cell sorted.
-template <int MODE, int PREC, int ILIN>
+//
+// SERVE: the step of a serving pool.  The droplet count comes from the control
+// block; a droplet whose request takes its last step here is written to the
+// request's output rows and leaves the pool (key -1: the next sort drops it);
+// no coupling amounts are formed.
+template <int MODE, int PREC, int ILIN, bool SERVE = false>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
   __shared__ int s_red[kTallyWidth];
@@ -245,7 +271,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   zero_tally_lds(s_red);
   __syncthreads();
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
-  const bool valid = t < a.n;
+  const bool valid = t < (SERVE ? a.serve->n_live : a.n);
   Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
   int key = -1;
   if (valid) {
@@ -265,20 +291,29 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     } else {
       be_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg);
     }
-    double am[6];
-    coupling_amounts<PREC == PREC_MIXED>(a.C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
+    if (SERVE && a.serve->last[home >> kServeIdxBits]) {
+      const int64_t row = a.serve->out_off[home >> kServeIdxBits] + (home & kServeIdxMask);
+      store_drop(a.out, a.out_cap, row, d);
+      a.out_mult[row] = mult;
+      a.keys[t] = -1;
+    } else {
+      if (!SERVE) {
+        double am[6];
+        coupling_amounts<PREC == PREC_MIXED>(a.C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) a.amounts[k * a.cap + t] = am[k];
-    store_dynamic(a.dst.hot, a.cap, t, d);
-    a.dst.hot[kHMS * a.cap + t] = d.m_s;
-    a.dst.hot[kHKAPPA * a.cap + t] = d.kappa;
-    a.dst.hot[kHRC * a.cap + t] = d.rc;
-    a.dst.mult[t] = mult;
-    a.dst.home[t] = home;
-    const Cell c = cell_near(a.g, s_z, s_zz, d.x, d.y, d.z, o.kpt);
-    key = ukey_of(a.g, c);
-    a.keys[t] = key;
-    if (a.write_cold) {
+        for (int k = 0; k < 6; ++k) a.amounts[k * a.cap + t] = am[k];
+      }
+      store_dynamic(a.dst.hot, a.cap, t, d);
+      a.dst.hot[kHMS * a.cap + t] = d.m_s;
+      a.dst.hot[kHKAPPA * a.cap + t] = d.kappa;
+      a.dst.hot[kHRC * a.cap + t] = d.rc;
+      a.dst.mult[t] = mult;
+      a.dst.home[t] = home;
+      const Cell c = cell_near(a.g, s_z, s_zz, d.x, d.y, d.z, o.kpt);
+      key = ukey_of(a.g, c);
+      a.keys[t] = key;
+    }
+    if (!SERVE && a.write_cold) {
       store_gathered(a.cold, a.cap, home, o);
       store_old(a.cold, a.cap, home, o);
       if (MODE == 1) store_rhs(a.cold, a.cap, home, o);
@@ -858,28 +893,33 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const int32_t* in, i
 
 // Counting-sort scatter: each run of equal keys in a wave takes a block of
 // slots inside its cell with one atomic; counts are decremented back to zero.
+// Scatter of slot indices into their cells' ranges (counts run down to zero).
+// Slots with a negative key are skipped.  n_dev, when set, holds the slot
+// count (graph replays with a changing pool).
 __global__ void __launch_bounds__(kBlock) k_bin_scatter(const int32_t* keys, int64_t n, const int32_t* offsets,
-                                                        int32_t* counts, int32_t* perm) {
+                                                        int32_t* counts, int32_t* perm, const int64_t* n_dev) {
+  if (n_dev) n = *n_dev;
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
   const bool valid = t < n;
   const int lane = lane_id();
   const int key = valid ? keys[t] : -1;
+  const bool live = key >= 0;
   const int prev = __shfl_up(key, 1, 64);
-  const bool head = valid && (lane == 0 || prev != key);
+  const bool head = live && (lane == 0 || prev != key);
   const unsigned long long heads = __ballot(head);
-  const unsigned long long vmask = __ballot(valid);
+  const unsigned long long brk = __ballot(head || !live);
   const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
   const int seg_start = max(0, 63 - __clzll((long long)(heads & upto)));
   int base = 0;
   if (head) {
-    const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1)) << (lane + 1);
-    const int end = after ? (__ffsll((long long)after) - 1) : (64 - __clzll((long long)vmask));
+    const unsigned long long after = (lane == 63) ? 0ull : (brk >> (lane + 1)) << (lane + 1);
+    const int end = after ? (__ffsll((long long)after) - 1) : 64;
     const int len = end - lane;
     const int old = atomicSub(&counts[key], len);
     base = offsets[key] + old - len;
   }
   base = __shfl(base, seg_start, 64);
-  if (valid) perm[base + (lane - seg_start)] = (int32_t)t;
+  if (live) perm[base + (lane - seg_start)] = (int32_t)t;
 }
 
 // Makes the counting sort stable and deterministic: orders every cell's
@@ -961,16 +1001,11 @@ __global__ void k_to_f32(const double* in, float* out, int64_t n) {
 // (5294-5305); the first maximum wins, like maxloc.
 __constant__ double c_radval[1024];
 
-__global__ void __launch_bounds__(kBlock) k_crit_radius(StepArgs a) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.n) return;
-  double* hot = a.src.hot;
-  if (a.fl.ievap != 1) {
-    hot[kHRC * a.cap + t] = 0.0;
-    return;
-  }
-  const double m_s = hot[kHMS * a.cap + t], kappa = hot[kHKAPPA * a.cap + t], Tf = hot[kHTP * a.cap + t];
-  const DevConsts& C = a.C;
+// crit_radius (particles.f90:5307-5327): the table radius of largest
+// equilibrium supersaturation; 0 when ievap /= 1 (create_particle, 2264-2266).
+__device__ __forceinline__ double crit_radius_of(const DevConsts& C, const DevFlags& fl, double m_s, double kappa,
+                                                 double Tf) {
+  if (fl.ievap != 1) return 0.0;
   const double A = 2 * C.Mw * C.Gam / C.Ru / C.rhow / Tf;
   const double B = kappa * m_s;
   const double D = C.rhos * C.pi2 * 2.0 / 3.0;
@@ -984,7 +1019,76 @@ __global__ void __launch_bounds__(kBlock) k_crit_radius(StepArgs a) {
       ib = i;
     }
   }
-  hot[kHRC * a.cap + t] = c_radval[ib];
+  return c_radval[ib];
+}
+
+__global__ void __launch_bounds__(kBlock) k_crit_radius(StepArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n) return;
+  double* hot = a.src.hot;
+  hot[kHRC * a.cap + t] =
+      crit_radius_of(a.C, a.fl, hot[kHMS * a.cap + t], hot[kHKAPPA * a.cap + t], hot[kHTP * a.cap + t]);
+}
+
+// Admission of one request into a serving pool: create_particle's state for
+// each droplet from (x, y, z, radius, m_s, kappa_s, mult) -- vp = uf(xp),
+// Tp = Tf(xp) (the it <= 1 set-up of particles.f90:2907-2910, with the FP64
+// gather of uf_interp_lin/uf_interp), rc = crit_radius, residence times and
+// activation count zero -- appended at slots base.. of the pool, with its
+// u-cell key counted for the next sort.
+struct AdmitArgs {
+  DevGrid g;
+  DevConsts C;
+  DevFlags fl;
+  DevFields F;
+  const double* in;        // [6][m]: x, y, z, radius, m_s, kappa_s
+  const int64_t* in_mult;  // [m]
+  int64_t m, cap, base;
+  int32_t tag;             // request slot << kServeIdxBits
+  double* hot;
+  int64_t* mult;
+  int32_t* home;
+  int32_t* keys;
+  int32_t* counts;
+};
+
+template <int ILIN>
+__global__ void __launch_bounds__(kBlock) k_serve_admit(AdmitArgs a) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  stage_grid(a.g, s_z, s_zz);
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  const bool valid = t < a.m;
+  int key = -1;
+  if (valid) {
+    const double x = a.in[t], y = a.in[a.m + t], z = a.in[2 * a.m + t];
+    const double m_s = a.in[4 * a.m + t], kappa = a.in[5 * a.m + t];
+    const Cell c = cell_of(a.g, s_z, s_zz, x, y, z);
+    const Fluid f = (ILIN == 1) ? gather_lin_f64<true>(a.g, a.F, s_z, s_zz, x, y, z, c)
+                                : gather6_f64(a.g, a.F, s_z, s_zz, x, y, z, c);
+    Drop d;
+    d.x = x;
+    d.y = y;
+    d.z = z;
+    d.vx = f.u[0];
+    d.vy = f.u[1];
+    d.vz = f.u[2];
+    d.Tp = f.T;
+    d.r = a.in[3 * a.m + t];
+    d.m_s = m_s;
+    d.kappa = kappa;
+    d.rc = crit_radius_of(a.C, a.fl, m_s, kappa, f.T);
+    d.res = 0.0;
+    d.actres = 0.0;
+    d.numact = 0.0;
+    const int64_t s = a.base + t;
+    store_drop(a.hot, a.cap, s, d);
+    a.mult[s] = a.in_mult[t];
+    a.home[s] = a.tag | (int32_t)t;
+    key = ukey_of(a.g, c);
+    a.keys[s] = key;
+  }
+  count_key(a.counts, key, valid);
 }
 
 // slot-order column -> put order via home

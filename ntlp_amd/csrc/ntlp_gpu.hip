@@ -87,6 +87,8 @@ struct Ctx {
   unsigned long long* tally = nullptr;        // kTallyRows per-step rows
   unsigned long long* tally_slots = nullptr;  // kTallySlots x kTallyWidth
   int64_t step_row = 0;
+  // a serving loop is open on these fields
+  bool serving = false;
   // timing
   bool timing = false;
   std::vector<Timed> pending;
@@ -313,7 +315,8 @@ int scan_counts() {
 int finish_bin(const int32_t* tie) {
   CHK(scan_counts());
   CHK(launch("bin_scatter", [&] {
-    k_bin_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(ctx.keys[ctx.cur], ctx.n, ctx.offsets, ctx.counts, ctx.perm);
+    k_bin_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(ctx.keys[ctx.cur], ctx.n, ctx.offsets, ctx.counts, ctx.perm,
+                                                              nullptr);
   }));
   CHK(launch("bin_cellsort", [&] {
     k_cell_sort<<<grid_flat(ctx.ncell), kBlock, 0, ctx.stream>>>(ctx.offsets, ctx.ncell, ctx.perm, tie);
@@ -461,6 +464,7 @@ extern "C" {
 const char* ntlp_gpu_last_error(void) { return g_err.c_str(); }
 
 int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags* f) {
+  if (ctx.serving) return fail(NTLP_E_STATE, "close the serving loop first");
   if (!grid || !c || !f) return fail(NTLP_E_INVALID, "null argument to ntlp_gpu_init");
   if (grid->nnx < 2 || grid->nny < 2 || grid->nnz < 3) return fail(NTLP_E_INVALID, "grid too small (nnx,nny >= 2, nnz >= 3)");
   if (grid->nnz + 2 > kMaxLevels) return fail(NTLP_E_RANGE, "nnz + 2 exceeds the LDS grid limit of 1024 levels");
@@ -567,6 +571,7 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
 }
 
 int ntlp_gpu_finalize(void) {
+  if (ctx.serving) return fail(NTLP_E_STATE, "close the serving loop first");
   if (ctx.stream) (void)hipStreamSynchronize(ctx.stream);
   release_all();
   return NTLP_OK;
@@ -582,6 +587,7 @@ int ntlp_gpu_set_dt(double dt) {
 int ntlp_gpu_set_fields(const double* u, const double* v, const double* w, const double* T, const double* q,
                         int32_t layout) {
   CHK(need_init());
+  if (ctx.serving) return fail(NTLP_E_STATE, "the serving loop reads the fields: close it first");
   const double* in[5] = {u, v, w, T, q};
   for (auto p : in)
     if (!p) return fail(NTLP_E_INVALID, "null field pointer");
@@ -1019,3 +1025,5 @@ int ntlp_gpu_sync(void) {
 }
 
 }  // extern "C"
+
+#include "serve.hpp"

@@ -1,0 +1,578 @@
+// serve.hpp -- continuous-batching serving loop of the droplet step.
+//
+// Requests are droplet ensembles (x, y, z, radius, m_s, kappa_s, mult) that
+// each advance a fixed number of backward-Euler steps against the fields
+// resident in the main context.  One engine thread owns a droplet pool on the
+// device and its own stream:
+//
+//   every iteration
+//     admit   queued requests, FIFO, while a request slot is free and the pool
+//             has room (capacity = HBM budget / bytes per droplet): copy the
+//             request in and append its droplets (k_serve_admit);
+//     control write the per-iteration control block (pool size, which
+//             requests take their last step, their output rows);
+//     step    one captured hipGraph per (pool size class, buffer parity):
+//             counting sort of the pool by u cell, then the fused step of
+//             every droplet; droplets of finishing requests go to the output
+//             rows and drop out of the next sort, so the pool compacts
+//             itself;
+//     retire  copy finished requests' rows out and signal their tickets.
+//
+// Requests that arrive while others are in the pool join at the next
+// iteration, and their steps run in the same kernels as everyone else's.  A
+// droplet's arithmetic does not depend on the other droplets in the pool
+// (the fields are fixed and the coupling scatter is not formed), so a
+// request's result is the same alone and in any mix.
+//
+// This file is included by ntlp_gpu.hip after the context helpers.
+#pragma once
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Device bytes per pool droplet: hot state and mult/home/keys twice
+// (ping-pong), perm, admission staging (6 FP64 + mult) and output rows.
+constexpr int64_t kServeBytesPerDroplet =
+    2 * (kNHot * 8 + 8 + 4 + 4) + 4 + (6 * 8 + 8) + (kNHot * 8 + 8);
+static_assert(kServeBytesPerDroplet == NTLP_SERVE_BYTES_PER_DROPLET, "pool bytes per droplet");
+constexpr int kServeCtlRing = 4;
+
+enum { kReqQueued = 0, kReqActive, kReqLeaving, kReqDone, kReqFailed };
+
+struct SReq {
+  int64_t ticket = 0;
+  int64_t n = 0;
+  const double* in = nullptr;
+  const int64_t* in_mult = nullptr;
+  double* out = nullptr;
+  int64_t* out_mult = nullptr;
+  Clock::time_point t_submit, t_admit, t_done;
+  int slot = -1;
+  int steps_done = 0;
+  int state = kReqQueued;
+};
+
+struct Serve {
+  bool open = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int steps = 10, prec = PREC_MIXED;
+  bool use_graph = true;
+  int64_t cap = 0;
+  // device pool
+  double* hot[2] = {};
+  int64_t* mult[2] = {};
+  int32_t* home[2] = {};
+  int32_t* keys[2] = {};
+  int32_t* perm = nullptr;
+  int32_t* counts = nullptr;
+  int32_t* offsets = nullptr;
+  int32_t* bsum = nullptr;
+  double* in = nullptr;
+  int64_t* in_mult = nullptr;
+  double* out = nullptr;
+  int64_t* out_mult = nullptr;
+  ServeCtl* ctl_dev = nullptr;
+  unsigned long long* tally = nullptr;
+  // host-side pool state (engine thread only)
+  ServeCtl* ctl_host[kServeCtlRing] = {};
+  hipEvent_t ctl_ev[kServeCtlRing] = {};
+  int ctl_i = 0;
+  int cur = 0;
+  int64_t hw = 0;
+  std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
+  std::vector<SReq*> active;
+  bool slot_used[kServeSlots] = {};
+  struct Pending {
+    hipEvent_t ev;
+    std::vector<SReq*> reqs;
+  };
+  std::deque<Pending> pending;
+  // shared with the client threads
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  std::deque<SReq*> queue;
+  std::unordered_map<int64_t, SReq*> reqs;
+  int64_t next_ticket = 1;
+  int err = 0;
+  std::string err_msg;
+  // statistics
+  int64_t submitted = 0, completed = 0, iterations = 0, droplet_steps = 0, max_pool = 0, graph_launches = 0;
+};
+
+Serve srv;
+
+int serve_fail(int code, const std::string& msg) {
+  std::lock_guard<std::mutex> lk(srv.mu);
+  if (!srv.err) {
+    srv.err = code;
+    srv.err_msg = msg;
+  }
+  return code;
+}
+
+#define SHIP(call)                                                                                 \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) return serve_fail(NTLP_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+void serve_release() {
+  for (auto& kv : srv.graphs) (void)hipGraphExecDestroy(kv.second);
+  srv.graphs.clear();
+  for (int b = 0; b < 2; ++b) {
+    dfree(srv.hot[b]);
+    dfree(srv.mult[b]);
+    dfree(srv.home[b]);
+    dfree(srv.keys[b]);
+  }
+  dfree(srv.perm);
+  dfree(srv.counts);
+  dfree(srv.offsets);
+  dfree(srv.bsum);
+  dfree(srv.in);
+  dfree(srv.in_mult);
+  dfree(srv.out);
+  dfree(srv.out_mult);
+  dfree(srv.ctl_dev);
+  dfree(srv.tally);
+  for (int i = 0; i < kServeCtlRing; ++i) {
+    if (srv.ctl_host[i]) (void)hipHostFree(srv.ctl_host[i]);
+    if (srv.ctl_ev[i]) (void)hipEventDestroy(srv.ctl_ev[i]);
+  }
+  for (auto& p : srv.pending) (void)hipEventDestroy(p.ev);
+  for (auto& kv : srv.reqs) delete kv.second;
+  if (srv.stream) (void)hipStreamDestroy(srv.stream);
+  srv.~Serve();
+  new (&srv) Serve();
+}
+
+StepArgs serve_step_args(int cur) {
+  StepArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = ctx.g;
+  a.C = ctx.C;
+  a.fl = ctx.fl;
+  a.F = fields_view();
+  a.n = srv.cap;
+  a.cap = srv.cap;
+  a.src = {srv.hot[cur], srv.mult[cur], srv.home[cur]};
+  a.dst = {srv.hot[1 - cur], srv.mult[1 - cur], srv.home[1 - cur]};
+  a.perm = srv.perm;
+  a.keys_in = srv.keys[cur];
+  a.keys = srv.keys[1 - cur];
+  a.counts = srv.counts;
+  a.tally = srv.tally;
+  a.serve = srv.ctl_dev;
+  a.out = srv.out;
+  a.out_mult = srv.out_mult;
+  a.out_cap = srv.cap;
+  a.it = 2;  // droplets are created with vp = uf and Tp = Tf at admission
+  return a;
+}
+
+template <int PREC>
+void serve_launch_step(const StepArgs& a, unsigned grid) {
+  if (ctx.fl.ilin == 1) k_step<2, PREC, 1, true><<<grid, kBlock, 0, srv.stream>>>(a);
+  else k_step<2, PREC, 0, true><<<grid, kBlock, 0, srv.stream>>>(a);
+}
+
+// The per-iteration kernels for a pool of at most `size` slots: counting sort
+// of the current buffer (the counts were gathered by the previous step and
+// the admissions), then the step.  Sizes come from the control block.
+void serve_enqueue_step(int64_t size, int cur) {
+  const int64_t nb = (ctx.ncell + kScanTile - 1) / kScanTile;
+  k_scan_reduce<<<(unsigned)nb, kScanThreads, 0, srv.stream>>>(srv.counts, ctx.ncell, srv.bsum);
+  k_scan_bsum<<<1, kScanThreads, 0, srv.stream>>>(srv.bsum, nb);
+  k_scan_down<<<(unsigned)nb, kScanThreads, 0, srv.stream>>>(srv.counts, ctx.ncell, srv.bsum, srv.offsets);
+  k_bin_scatter<<<grid8(size), kBlock, 0, srv.stream>>>(srv.keys[cur], size, srv.offsets, srv.counts, srv.perm,
+                                                        &srv.ctl_dev->hw);
+  k_cell_sort<<<grid_flat(ctx.ncell), kBlock, 0, srv.stream>>>(srv.offsets, ctx.ncell, srv.perm, nullptr);
+  const StepArgs a = serve_step_args(cur);
+  if (srv.prec == PREC_FP64) serve_launch_step<PREC_FP64>(a, grid8(size));
+  else if (srv.prec == PREC_MIXED) serve_launch_step<PREC_MIXED>(a, grid8(size));
+  else serve_launch_step<PREC_FP64_AJ>(a, grid8(size));
+}
+
+int serve_run_step(int64_t slots) {
+  int64_t size = 4096;
+  while (size < slots) size *= 2;
+  size = std::min(size, srv.cap);
+  if (!srv.use_graph) {
+    serve_enqueue_step(size, srv.cur);
+    SHIP(hipGetLastError());
+    return 0;
+  }
+  const auto key = std::make_pair(size, srv.cur);
+  auto it = srv.graphs.find(key);
+  if (it == srv.graphs.end()) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    SHIP(hipStreamBeginCapture(srv.stream, hipStreamCaptureModeThreadLocal));
+    serve_enqueue_step(size, srv.cur);
+    const hipError_t launch_err = hipGetLastError();
+    const hipError_t end_err = hipStreamEndCapture(srv.stream, &graph);
+    SHIP(launch_err);
+    SHIP(end_err);
+    const hipError_t inst_err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SHIP(inst_err);
+    it = srv.graphs.emplace(key, exec).first;
+  }
+  SHIP(hipGraphLaunch(it->second, srv.stream));
+  srv.graph_launches++;
+  return 0;
+}
+
+// Marks requests whose output copies have landed as done.  wait_all: block
+// on every pending copy; otherwise only poll (and bound the run-ahead).
+int serve_retire(bool wait_all) {
+  while (!srv.pending.empty()) {
+    auto& p = srv.pending.front();
+    hipError_t q = hipEventQuery(p.ev);
+    if (q == hipErrorNotReady) {
+      if (!wait_all && srv.pending.size() <= 2) break;
+      SHIP(hipEventSynchronize(p.ev));
+    } else if (q != hipSuccess) {
+      return serve_fail(NTLP_E_HIP, std::string("serving step failed: ") + hipGetErrorString(q));
+    }
+    const auto now = Clock::now();
+    {
+      std::lock_guard<std::mutex> lk(srv.mu);
+      for (SReq* r : p.reqs) {
+        r->state = kReqDone;
+        r->t_done = now;
+        srv.completed++;
+      }
+    }
+    (void)hipEventDestroy(p.ev);
+    srv.pending.pop_front();
+    srv.cv_done.notify_all();
+  }
+  return 0;
+}
+
+int serve_iteration(std::vector<SReq*>& admit) {
+  // admissions: copy in and append at slots hw..
+  int64_t in_off = 0;
+  for (SReq* r : admit) {
+    SHIP(hipMemcpyAsync(srv.in + 6 * in_off, r->in, sizeof(double) * 6 * r->n, hipMemcpyHostToDevice, srv.stream));
+    SHIP(hipMemcpyAsync(srv.in_mult + in_off, r->in_mult, sizeof(int64_t) * r->n, hipMemcpyHostToDevice, srv.stream));
+    AdmitArgs a;
+    a.g = ctx.g;
+    a.C = ctx.C;
+    a.fl = ctx.fl;
+    a.F = fields_view();
+    a.in = srv.in + 6 * in_off;
+    a.in_mult = srv.in_mult + in_off;
+    a.m = r->n;
+    a.cap = srv.cap;
+    a.base = srv.hw;
+    a.tag = (int32_t)(r->slot << kServeIdxBits);
+    a.hot = srv.hot[srv.cur];
+    a.mult = srv.mult[srv.cur];
+    a.home = srv.home[srv.cur];
+    a.keys = srv.keys[srv.cur];
+    a.counts = srv.counts;
+    if (ctx.fl.ilin == 1) k_serve_admit<1><<<grid8(r->n), kBlock, 0, srv.stream>>>(a);
+    else k_serve_admit<0><<<grid8(r->n), kBlock, 0, srv.stream>>>(a);
+    SHIP(hipGetLastError());
+    srv.hw += r->n;
+    in_off += r->n;
+    r->t_admit = Clock::now();
+    r->state = kReqActive;
+    srv.active.push_back(r);
+  }
+  // control block
+  const int ci = srv.ctl_i;
+  srv.ctl_i = (srv.ctl_i + 1) % kServeCtlRing;
+  SHIP(hipEventSynchronize(srv.ctl_ev[ci]));
+  ServeCtl* c = srv.ctl_host[ci];
+  memset(c, 0, sizeof(ServeCtl));
+  c->hw = srv.hw;
+  int64_t n_step = 0, off = 0;
+  std::vector<SReq*> leaving;
+  for (SReq* r : srv.active) {
+    n_step += r->n;
+    if (r->steps_done + 1 == srv.steps) {
+      c->last[r->slot] = 1;
+      c->out_off[r->slot] = off;
+      off += r->n;
+      leaving.push_back(r);
+    }
+  }
+  c->n_live = n_step;
+  SHIP(hipMemcpyAsync(srv.ctl_dev, c, sizeof(ServeCtl), hipMemcpyHostToDevice, srv.stream));
+  SHIP(hipEventRecord(srv.ctl_ev[ci], srv.stream));
+  CHK(serve_run_step(srv.hw));
+  // copy out the finished requests (rows are [kNHot][cap])
+  for (SReq* r : leaving) {
+    const int64_t o = c->out_off[r->slot];
+    SHIP(hipMemcpy2DAsync(r->out, sizeof(double) * r->n, srv.out + o, sizeof(double) * srv.cap, sizeof(double) * r->n,
+                          kNHot, hipMemcpyDeviceToHost, srv.stream));
+    SHIP(hipMemcpyAsync(r->out_mult, srv.out_mult + o, sizeof(int64_t) * r->n, hipMemcpyDeviceToHost, srv.stream));
+    r->state = kReqLeaving;
+  }
+  if (!leaving.empty()) {
+    hipEvent_t ev;
+    SHIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    SHIP(hipEventRecord(ev, srv.stream));
+    srv.pending.push_back({ev, leaving});
+  }
+  for (SReq* r : srv.active) r->steps_done++;
+  std::vector<SReq*> stay;
+  {
+    std::lock_guard<std::mutex> lk(srv.mu);
+    for (SReq* r : srv.active) {
+      if (r->state == kReqLeaving) srv.slot_used[r->slot] = false;
+      else stay.push_back(r);
+    }
+  }
+  srv.active.swap(stay);
+  srv.max_pool = std::max(srv.max_pool, srv.hw);
+  // the step wrote every stepped droplet to slots 0..n_step-1 (leaving ones
+  // with key -1, dropped by the next sort); an empty pool starts over
+  srv.hw = srv.active.empty() ? 0 : n_step;
+  srv.cur = 1 - srv.cur;
+  srv.iterations++;
+  srv.droplet_steps += n_step;
+  return serve_retire(false);
+}
+
+void serve_loop() {
+  if (hipSetDevice(srv.device) != hipSuccess) {
+    serve_fail(NTLP_E_HIP, "hipSetDevice in the serving thread failed");
+    return;
+  }
+  for (;;) {
+    std::vector<SReq*> admit;
+    {
+      std::unique_lock<std::mutex> lk(srv.mu);
+      srv.cv_work.wait(lk, [] {
+        return srv.stop || srv.err || !srv.queue.empty() || !srv.active.empty() || !srv.pending.empty();
+      });
+      if (srv.err) break;
+      if (srv.stop && srv.queue.empty() && srv.active.empty() && srv.pending.empty()) break;
+      int64_t room = srv.cap - srv.hw;
+      while (!srv.queue.empty()) {
+        SReq* r = srv.queue.front();
+        int slot = -1;
+        for (int s = 0; s < kServeSlots; ++s)
+          if (!srv.slot_used[s]) {
+            slot = s;
+            break;
+          }
+        if (slot < 0 || r->n > room) break;  // FIFO: the head waits for room
+        srv.queue.pop_front();
+        srv.slot_used[slot] = true;
+        r->slot = slot;
+        room -= r->n;
+        admit.push_back(r);
+      }
+    }
+    int rc = 0;
+    if (admit.empty() && srv.active.empty()) rc = serve_retire(true);
+    else rc = serve_iteration(admit);
+    if (rc) break;
+  }
+  // on an error every request still in flight fails
+  std::lock_guard<std::mutex> lk(srv.mu);
+  if (srv.err) {
+    for (auto& kv : srv.reqs)
+      if (kv.second->state != kReqDone) kv.second->state = kReqFailed;
+  }
+  srv.cv_done.notify_all();
+}
+
+}  // namespace
+
+extern "C" {
+
+int ntlp_serve_open(const ntlp_serve_config* cfg, int64_t* capacity) {
+  CHK(need_fields());
+  if (srv.open) return fail(NTLP_E_STATE, "the serving loop is already open");
+  if (!cfg) return fail(NTLP_E_INVALID, "null config");
+  CHK(check_prec(cfg->precision));
+  if (cfg->steps < 1) return fail(NTLP_E_INVALID, "steps must be >= 1");
+  const int64_t cap = cfg->hbm_budget_bytes / kServeBytesPerDroplet;
+  if (cap < 4096) return fail(NTLP_E_RANGE, "HBM budget below 4096 droplets");
+  if (cap > INT32_MAX) return fail(NTLP_E_RANGE, "HBM budget above 2^31 droplets");
+  srv.device = 0;
+  HIPCHK(hipGetDevice(&srv.device));
+  srv.steps = cfg->steps;
+  srv.prec = cfg->precision;
+  srv.use_graph = cfg->use_graph != 0;
+  srv.cap = cap;
+  int rc = 0;
+  auto alloc_all = [&]() -> int {
+    HIPCHK(hipStreamCreateWithFlags(&srv.stream, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b) {
+      CHK(dalloc(srv.hot[b], kNHot * cap, "serving pool"));
+      CHK(dalloc(srv.mult[b], cap, "serving pool"));
+      CHK(dalloc(srv.home[b], cap, "serving pool"));
+      CHK(dalloc(srv.keys[b], cap, "serving pool"));
+    }
+    CHK(dalloc(srv.perm, cap, "serving pool"));
+    CHK(dalloc(srv.counts, ctx.ncell, "serving bins"));
+    CHK(dalloc(srv.offsets, ctx.ncell + 1, "serving bins"));
+    CHK(dalloc(srv.bsum, ctx.nbsum, "serving bins"));
+    CHK(dalloc(srv.in, 6 * cap, "serving admission staging"));
+    CHK(dalloc(srv.in_mult, cap, "serving admission staging"));
+    CHK(dalloc(srv.out, kNHot * cap, "serving output rows"));
+    CHK(dalloc(srv.out_mult, cap, "serving output rows"));
+    CHK(dalloc(srv.ctl_dev, 1, "serving control"));
+    CHK(dalloc(srv.tally, kTallySlots * kTallyWidth, "serving counters"));
+    for (int i = 0; i < kServeCtlRing; ++i) {
+      HIPCHK(hipHostMalloc((void**)&srv.ctl_host[i], sizeof(ServeCtl), hipHostMallocDefault));
+      HIPCHK(hipEventCreateWithFlags(&srv.ctl_ev[i], hipEventDisableTiming));
+    }
+    HIPCHK(hipMemsetAsync(srv.counts, 0, sizeof(int32_t) * ctx.ncell, srv.stream));
+    HIPCHK(hipMemsetAsync(srv.tally, 0, sizeof(unsigned long long) * kTallySlots * kTallyWidth, srv.stream));
+    HIPCHK(hipMemsetAsync(srv.keys[0], 0xff, sizeof(int32_t) * cap, srv.stream));
+    HIPCHK(hipMemsetAsync(srv.keys[1], 0xff, sizeof(int32_t) * cap, srv.stream));
+    // the critical-radius table (setup_radval, particles.f90:5294-5305)
+    static double table[1024];
+    const double radstart = -8.0, radend = -3.0, dr = (radstart - radend) / 1024.0;
+    for (int i = 1; i <= 1024; ++i) table[i - 1] = std::pow(10.0, radstart - (double)(i - 1) * dr);
+    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_radval), table, sizeof(table), 0, hipMemcpyHostToDevice, srv.stream));
+    HIPCHK(hipStreamSynchronize(srv.stream));
+    return 0;
+  };
+  rc = alloc_all();
+  if (rc) {
+    const std::string msg = g_err;
+    serve_release();
+    return fail(rc, msg);
+  }
+  srv.open = true;
+  ctx.serving = true;
+  srv.th = std::thread(serve_loop);
+  if (capacity) *capacity = cap;
+  return NTLP_OK;
+}
+
+int ntlp_serve_submit(const double* in, const int64_t* in_mult, int64_t n, double* out, int64_t* out_mult,
+                      int64_t* ticket) {
+  if (!srv.open) return fail(NTLP_E_STATE, "the serving loop is not open");
+  if (!in || !in_mult || !out || !out_mult || !ticket) return fail(NTLP_E_INVALID, "null argument");
+  if (n < 1) return fail(NTLP_E_INVALID, "a request needs at least one droplet");
+  if (n > srv.cap || n > kServeIdxMask + 1)
+    return fail(NTLP_E_RANGE, "request larger than the pool (" + std::to_string(srv.cap) + " droplets) or 2^23");
+  SReq* r = new SReq();
+  r->n = n;
+  r->in = in;
+  r->in_mult = in_mult;
+  r->out = out;
+  r->out_mult = out_mult;
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(srv.mu);
+    if (srv.err) {
+      delete r;
+      return fail(srv.err, srv.err_msg);
+    }
+    r->ticket = srv.next_ticket++;
+    srv.reqs[r->ticket] = r;
+    srv.queue.push_back(r);
+    srv.submitted++;
+  }
+  srv.cv_work.notify_one();
+  *ticket = r->ticket;
+  return NTLP_OK;
+}
+
+int ntlp_serve_poll(int64_t ticket) {
+  if (!srv.open) return fail(NTLP_E_STATE, "the serving loop is not open");
+  std::lock_guard<std::mutex> lk(srv.mu);
+  auto it = srv.reqs.find(ticket);
+  if (it == srv.reqs.end()) return fail(NTLP_E_INVALID, "unknown ticket");
+  if (it->second->state == kReqFailed) return fail(srv.err ? srv.err : NTLP_E_HIP, srv.err_msg);
+  return it->second->state == kReqDone ? 1 : 0;
+}
+
+int ntlp_serve_wait(int64_t ticket, double* times_ms) {
+  if (!srv.open) return fail(NTLP_E_STATE, "the serving loop is not open");
+  std::unique_lock<std::mutex> lk(srv.mu);
+  auto it = srv.reqs.find(ticket);
+  if (it == srv.reqs.end()) return fail(NTLP_E_INVALID, "unknown ticket");
+  SReq* r = it->second;
+  srv.cv_done.wait(lk, [r] { return r->state == kReqDone || r->state == kReqFailed; });
+  if (r->state == kReqFailed) return fail(srv.err ? srv.err : NTLP_E_HIP, srv.err_msg);
+  if (times_ms) {
+    times_ms[0] = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    times_ms[1] = std::chrono::duration<double, std::milli>(r->t_admit - r->t_submit).count();
+  }
+  srv.reqs.erase(it);
+  delete r;
+  return NTLP_OK;
+}
+
+int ntlp_serve_stats(ntlp_serve_totals* out) {
+  if (!srv.open) return fail(NTLP_E_STATE, "the serving loop is not open");
+  if (!out) return fail(NTLP_E_INVALID, "null output");
+  memset(out, 0, sizeof(*out));
+  {
+    std::lock_guard<std::mutex> lk(srv.mu);
+    out->submitted = srv.submitted;
+    out->completed = srv.completed;
+    out->queued = (int64_t)srv.queue.size();
+  }
+  return NTLP_OK;
+}
+
+int ntlp_serve_close(ntlp_serve_totals* final_stats) {
+  if (!srv.open) return fail(NTLP_E_STATE, "the serving loop is not open");
+  {
+    std::lock_guard<std::mutex> lk(srv.mu);
+    srv.stop = true;
+  }
+  srv.cv_work.notify_all();
+  if (srv.th.joinable()) srv.th.join();
+  const int err = srv.err;
+  const std::string msg = srv.err_msg;
+  if (final_stats) {
+    memset(final_stats, 0, sizeof(*final_stats));
+    final_stats->submitted = srv.submitted;
+    final_stats->completed = srv.completed;
+    final_stats->iterations = srv.iterations;
+    final_stats->droplet_steps = srv.droplet_steps;
+    final_stats->max_pool = srv.max_pool;
+    final_stats->graph_launches = srv.graph_launches;
+    final_stats->graphs = (int64_t)srv.graphs.size();
+    final_stats->capacity = srv.cap;
+    std::vector<unsigned long long> t((size_t)kTallySlots * kTallyWidth);
+    if (hipMemcpy(t.data(), srv.tally, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost) == hipSuccess) {
+      int64_t v[kTallyWidth] = {};
+      for (int s = 0; s < kTallySlots; ++s)
+        for (int c = 0; c < kTallyWidth; ++c) v[c] += (int64_t)t[(size_t)s * kTallyWidth + c];
+      ntlp_counters& nc = final_stats->newton;
+      nc.num100 = v[kTNum100];
+      nc.numimpos = v[kTNumImpos];
+      nc.actnum = v[kTActnum];
+      nc.denum = v[kTDenum];
+      nc.gn_iters = v[kTGnIters];
+      nc.lm_iters = v[kTLmIters];
+      nc.solves = v[kTSolves];
+      nc.steps = srv.iterations;
+      for (int k = 0; k < NTLP_GN_HIST; ++k) nc.gn_hist[k] = v[kTHist0 + k];
+    }
+  }
+  if (srv.stream) (void)hipStreamSynchronize(srv.stream);
+  serve_release();
+  ctx.serving = false;
+  if (err) return fail(err, msg);
+  return NTLP_OK;
+}
+
+}  // extern "C"
