@@ -142,6 +142,159 @@ def cpu_baseline(wl_id: int, seed: int, target_s: float = 12.0):
                       f"ensemble and fields, {reps} consecutive steps, {t:.1f}s"}
 
 
+def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budget_bytes: int = 32 << 30,
+                n_templates: int = 4, nmax: int = 4_000_000):
+    """The serving stream against resident 128^3 fields: Poisson arrivals at
+    `rate` req/s for `duration_s`, request sizes log-uniform on [1e3, 4e6]
+    droplets, each request advancing `steps` mixed-precision BE steps.
+
+    Request payloads are windows (random offset, the request's size) of
+    `n_templates` droplet ensembles generated up front in pinned memory, each
+    with its own seed, radius median drawn from 2-20 um and aerosol median
+    from 0.02-0.2 um (generating every request in Python at 200 req/s would
+    measure numpy, not the server).  Results land in a pinned FIFO ring.
+    Reports p50/p99 latency (submission to result in host memory), then the
+    throughput of a saturated stream against the plain batched step on the
+    pool size the saturated stream kept."""
+    import queue
+    import threading
+
+    from ntlp_amd import (MODE_BE, PREC_MIXED, DropletStep, Flags, Particles, PinnedArray, ServeLoop, SERVE_IN,
+                          synth)
+
+    wl = synth.workload(4, n=1, seed=seed)
+    flags = Flags(icouple=0, iTcouple=0, iHcouple=0)
+    s = DropletStep(wl.grid, wl.consts, flags)
+    s.set_fields(*wl.fields)
+    g = wl.grid
+    t = time.time()
+    bank = PinnedArray((n_templates, 6, nmax))
+    bmult = PinnedArray((n_templates, nmax), np.int64)
+    meta = []
+    for k in range(n_templates):
+        rng = np.random.default_rng(1000 * seed + k)
+        r_med, a_med = rng.uniform(2e-6, 20e-6), rng.uniform(0.02e-6, 0.2e-6)
+        b = bank.array[k]
+        b[0] = rng.uniform(0, g.xl, nmax)
+        b[1] = rng.uniform(0, g.yl, nmax)
+        b[2] = rng.uniform(0, g.zl, nmax)
+        b[3] = synth.lognormal(rng, nmax, r_med, 0.4)
+        b[4] = synth.dry_mass(synth.lognormal(rng, nmax, a_med, 0.5))
+        b[5] = 1.2
+        bmult.array[k] = 1
+        meta.append((r_med, a_med))
+    log(f"[serve] request bank: {n_templates} ensembles x {nmax} droplets in {time.time() - t:.1f}s")
+    ring_rows = 3 * nmax
+    outbuf = PinnedArray((14, ring_rows))
+    outmult = PinnedArray((ring_rows,), np.int64)
+
+    class OutRing:  # FIFO rows of the pinned result buffer
+        def __init__(self):
+            self.head, self.blocks, self.cv = 0, [], threading.Condition()
+
+        def take(self, n):
+            with self.cv:
+                while True:
+                    start = self.head if self.head + n <= ring_rows else 0
+                    busy = [(a, b) for a, b in self.blocks]
+                    if all(start + n <= a or start >= b for a, b in busy):
+                        self.blocks.append((start, start + n))
+                        self.head = start + n
+                        return start
+                    self.cv.wait()
+
+        def give(self):
+            with self.cv:
+                self.blocks.pop(0)
+                self.cv.notify_all()
+
+    def run_stream(loop, arrivals, sizes, rng):
+        ring = OutRing()
+        done_q = queue.Queue()
+        lat, qd, comp = [], [], []
+
+        def waiter():
+            while True:
+                item = done_q.get()
+                if item is None:
+                    return
+                tk, n = item
+                a, b = loop.wait(tk)
+                lat.append(a)
+                qd.append(b)
+                comp.append(time.perf_counter())
+                ring.give()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for i, (ta, n) in enumerate(zip(arrivals, sizes)):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            k = i % n_templates
+            o = int(rng.integers(0, nmax - n + 1))
+            row = ring.take(n)
+            tk = loop.submit(bank.array[k][:, o:o + n], bmult.array[k][o:o + n], outbuf.array[:, row:row + n],
+                             outmult.array[row:row + n])
+            done_q.put((tk, n))
+        done_q.put(None)
+        th.join()
+        return np.array(lat), np.array(qd), t0, max(comp)
+
+    rng = np.random.default_rng(seed + 77)
+    nreq = max(1, int(rate * duration_s))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    sizes = np.exp(rng.uniform(np.log(1e3), np.log(4e6), nreq)).astype(np.int64)
+    loop = ServeLoop(s, budget_bytes, steps=steps, precision=PREC_MIXED)
+    lat, qd, t0, t1 = run_stream(loop, arrivals, sizes, rng)
+    tot = loop.close()
+    work = float(sizes.sum()) * steps
+    stream = {"requests": int(nreq), "rate_req_per_s": rate, "duration_s": float(arrivals[-1]),
+              "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+              "p50_queue_ms": float(np.percentile(qd, 50)), "p99_queue_ms": float(np.percentile(qd, 99)),
+              "throughput_du_s": work / (t1 - t0), "offered_du_s": rate * float(sizes.mean()) * steps,
+              "mean_pool": tot["droplet_steps"] / max(tot["iterations"], 1), "iterations": tot["iterations"],
+              "graphs": tot["graphs"], "capacity": tot["capacity"]}
+    log(f"[serve] stream: p50 {stream['p50_ms']:.1f} ms, p99 {stream['p99_ms']:.1f} ms, "
+        f"{stream['throughput_du_s']:.3g} du/s")
+
+    # saturated: arrivals all at t = 0 (a closed loop bounded by the result ring)
+    nsat = max(16, nreq // 4)
+    sizes_s = np.exp(rng.uniform(np.log(1e3), np.log(4e6), nsat)).astype(np.int64)
+    loop = ServeLoop(s, budget_bytes, steps=steps, precision=PREC_MIXED)
+    lat_s, _, t0, t1 = run_stream(loop, np.zeros(nsat), sizes_s, rng)
+    tot_s = loop.close()
+    sat_tp = float(sizes_s.sum()) * steps / (t1 - t0)
+    mean_pool = tot_s["droplet_steps"] / max(tot_s["iterations"], 1)
+    # the plain batched step on a pool of that size
+    nb = int(min(max(mean_pool, 1e4), nmax))
+    p = Particles.empty(nb)
+    for c, name in enumerate(SERVE_IN):
+        p[name] = bank.array[0][c, :nb]
+    s.put(p)
+    s.interp(set_vp=True, set_tp=True)
+    s.crit_radius()
+    s.step(MODE_BE, PREC_MIXED, nsteps=2, it0=2)
+    s.sync()
+    tb = time.perf_counter()
+    s.step(MODE_BE, PREC_MIXED, nsteps=steps, it0=4)
+    s.sync()
+    batched = nb * steps / (time.perf_counter() - tb)
+    saturated = {"requests": int(nsat), "throughput_du_s": sat_tp, "mean_pool": mean_pool,
+                 "p50_ms": float(np.percentile(lat_s, 50)), "p99_ms": float(np.percentile(lat_s, 99)),
+                 "batched_du_s": batched, "batched_droplets": nb, "sustained_over_batched": sat_tp / batched}
+    log(f"[serve] saturated {sat_tp:.3g} du/s vs batched {batched:.3g} du/s on {nb} droplets")
+    s.close()
+    bank.free()
+    bmult.free()
+    outbuf.free()
+    outmult.free()
+    return {"workload": "128^3 resident fields, Poisson arrivals, sizes log-uniform 1e3-4e6, "
+                        f"{steps} BE steps each, mixed precision, HBM budget {budget_bytes >> 30} GiB",
+            "stream": stream, "saturated": saturated}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +305,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -172,6 +329,10 @@ def main():
         sys.exit(2)
 
     from ntlp_amd import MODE_BE, PREC_FP64, PREC_MIXED
+
+    if args.serve_only:
+        print(json.dumps(serve_bench(args.seed, args.serve_seconds, args.serve_rate)), flush=True)
+        return
 
     wl = prepare(args.workload, args.droplets, args.seed)
     s = init_device(wl)
@@ -250,6 +411,12 @@ def main():
         cpu = cpu_baseline(args.workload, args.seed)
         log(f"[bench] cpu baseline {cpu['value']:.3g} droplet-updates/s")
 
+    serve = None
+    if not args.no_serve:
+        s.close()
+        log("[bench] serving stream ...")
+        serve = serve_bench(args.seed, args.serve_seconds, args.serve_rate)
+
     value = n * args.steps / t_mixed
     per_step_iters = (rows[:, 4] / np.maximum(rows[:, 6], 1)).tolist() if len(rows) else []
     out = {
@@ -278,6 +445,7 @@ def main():
         "kernel_ms": {k: v[1] / v[0] for k, v in ktimes.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

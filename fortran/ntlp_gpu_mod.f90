@@ -8,6 +8,8 @@
 !   les.F:404  call particle_update_rk3(istage)  ->  call gpu_particle_update_rk3(istage, it)
 !   les.F:422  call particle_update_BE           ->  call gpu_particle_update_BE(it)
 !   les.F:431  call particle_coupling_update     ->  call gpu_particle_coupling_update(partsrc, ...)
+!   les.F:435  call particle_coupling_exchange   ->  call gpu_particle_coupling_exchange(partsrc, ...)
+!   les.F:439  call apply_particle_coupling      ->  call gpu_apply_particle_coupling(t1, t2)
 !   particles.f90:91  fill_ext                   ->  call gpu_set_fields(u, v, w, T, q, layout)
 !
 ! Every wrapper stops the program with the library's error text when a call
@@ -60,10 +62,12 @@ module ntlp_gpu
        ntlp_gpu_particles_get_records, ntlp_gpu_particles_count, ntlp_gpu_interp, ntlp_gpu_crit_radius, &
        ntlp_gpu_particle_update_rk3, ntlp_gpu_particle_update_be, ntlp_gpu_particle_update_rk3_prec, &
        ntlp_gpu_particle_update_be_prec, ntlp_gpu_particle_coupling_update, ntlp_gpu_get_sources, &
+       ntlp_gpu_particle_coupling_exchange, ntlp_gpu_apply_particle_coupling, &
        ntlp_gpu_bin, ntlp_gpu_wcell_counts, ntlp_gpu_num_ucells, ntlp_gpu_step, ntlp_gpu_counters, &
        ntlp_gpu_step_counters, ntlp_gpu_sync
   public :: gpu_check, gpu_setup, gpu_set_fields, gpu_put, gpu_get, gpu_interp, gpu_crit_radius, &
-       gpu_particle_update_rk3, gpu_particle_update_BE, gpu_particle_coupling_update, gpu_step, gpu_counters
+       gpu_particle_update_rk3, gpu_particle_update_BE, gpu_particle_coupling_update, gpu_step, gpu_counters, &
+       gpu_particle_coupling_exchange, gpu_apply_particle_coupling
 
   interface
      integer(c_int32_t) function ntlp_gpu_init(grid, consts, flags) bind(c, name="ntlp_gpu_init")
@@ -146,6 +150,17 @@ module ntlp_gpu
        type(c_ptr), value :: src_out
        integer(c_int32_t), value :: deterministic
      end function ntlp_gpu_particle_coupling_update
+     integer(c_int32_t) function ntlp_gpu_particle_coupling_exchange(partsrc, partTsrc, partHsrc, partTEsrc) &
+          bind(c, name="ntlp_gpu_particle_coupling_exchange")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: partsrc, partTsrc, partHsrc, partTEsrc
+     end function ntlp_gpu_particle_coupling_exchange
+     integer(c_int32_t) function ntlp_gpu_apply_particle_coupling(T, q, layout) &
+          bind(c, name="ntlp_gpu_apply_particle_coupling")
+       import :: c_int32_t, c_double
+       real(c_double), intent(inout) :: T(*), q(*)
+       integer(c_int32_t), value :: layout
+     end function ntlp_gpu_apply_particle_coupling
      integer(c_int32_t) function ntlp_gpu_get_sources(src_out) bind(c, name="ntlp_gpu_get_sources")
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: src_out
@@ -285,6 +300,21 @@ contains
     partHsrc_t = reshape(buf(4 * m + 1:5 * m), shape(partHsrc_t))
     partTEsrc_t = reshape(buf(5 * m + 1:6 * m), shape(partTEsrc_t))
   end subroutine gpu_particle_coupling_update
+
+  ! particle_coupling_exchange (particles.f90:1066-1149) on one periodic domain:
+  ! partsrc(1:nnx,1:nny,0:nnz+1,1:3), partTsrc, partHsrc, partTEsrc(1:nnx,1:nny,0:nnz+1).
+  subroutine gpu_particle_coupling_exchange(partsrc, partTsrc, partHsrc, partTEsrc)
+    real(c_double), intent(out), target :: partsrc(:, :, :, :), partTsrc(:, :, :), partHsrc(:, :, :), partTEsrc(:, :, :)
+    call gpu_check(ntlp_gpu_particle_coupling_exchange(c_loc(partsrc), c_loc(partTsrc), c_loc(partHsrc), &
+         c_loc(partTEsrc)), 'particle_coupling_exchange')
+  end subroutine gpu_particle_coupling_exchange
+
+  ! apply_particle_coupling (particles.f90:1278-1313) on t(:,:,1,:) and t(:,:,2,:)
+  ! given as x-pencil arrays (1:nnx+2, 1:nny, 0:nnz+1).
+  subroutine gpu_apply_particle_coupling(T, q)
+    real(c_double), intent(inout) :: T(:, :, :), q(:, :, :)
+    call gpu_check(ntlp_gpu_apply_particle_coupling(T, q, NTLP_LAYOUT_XPENCIL), 'apply_particle_coupling')
+  end subroutine gpu_apply_particle_coupling
 
   ! nsteps fused droplet steps (gather, advance, boundary conditions, coupling, sort).
   subroutine gpu_step(mode, prec, nsteps, it0)

@@ -14,7 +14,8 @@
  *    duration of the call.
  *  - One process-wide context on the current HIP device, like the module
  *    globals of particles.f90.  Calls from several host threads must be
- *    serialised by the caller.
+ *    serialised by the caller, except ntlp_serve_submit/poll/wait/stats,
+ *    which any thread may call while the serving loop is open.
  *  - Droplets keep the order in which they were put: ntlp_gpu_particles_get
  *    returns them in that order, whatever order the device keeps internally.
  */
@@ -170,6 +171,19 @@ int ntlp_gpu_particle_coupling_update(double *src_out, int32_t deterministic);
 /* Copies the resident coupling sources (same layout as src_out above) as the
  * last coupling update or fused step left them. */
 int ntlp_gpu_get_sources(double *src_out);
+/* particle_coupling_exchange (particles.f90:1066-1149) on one periodic domain:
+ * folds the +1 ghost plane of the resident sources (x = nnx+1 onto x = 1, then
+ * y = nny+1 onto y = 1, the reference's order of additions) and transposes
+ * them to the flow solver's layout (ztox_trans).  Each non-NULL pointer
+ * receives partsrc(1:nnx,1:nny,0:nnz+1,1:3), partTsrc, partHsrc or
+ * partTEsrc(1:nnx,1:nny,0:nnz+1).  The resident unfolded sources are kept. */
+int ntlp_gpu_particle_coupling_exchange(double *partsrc, double *partTsrc, double *partHsrc, double *partTEsrc);
+/* apply_particle_coupling (particles.f90:1278-1313): T and q (t(:,:,1,:) and
+ * t(:,:,2,:)) in NTLP_LAYOUT_XPENCIL (1:nnx+2, 1:nny, 0:nnz+1) or
+ * NTLP_LAYOUT_CORE get dt times the exchanged sources added at levels
+ * 1..nnz, in place; the resident T and q that the next step gathers are
+ * replaced by the results. */
+int ntlp_gpu_apply_particle_coupling(double *T, double *q, int32_t layout);
 
 /* ------------------------------------------------------------- binning */
 
@@ -223,11 +237,11 @@ int ntlp_gpu_sync(void);
  * pool size class when use_graph=1) and releases the requests that finish.
  * A request's result does not depend on what else is in the pool.
  *
- * Request input  in[6*n]: column blocks x, y, z, radius, m_s, kappa_s (each n
- *                values), in_mult[n].
- * Request output out[NTLP_SERVE_COLS*n]: column blocks x, y, z, vx, vy, vz,
- *                Tp, radius, m_s, kappa_s, rc, res, actres, numact;
- *                out_mult[n].
+ * Request input  in: 6 column blocks x, y, z, radius, m_s, kappa_s (n values
+ *                each, ld_in apart), in_mult[n].
+ * Request output out: NTLP_SERVE_COLS column blocks x, y, z, vx, vy, vz, Tp,
+ *                radius, m_s, kappa_s, rc, res, actres, numact (ld_out
+ *                apart); out_mult[n].
  * The four buffers are read and written asynchronously by the serving thread
  * and must stay valid until ntlp_serve_wait returns for the ticket (pinned
  * host memory makes the copies asynchronous DMA).
@@ -248,9 +262,11 @@ typedef struct ntlp_serve_totals {
   ntlp_counters newton; /* Newton bookkeeping over every served droplet-step */
 } ntlp_serve_totals;
 int ntlp_serve_open(const ntlp_serve_config *cfg, int64_t *capacity);
-/* Queues a request (1 <= n <= capacity, n <= 2^23) and returns at once. */
-int ntlp_serve_submit(const double *in, const int64_t *in_mult, int64_t n, double *out, int64_t *out_mult,
-                      int64_t *ticket);
+/* Queues a request (1 <= n <= capacity, n <= 2^23) and returns at once.
+ * ld_in / ld_out: distance in elements between the column blocks of in and
+ * out (0 means n). */
+int ntlp_serve_submit(const double *in, int64_t ld_in, const int64_t *in_mult, int64_t n, double *out,
+                      int64_t ld_out, int64_t *out_mult, int64_t *ticket);
 /* 1 when the ticket's result is in place, 0 while it is queued or running. */
 int ntlp_serve_poll(int64_t ticket);
 /* Blocks until the result is in place; times_ms (may be NULL) receives the
@@ -262,6 +278,9 @@ int ntlp_serve_stats(ntlp_serve_totals *out);
 /* Serves every queued request to the end, stops the loop and releases the
  * pool; final_stats (may be NULL) receives the loop's totals. */
 int ntlp_serve_close(ntlp_serve_totals *final_stats);
+/* Page-locked host memory for request buffers (hipHostMalloc / hipHostFree). */
+int ntlp_host_alloc(int64_t bytes, void **ptr);
+int ntlp_host_free(void *ptr);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,7 @@ import numpy as np
 from . import grid as _grid  # noqa: F401  (re-exported helpers)
 from .grid import Grid
 
-__all__ = ["Grid", "Consts", "Flags", "Particles", "DropletStep", "ServeLoop", "NtlpError", "load_library",
+__all__ = ["Grid", "Consts", "Flags", "Particles", "DropletStep", "ServeLoop", "PinnedArray", "NtlpError", "load_library",
            "NREAL", "COL", "MODE_RK3", "MODE_BE", "PREC_FP64", "PREC_MIXED", "PREC_FP64_AJ",
            "LAYOUT_EXT", "LAYOUT_CORE", "LAYOUT_XPENCIL", "LIB_PATH"]
 
@@ -216,6 +216,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_particle_update_be_prec": (ctypes.c_int, [i32, i32]),
         "ntlp_gpu_particle_coupling_update": (ctypes.c_int, [vp, i32]),
         "ntlp_gpu_get_sources": (ctypes.c_int, [vp]),
+        "ntlp_gpu_particle_coupling_exchange": (ctypes.c_int, [vp, vp, vp, vp]),
+        "ntlp_gpu_apply_particle_coupling": (ctypes.c_int, [vp, vp, i32]),
         "ntlp_gpu_bin": (ctypes.c_int, [vp, vp, vp, vp]),
         "ntlp_gpu_wcell_counts": (ctypes.c_int, [vp]),
         "ntlp_gpu_num_ucells": (i64, []),
@@ -226,7 +228,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_kernel_times": (ctypes.c_int, [vp, vp, vp, i32]),
         "ntlp_gpu_sync": (ctypes.c_int, []),
         "ntlp_serve_open": (ctypes.c_int, [ctypes.POINTER(_CServeConfig), ctypes.POINTER(i64)]),
-        "ntlp_serve_submit": (ctypes.c_int, [vp, vp, i64, vp, vp, ctypes.POINTER(i64)]),
+        "ntlp_serve_submit": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, vp, ctypes.POINTER(i64)]),
+        "ntlp_host_alloc": (ctypes.c_int, [i64, ctypes.POINTER(vp)]),
+        "ntlp_host_free": (ctypes.c_int, [vp]),
         "ntlp_serve_poll": (ctypes.c_int, [i64]),
         "ntlp_serve_wait": (ctypes.c_int, [i64, vp]),
         "ntlp_serve_stats": (ctypes.c_int, [ctypes.POINTER(_CServeTotals)]),
@@ -333,6 +337,28 @@ class DropletStep:
         self._check(self.lib.ntlp_gpu_get_sources(_ptr(out)))
         return out
 
+    def particle_coupling_exchange(self):
+        """particle_coupling_exchange (particles.f90:1066-1149): the folded
+        sources in the flow solver's x-pencil layout, as C arrays
+        partsrc (3, nnz+2, nny, nnx) and partTsrc, partHsrc, partTEsrc
+        (nnz+2, nny, nnx)."""
+        g = self.grid
+        partsrc = np.empty((3, g.nnz + 2, g.nny, g.nnx))
+        rest = [np.empty((g.nnz + 2, g.nny, g.nnx)) for _ in range(3)]
+        self._check(self.lib.ntlp_gpu_particle_coupling_exchange(_ptr(partsrc), *[_ptr(a) for a in rest]))
+        return (partsrc, *rest)
+
+    def apply_particle_coupling(self, T: np.ndarray, q: np.ndarray, layout: int = LAYOUT_XPENCIL) -> None:
+        """apply_particle_coupling (particles.f90:1278-1313) on T and q in place
+        (x-pencil C arrays (nnz+2, nny, nnx+2), or core (nnx, nny, nnz+2)); the
+        resident T and q become the results."""
+        g = self.grid
+        shape = (g.nnz + 2, g.nny, g.nnx + 2) if layout == LAYOUT_XPENCIL else (g.nnx, g.nny, g.nnz + 2)
+        for a in (T, q):
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.shape != shape:
+                raise ValueError(f"T and q must be contiguous float64 arrays of shape {shape}")
+        self._check(self.lib.ntlp_gpu_apply_particle_coupling(_ptr(T), _ptr(q), layout))
+
     # -------------------------------------------------------------- binning
     def bin(self):
         n, nc = self.n, int(self.lib.ntlp_gpu_num_ucells())
@@ -416,14 +442,18 @@ class ServeLoop:
         return self.step._check(rc)
 
     def submit(self, inp: np.ndarray, mult: np.ndarray, out: np.ndarray, out_mult: np.ndarray) -> int:
+        """inp (6, n) and out (14, n) may be column slices of wider arrays (rows
+        contiguous, any row stride); mult and out_mult are contiguous."""
         n = int(inp.shape[1])
-        for a, shape, dt in ((inp, (6, n), np.float64), (mult, (n,), np.int64), (out, (14, n), np.float64),
-                             (out_mult, (n,), np.int64)):
-            if a.shape != shape or a.dtype != dt or not a.flags.c_contiguous:
-                raise ValueError(f"serving buffers must be C-contiguous {dt.__name__} of shape {shape}")
+        for a, shape in ((inp, (6, n)), (out, (14, n))):
+            if a.shape != shape or a.dtype != np.float64 or a.strides[1] != 8 or a.strides[0] % 8:
+                raise ValueError(f"serving arrays must be float64 of shape {shape} with contiguous rows")
+        for a in (mult, out_mult):
+            if a.shape != (n,) or a.dtype != np.int64 or not a.flags.c_contiguous:
+                raise ValueError(f"multiplicities must be contiguous int64 of shape ({n},)")
         t = ctypes.c_int64()
-        self._check(self.lib.ntlp_serve_submit(_ptr(inp), _ptr(mult), n, _ptr(out), _ptr(out_mult),
-                                               ctypes.byref(t)))
+        self._check(self.lib.ntlp_serve_submit(_ptr(inp), inp.strides[0] // 8, _ptr(mult), n, _ptr(out),
+                                               out.strides[0] // 8, _ptr(out_mult), ctypes.byref(t)))
         self._held[t.value] = (inp, mult, out, out_mult)
         return int(t.value)
 
@@ -451,3 +481,31 @@ class ServeLoop:
         d["newton"] = {n: int(getattr(c, n)) for n in
                        ("num100", "numimpos", "actnum", "denum", "gn_iters", "lm_iters", "solves", "steps")}
         return d
+
+
+class PinnedArray:
+    """Page-locked host memory from the library (hipHostMalloc), viewed as a
+    numpy array; freed by :meth:`free` or when garbage collected."""
+
+    def __init__(self, shape, dtype=np.float64):
+        self.lib = load_library()
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = ctypes.c_void_p()
+        rc = self.lib.ntlp_host_alloc(nbytes, ctypes.byref(p))
+        if rc != 0:
+            raise NtlpError(rc, self.lib.ntlp_gpu_last_error().decode())
+        self._p = p
+        buf = (ctypes.c_char * max(nbytes, 1)).from_address(p.value)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self) -> None:
+        if self._p is not None:
+            self.array = None
+            self.lib.ntlp_host_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

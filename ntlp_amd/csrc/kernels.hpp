@@ -763,6 +763,91 @@ __global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
   }
 }
 
+// ------------------------------------------- coupling exchange and apply
+
+// particle_coupling_exchange (particles.f90:1066-1149) on one periodic
+// domain, fused with its ztox_trans: the +1 ghost plane x = nnx+1 folds onto
+// x = 1 (all rows, the ghost row included), then the ghost row y = nny+1 onto
+// y = 1, in the reference's order of additions; the result is written in the
+// flow solver's x-pencil layout xsrc[f][k][iy][ix] (k = 0..nnz+1).  One
+// workgroup transposes a 64 (k) x 32 (ix) tile of one row iy through LDS so
+// that both the k-fastest reads and the x-fastest writes are coalesced.
+constexpr int kFoldK = 64, kFoldX = 32;
+
+__global__ void __launch_bounds__(kBlock) k_fold_sources(const double* src, int64_t nsrc, int nnx, int nny, int nnz,
+                                                         double* xsrc) {
+  __shared__ double tile[kFoldX][kFoldK + 1];
+  const int nz2 = nnz + 2;
+  const int ntk = (nz2 + kFoldK - 1) / kFoldK, ntx = (nnx + kFoldX - 1) / kFoldX;
+  int64_t b = blockIdx.x;
+  const int xt = (int)(b % ntx);
+  b /= ntx;
+  const int kt = (int)(b % ntk);
+  b /= ntk;
+  const int iy = (int)(b % nny) + 1;
+  const int f = (int)(b / nny);
+  const double* s = src + (int64_t)f * nsrc;
+  const int64_t sj = nz2, si = (int64_t)nz2 * (nny + 1);
+  const int K0 = kt * kFoldK, X0 = 1 + xt * kFoldX;
+  for (int e = threadIdx.x; e < kFoldK * kFoldX; e += blockDim.x) {
+    const int kk = e % kFoldK, xx = e / kFoldK;
+    const int k = K0 + kk, ix = X0 + xx;
+    if (k < nz2 && ix <= nnx) {
+      double v = s[k + sj * (iy - 1) + si * (ix - 1)];
+      if (ix == 1) v = v + s[k + sj * (iy - 1) + si * nnx];
+      if (iy == 1) {
+        double w = s[k + sj * nny + si * (ix - 1)];
+        if (ix == 1) w = w + s[k + sj * nny + si * nnx];
+        v = v + w;
+      }
+      tile[xx][kk] = v;
+    }
+  }
+  __syncthreads();
+  double* o = xsrc + (int64_t)f * nz2 * nny * nnx;
+  for (int e = threadIdx.x; e < kFoldK * kFoldX; e += blockDim.x) {
+    const int xx = e % kFoldX, kk = e / kFoldX;
+    const int k = K0 + kk, ix = X0 + xx;
+    if (k < nz2 && ix <= nnx) o[(ix - 1) + (int64_t)nnx * ((iy - 1) + (int64_t)nny * k)] = tile[xx][kk];
+  }
+}
+
+// apply_particle_coupling (particles.f90:1278-1313): t(ix,iy,1,iz) and
+// t(ix,iy,2,iz) += dt * sources for iz = 1..nnz, on T and q arrays in the
+// x-pencil layout (1:nnx+2, 1:nny, 0:nnz+1) or the core layout (0:nnz+1,
+// 1:nny, 1:nnx).
+__global__ void k_apply_coupling(double* T, double* q, const double* xsrc, DevGrid g, DevConsts C, DevFlags fl,
+                                 int xpencil) {
+  const int nx = g.nnx, ny = g.nny, nz = g.nnz;
+  const int64_t total = (int64_t)nx * ny * nz;
+  const int64_t nf = (int64_t)(nz + 2) * ny * nx;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    int ix, iy, iz;
+    if (xpencil) {
+      ix = (int)(e % nx) + 1;
+      iy = (int)((e / nx) % ny) + 1;
+      iz = (int)(e / ((int64_t)nx * ny)) + 1;
+    } else {
+      iz = (int)(e % nz) + 1;
+      iy = (int)((e / nz) % ny) + 1;
+      ix = (int)(e / ((int64_t)nz * ny)) + 1;
+    }
+    const int64_t xs = (ix - 1) + (int64_t)nx * ((iy - 1) + (int64_t)ny * iz);
+    const int64_t t = xpencil ? (ix - 1) + (int64_t)(nx + 2) * ((iy - 1) + (int64_t)ny * iz)
+                              : iz + (int64_t)(nz + 2) * ((iy - 1) + (int64_t)ny * (ix - 1));
+    double Tv = T[t];
+    if (fl.iTcouple == 1) {
+      if (fl.iexner) Tv = Tv + C.dt * (xsrc[3 * nf + xs] / exner(C, func_p_base(C, g.zz[iz])));
+      else Tv = Tv + C.dt * xsrc[3 * nf + xs];
+    }
+    if (fl.iHcouple == 1) {
+      Tv = Tv + C.dt * xsrc[5 * nf + xs];
+      q[t] = q[t] + C.dt * xsrc[4 * nf + xs];
+    }
+    T[t] = Tv;
+  }
+}
+
 // ---------------------------------------------------------------- binning
 
 struct BinArgs {

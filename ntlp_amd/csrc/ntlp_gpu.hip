@@ -83,6 +83,9 @@ struct Ctx {
   // coupling sources
   double* src = nullptr;
   int64_t nsrc = 0;
+  bool src_valid = false;    // src holds the last coupling update
+  double* xsrc = nullptr;    // folded x-pencil sources of the last exchange
+  bool xsrc_valid = false;
   // counters
   unsigned long long* tally = nullptr;        // kTallyRows per-step rows
   unsigned long long* tally_slots = nullptr;  // kTallySlots x kTallyWidth
@@ -250,6 +253,7 @@ void release_all() {
   dfree(ctx.offsets);
   dfree(ctx.bsum);
   dfree(ctx.src);
+  dfree(ctx.xsrc);
   dfree(ctx.tally);
   dfree(ctx.tally_slots);
   dfree(ctx.d_grid);
@@ -366,6 +370,8 @@ int scatter_atomic() {
 // atomic-free deterministic scatter over the current cell sort (perm and
 // offsets must describe the current slots)
 int pull_sorted() {
+  ctx.src_valid = true;
+  ctx.xsrc_valid = false;
   PullArgs p;
   p.g = ctx.g;
   p.hot = ctx.hot[ctx.cur];
@@ -390,6 +396,8 @@ constexpr int kTileI = 4, kTileJ = 4, kTileK = 32;
 // tiled scatter over the current cell sort (keys, perm and offsets must
 // describe the current slots); writes every node of the 6 fields
 int tile_sorted() {
+  ctx.src_valid = true;
+  ctx.xsrc_valid = false;
   TileArgs t;
   t.g = ctx.g;
   t.hot = ctx.hot[ctx.cur];
@@ -837,6 +845,69 @@ int ntlp_gpu_particle_coupling_update(double* src_out, int32_t deterministic) {
     HIPCHK(hipMemcpyAsync(src_out, ctx.src, sizeof(double) * 6 * ctx.nsrc, hipMemcpyDeviceToHost, ctx.stream));
   HIPCHK(hipStreamSynchronize(ctx.stream));
   return NTLP_OK;
+}
+
+int ntlp_gpu_particle_coupling_exchange(double* partsrc, double* partTsrc, double* partHsrc, double* partTEsrc) {
+  CHK(need_init());
+  if (!ctx.src_valid) return fail(NTLP_E_STATE, "no coupling sources: run a coupling update or a fused step first");
+  const int nx = ctx.g.nnx, ny = ctx.g.nny, nz = ctx.g.nnz;
+  const int64_t nf = (int64_t)(nz + 2) * ny * nx;
+  if (!ctx.xsrc) CHK(dalloc(ctx.xsrc, 6 * nf, "exchanged sources"));
+  const int64_t blocks = 6LL * ny * ((nz + 2 + kFoldK - 1) / kFoldK) * ((nx + kFoldX - 1) / kFoldX);
+  double* xs = ctx.xsrc;
+  const double* src = ctx.src;
+  const int64_t nsrc = ctx.nsrc;
+  CHK(launch("exchange", [&] { k_fold_sources<<<(unsigned)blocks, kBlock, 0, ctx.stream>>>(src, nsrc, nx, ny, nz, xs); }));
+  double* outs[4] = {partsrc, partTsrc, partHsrc, partTEsrc};
+  const int first[4] = {0, 3, 4, 5}, count[4] = {3, 1, 1, 1};
+  for (int i = 0; i < 4; ++i)
+    if (outs[i])
+      HIPCHK(hipMemcpyAsync(outs[i], ctx.xsrc + first[i] * nf, sizeof(double) * count[i] * nf, hipMemcpyDeviceToHost,
+                            ctx.stream));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  ctx.xsrc_valid = true;
+  return NTLP_OK;
+}
+
+int ntlp_gpu_apply_particle_coupling(double* T, double* q, int32_t layout) {
+  CHK(need_fields());
+  if (ctx.serving) return fail(NTLP_E_STATE, "the serving loop reads the fields: close it first");
+  if (!ctx.xsrc_valid) return fail(NTLP_E_STATE, "no exchanged sources: call ntlp_gpu_particle_coupling_exchange first");
+  if (!T || !q) return fail(NTLP_E_INVALID, "null field pointer");
+  if (layout != NTLP_LAYOUT_XPENCIL && layout != NTLP_LAYOUT_CORE)
+    return fail(NTLP_E_INVALID, "apply_particle_coupling takes x-pencil or core arrays");
+  const int nx = ctx.g.nnx, ny = ctx.g.nny, nz = ctx.g.nnz;
+  const int64_t elems = layout == NTLP_LAYOUT_XPENCIL ? (int64_t)(nx + 2) * ny * (nz + 2) : (int64_t)(nz + 2) * ny * nx;
+  double* stage[2] = {nullptr, nullptr};
+  int rc = dalloc(stage[0], elems, "field staging");
+  if (!rc) rc = dalloc(stage[1], elems, "field staging");
+  double* host[2] = {T, q};
+  for (int i = 0; i < 2 && !rc; ++i)
+    if (hipMemcpyAsync(stage[i], host[i], sizeof(double) * elems, hipMemcpyHostToDevice, ctx.stream) != hipSuccess)
+      rc = fail(NTLP_E_HIP, "upload of T/q failed");
+  if (!rc) {
+    const int xp = layout == NTLP_LAYOUT_XPENCIL;
+    rc = launch("apply_coupling", [&] {
+      k_apply_coupling<<<grid_flat((int64_t)nx * ny * nz), kBlock, 0, ctx.stream>>>(stage[0], stage[1], ctx.xsrc,
+                                                                                    ctx.g, ctx.C, ctx.fl, xp);
+    });
+  }
+  for (int i = 0; i < 2 && !rc; ++i) {
+    if (hipMemcpyAsync(host[i], stage[i], sizeof(double) * elems, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess)
+      rc = fail(NTLP_E_HIP, "download of T/q failed");
+    // the resident T (ext 3) and q (ext 4) the next step gathers
+    double* dst = ctx.ext[3 + i];
+    const double* stg = stage[i];
+    if (!rc)
+      rc = launch("fill_ext", [&] {
+        if (layout == NTLP_LAYOUT_CORE) k_fill_ext<<<grid_flat(ctx.ext_elems), kBlock, 0, ctx.stream>>>(stg, dst, nx, ny, nz);
+        else k_xpencil_to_ext<<<grid_flat(ctx.ext_elems), kBlock, 0, ctx.stream>>>(stg, dst, nx, ny, nz);
+      });
+  }
+  if (hipStreamSynchronize(ctx.stream) != hipSuccess && !rc) rc = fail(NTLP_E_HIP, "apply_particle_coupling failed");
+  dfree(stage[0]);
+  dfree(stage[1]);
+  return rc;
 }
 
 int ntlp_gpu_get_sources(double* src_out) {

@@ -6,18 +6,22 @@
 // device and its own stream:
 //
 //   every iteration
-//     admit   queued requests, FIFO, while a request slot is free and the pool
-//             has room (capacity = HBM budget / bytes per droplet): copy the
-//             request in and append its droplets (k_serve_admit);
-//     control write the per-iteration control block (pool size, which
-//             requests take their last step, their output rows);
-//     step    one captured hipGraph per (pool size class, buffer parity):
-//             counting sort of the pool by u cell, then the fused step of
-//             every droplet; droplets of finishing requests go to the output
-//             rows and drop out of the next sort, so the pool compacts
-//             itself;
-//     retire  copy finished requests' rows out and signal their tickets.
+//     accept  queued requests, FIFO, while a request slot is free, the pool
+//             has room (capacity = HBM budget / bytes per droplet) and the
+//             input ring has space; start their upload on the upload stream;
+//     admit   requests whose upload has landed: append their droplets to the
+//             pool (k_serve_admit);
+//     step    write the control block (pool size, which requests take their
+//             last step, their rows in the output ring), then replay one
+//             captured hipGraph per (pool size class, buffer parity):
+//             counting sort of the pool by u cell and the fused step of every
+//             droplet; droplets of finishing requests go to the output ring
+//             and drop out of the next sort, so the pool compacts itself;
+//     retire  download finished requests on the download stream and signal
+//             their tickets when the copies land.
 //
+// Uploads, steps and downloads run on three streams (all of default
+// priority), so a large request's copies overlap the steps of the others.
 // Requests that arrive while others are in the pool join at the next
 // iteration, and their steps run in the same kernels as everyone else's.  A
 // droplet's arithmetic does not depend on the other droplets in the pool
@@ -46,7 +50,49 @@ constexpr int64_t kServeBytesPerDroplet =
 static_assert(kServeBytesPerDroplet == NTLP_SERVE_BYTES_PER_DROPLET, "pool bytes per droplet");
 constexpr int kServeCtlRing = 4;
 
-enum { kReqQueued = 0, kReqActive, kReqLeaving, kReqDone, kReqFailed };
+enum { kReqQueued = 0, kReqLoading, kReqActive, kReqLeaving, kReqDone, kReqFailed };
+
+// FIFO allocator of droplet rows in a device ring: blocks are released in the
+// order they were taken (copies on one stream complete in order).  A block
+// that does not fit before the end of the ring starts at row 0; the rows it
+// skips are held with it.
+struct Ring {
+  int64_t size = 0, head = 0, tail = 0, used = 0;
+  std::deque<std::pair<int64_t, int64_t>> blocks;  // (first held row, rows held)
+  int64_t alloc(int64_t n) {
+    if (n < 1 || n > size) return -1;
+    if (used == 0) head = tail = 0;
+    if (head >= tail && used < size) {
+      if (size - head >= n) {
+        blocks.push_back({head, n});
+        used += n;
+        head += n;
+        return head - n;
+      }
+      if (tail >= n) {
+        blocks.push_back({head, size - head + n});
+        used += size - head + n;
+        head = n;
+        return 0;
+      }
+      return -1;
+    }
+    if (head < tail && tail - head >= n) {
+      blocks.push_back({head, n});
+      used += n;
+      head += n;
+      return head - n;
+    }
+    return -1;
+  }
+  void release() {
+    const auto b = blocks.front();
+    blocks.pop_front();
+    used -= b.second;
+    tail = (b.first + b.second) % size;
+    if (used == 0) head = tail = 0;
+  }
+};
 
 struct SReq {
   int64_t ticket = 0;
@@ -55,16 +101,21 @@ struct SReq {
   const int64_t* in_mult = nullptr;
   double* out = nullptr;
   int64_t* out_mult = nullptr;
+  int64_t ld_in = 0, ld_out = 0;
   Clock::time_point t_submit, t_admit, t_done;
   int slot = -1;
   int steps_done = 0;
   int state = kReqQueued;
+  int64_t in_row = -1, out_row = -1;  // rows in the input / output rings
+  hipEvent_t ev_in = nullptr;         // upload landed
 };
 
 struct Serve {
   bool open = false;
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // steps
+  hipStream_t up = nullptr;      // uploads
+  hipStream_t down = nullptr;    // downloads
   int steps = 10, prec = PREC_MIXED;
   bool use_graph = true;
   int64_t cap = 0;
@@ -91,12 +142,16 @@ struct Serve {
   int64_t hw = 0;
   std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
   std::vector<SReq*> active;
+  std::deque<SReq*> loading;  // accepted, upload in flight
   bool slot_used[kServeSlots] = {};
+  int64_t reserved = 0;       // pool rows promised to loading requests
+  Ring in_ring, out_ring;
   struct Pending {
     hipEvent_t ev;
     std::vector<SReq*> reqs;
   };
-  std::deque<Pending> pending;
+  std::deque<Pending> pending;   // downloads in flight
+  std::deque<hipEvent_t> admitted;  // admissions that still read the input ring
   // shared with the client threads
   std::thread th;
   std::mutex mu;
@@ -152,8 +207,14 @@ void serve_release() {
     if (srv.ctl_ev[i]) (void)hipEventDestroy(srv.ctl_ev[i]);
   }
   for (auto& p : srv.pending) (void)hipEventDestroy(p.ev);
-  for (auto& kv : srv.reqs) delete kv.second;
+  for (auto& e : srv.admitted) (void)hipEventDestroy(e);
+  for (auto& kv : srv.reqs) {
+    if (kv.second->ev_in) (void)hipEventDestroy(kv.second->ev_in);
+    delete kv.second;
+  }
   if (srv.stream) (void)hipStreamDestroy(srv.stream);
+  if (srv.up) (void)hipStreamDestroy(srv.up);
+  if (srv.down) (void)hipStreamDestroy(srv.down);
   srv.~Serve();
   new (&srv) Serve();
 }
@@ -235,18 +296,19 @@ int serve_run_step(int64_t slots) {
   return 0;
 }
 
-// Marks requests whose output copies have landed as done.  wait_all: block
-// on every pending copy; otherwise only poll (and bound the run-ahead).
-int serve_retire(bool wait_all) {
+// Completed downloads: their requests are done and their output rows free.
+// block: wait for the oldest download if none has landed.
+int serve_retire(bool block) {
   while (!srv.pending.empty()) {
     auto& p = srv.pending.front();
-    hipError_t q = hipEventQuery(p.ev);
+    const hipError_t q = hipEventQuery(p.ev);
     if (q == hipErrorNotReady) {
-      if (!wait_all && srv.pending.size() <= 2) break;
+      if (!block && srv.pending.size() <= 8) break;
       SHIP(hipEventSynchronize(p.ev));
     } else if (q != hipSuccess) {
-      return serve_fail(NTLP_E_HIP, std::string("serving step failed: ") + hipGetErrorString(q));
+      return serve_fail(NTLP_E_HIP, std::string("serving download failed: ") + hipGetErrorString(q));
     }
+    block = false;
     const auto now = Clock::now();
     {
       std::lock_guard<std::mutex> lk(srv.mu);
@@ -256,26 +318,56 @@ int serve_retire(bool wait_all) {
         srv.completed++;
       }
     }
+    for (size_t i = 0; i < p.reqs.size(); ++i) srv.out_ring.release();
     (void)hipEventDestroy(p.ev);
     srv.pending.pop_front();
     srv.cv_done.notify_all();
   }
+  // input rows whose admission kernels have run
+  while (!srv.admitted.empty()) {
+    const hipError_t q = hipEventQuery(srv.admitted.front());
+    if (q == hipErrorNotReady) break;
+    if (q != hipSuccess) return serve_fail(NTLP_E_HIP, std::string("serving admission failed: ") + hipGetErrorString(q));
+    (void)hipEventDestroy(srv.admitted.front());
+    srv.admitted.pop_front();
+    srv.in_ring.release();
+  }
   return 0;
 }
 
-int serve_iteration(std::vector<SReq*>& admit) {
-  // admissions: copy in and append at slots hw..
-  int64_t in_off = 0;
-  for (SReq* r : admit) {
-    SHIP(hipMemcpyAsync(srv.in + 6 * in_off, r->in, sizeof(double) * 6 * r->n, hipMemcpyHostToDevice, srv.stream));
-    SHIP(hipMemcpyAsync(srv.in_mult + in_off, r->in_mult, sizeof(int64_t) * r->n, hipMemcpyHostToDevice, srv.stream));
+// Starts the uploads of newly accepted requests.
+int serve_upload(std::vector<SReq*>& accepted) {
+  for (SReq* r : accepted) {
+    SHIP(hipMemcpy2DAsync(srv.in + 6 * r->in_row, sizeof(double) * r->n, r->in, sizeof(double) * r->ld_in,
+                          sizeof(double) * r->n, 6, hipMemcpyHostToDevice, srv.up));
+    SHIP(hipMemcpyAsync(srv.in_mult + r->in_row, r->in_mult, sizeof(int64_t) * r->n, hipMemcpyHostToDevice, srv.up));
+    SHIP(hipEventCreateWithFlags(&r->ev_in, hipEventDisableTiming));
+    SHIP(hipEventRecord(r->ev_in, srv.up));
+    srv.loading.push_back(r);
+  }
+  return 0;
+}
+
+// Appends the requests whose upload has landed (waits for the first one when
+// the pool would otherwise have nothing to step).
+int serve_admit() {
+  while (!srv.loading.empty()) {
+    SReq* r = srv.loading.front();
+    const hipError_t q = hipEventQuery(r->ev_in);
+    if (q == hipErrorNotReady) {
+      if (!srv.active.empty()) break;
+      SHIP(hipEventSynchronize(r->ev_in));
+    } else if (q != hipSuccess) {
+      return serve_fail(NTLP_E_HIP, std::string("serving upload failed: ") + hipGetErrorString(q));
+    }
+    SHIP(hipStreamWaitEvent(srv.stream, r->ev_in, 0));
     AdmitArgs a;
     a.g = ctx.g;
     a.C = ctx.C;
     a.fl = ctx.fl;
     a.F = fields_view();
-    a.in = srv.in + 6 * in_off;
-    a.in_mult = srv.in_mult + in_off;
+    a.in = srv.in + 6 * r->in_row;
+    a.in_mult = srv.in_mult + r->in_row;
     a.m = r->n;
     a.cap = srv.cap;
     a.base = srv.hw;
@@ -288,27 +380,43 @@ int serve_iteration(std::vector<SReq*>& admit) {
     if (ctx.fl.ilin == 1) k_serve_admit<1><<<grid8(r->n), kBlock, 0, srv.stream>>>(a);
     else k_serve_admit<0><<<grid8(r->n), kBlock, 0, srv.stream>>>(a);
     SHIP(hipGetLastError());
+    hipEvent_t ev;
+    SHIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    SHIP(hipEventRecord(ev, srv.stream));
+    srv.admitted.push_back(ev);
+    (void)hipEventDestroy(r->ev_in);
+    r->ev_in = nullptr;
+    srv.loading.pop_front();
     srv.hw += r->n;
-    in_off += r->n;
+    srv.reserved -= r->n;
     r->t_admit = Clock::now();
     r->state = kReqActive;
     srv.active.push_back(r);
   }
-  // control block
+  return 0;
+}
+
+// One step of the pool and the downloads of the requests it finishes.
+int serve_step() {
   const int ci = srv.ctl_i;
   srv.ctl_i = (srv.ctl_i + 1) % kServeCtlRing;
   SHIP(hipEventSynchronize(srv.ctl_ev[ci]));
   ServeCtl* c = srv.ctl_host[ci];
   memset(c, 0, sizeof(ServeCtl));
   c->hw = srv.hw;
-  int64_t n_step = 0, off = 0;
+  int64_t n_step = 0;
   std::vector<SReq*> leaving;
   for (SReq* r : srv.active) {
     n_step += r->n;
     if (r->steps_done + 1 == srv.steps) {
+      int64_t row;
+      while ((row = srv.out_ring.alloc(r->n)) < 0) {
+        if (srv.pending.empty()) return serve_fail(NTLP_E_RANGE, "output ring exhausted");
+        CHK(serve_retire(true));
+      }
+      r->out_row = row;
       c->last[r->slot] = 1;
-      c->out_off[r->slot] = off;
-      off += r->n;
+      c->out_off[r->slot] = row;
       leaving.push_back(r);
     }
   }
@@ -316,19 +424,22 @@ int serve_iteration(std::vector<SReq*>& admit) {
   SHIP(hipMemcpyAsync(srv.ctl_dev, c, sizeof(ServeCtl), hipMemcpyHostToDevice, srv.stream));
   SHIP(hipEventRecord(srv.ctl_ev[ci], srv.stream));
   CHK(serve_run_step(srv.hw));
-  // copy out the finished requests (rows are [kNHot][cap])
-  for (SReq* r : leaving) {
-    const int64_t o = c->out_off[r->slot];
-    SHIP(hipMemcpy2DAsync(r->out, sizeof(double) * r->n, srv.out + o, sizeof(double) * srv.cap, sizeof(double) * r->n,
-                          kNHot, hipMemcpyDeviceToHost, srv.stream));
-    SHIP(hipMemcpyAsync(r->out_mult, srv.out_mult + o, sizeof(int64_t) * r->n, hipMemcpyDeviceToHost, srv.stream));
-    r->state = kReqLeaving;
-  }
   if (!leaving.empty()) {
-    hipEvent_t ev;
-    SHIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    SHIP(hipEventRecord(ev, srv.stream));
-    srv.pending.push_back({ev, leaving});
+    hipEvent_t stepped, landed;
+    SHIP(hipEventCreateWithFlags(&stepped, hipEventDisableTiming));
+    SHIP(hipEventRecord(stepped, srv.stream));
+    SHIP(hipStreamWaitEvent(srv.down, stepped, 0));
+    (void)hipEventDestroy(stepped);
+    for (SReq* r : leaving) {
+      SHIP(hipMemcpy2DAsync(r->out, sizeof(double) * r->ld_out, srv.out + r->out_row, sizeof(double) * srv.cap,
+                            sizeof(double) * r->n, kNHot, hipMemcpyDeviceToHost, srv.down));
+      SHIP(hipMemcpyAsync(r->out_mult, srv.out_mult + r->out_row, sizeof(int64_t) * r->n, hipMemcpyDeviceToHost,
+                          srv.down));
+      r->state = kReqLeaving;
+    }
+    SHIP(hipEventCreateWithFlags(&landed, hipEventDisableTiming));
+    SHIP(hipEventRecord(landed, srv.down));
+    srv.pending.push_back({landed, leaving});
   }
   for (SReq* r : srv.active) r->steps_done++;
   std::vector<SReq*> stay;
@@ -347,7 +458,7 @@ int serve_iteration(std::vector<SReq*>& admit) {
   srv.cur = 1 - srv.cur;
   srv.iterations++;
   srv.droplet_steps += n_step;
-  return serve_retire(false);
+  return 0;
 }
 
 void serve_loop() {
@@ -356,34 +467,46 @@ void serve_loop() {
     return;
   }
   for (;;) {
-    std::vector<SReq*> admit;
+    std::vector<SReq*> accepted;
     {
       std::unique_lock<std::mutex> lk(srv.mu);
       srv.cv_work.wait(lk, [] {
-        return srv.stop || srv.err || !srv.queue.empty() || !srv.active.empty() || !srv.pending.empty();
+        return srv.stop || srv.err || !srv.queue.empty() || !srv.active.empty() || !srv.loading.empty() ||
+               !srv.pending.empty() || !srv.admitted.empty();
       });
       if (srv.err) break;
-      if (srv.stop && srv.queue.empty() && srv.active.empty() && srv.pending.empty()) break;
-      int64_t room = srv.cap - srv.hw;
-      while (!srv.queue.empty()) {
+      if (srv.stop && srv.queue.empty() && srv.active.empty() && srv.loading.empty() && srv.pending.empty() &&
+          srv.admitted.empty())
+        break;
+      while (!srv.queue.empty()) {  // FIFO: the head waits for a slot, pool room and input rows
         SReq* r = srv.queue.front();
         int slot = -1;
-        for (int s = 0; s < kServeSlots; ++s)
-          if (!srv.slot_used[s]) {
-            slot = s;
+        for (int k = 0; k < kServeSlots; ++k)
+          if (!srv.slot_used[k]) {
+            slot = k;
             break;
           }
-        if (slot < 0 || r->n > room) break;  // FIFO: the head waits for room
+        if (slot < 0 || r->n > srv.cap - srv.hw - srv.reserved) break;
+        const int64_t row = srv.in_ring.alloc(r->n);
+        if (row < 0) break;
         srv.queue.pop_front();
         srv.slot_used[slot] = true;
+        srv.reserved += r->n;
         r->slot = slot;
-        room -= r->n;
-        admit.push_back(r);
+        r->in_row = row;
+        r->state = kReqLoading;
+        accepted.push_back(r);
       }
     }
-    int rc = 0;
-    if (admit.empty() && srv.active.empty()) rc = serve_retire(true);
-    else rc = serve_iteration(admit);
+    int rc = serve_upload(accepted);
+    if (!rc) rc = serve_admit();
+    if (!rc) {
+      if (!srv.active.empty()) rc = serve_step();
+      if (!rc) rc = serve_retire(srv.active.empty() && srv.loading.empty() && !srv.pending.empty());
+      if (!rc && srv.active.empty() && srv.loading.empty() && srv.pending.empty() && !srv.admitted.empty())
+        rc = (hipEventSynchronize(srv.admitted.front()) == hipSuccess) ? serve_retire(false)
+                                                                     : serve_fail(NTLP_E_HIP, "admission failed");
+    }
     if (rc) break;
   }
   // on an error every request still in flight fails
@@ -417,6 +540,10 @@ int ntlp_serve_open(const ntlp_serve_config* cfg, int64_t* capacity) {
   int rc = 0;
   auto alloc_all = [&]() -> int {
     HIPCHK(hipStreamCreateWithFlags(&srv.stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&srv.up, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&srv.down, hipStreamNonBlocking));
+    srv.in_ring.size = cap;
+    srv.out_ring.size = cap;
     for (int b = 0; b < 2; ++b) {
       CHK(dalloc(srv.hot[b], kNHot * cap, "serving pool"));
       CHK(dalloc(srv.mult[b], cap, "serving pool"));
@@ -462,11 +589,14 @@ int ntlp_serve_open(const ntlp_serve_config* cfg, int64_t* capacity) {
   return NTLP_OK;
 }
 
-int ntlp_serve_submit(const double* in, const int64_t* in_mult, int64_t n, double* out, int64_t* out_mult,
-                      int64_t* ticket) {
+int ntlp_serve_submit(const double* in, int64_t ld_in, const int64_t* in_mult, int64_t n, double* out,
+                      int64_t ld_out, int64_t* out_mult, int64_t* ticket) {
   if (!srv.open) return fail(NTLP_E_STATE, "the serving loop is not open");
   if (!in || !in_mult || !out || !out_mult || !ticket) return fail(NTLP_E_INVALID, "null argument");
   if (n < 1) return fail(NTLP_E_INVALID, "a request needs at least one droplet");
+  if (ld_in == 0) ld_in = n;
+  if (ld_out == 0) ld_out = n;
+  if (ld_in < n || ld_out < n) return fail(NTLP_E_INVALID, "column stride smaller than the request");
   if (n > srv.cap || n > kServeIdxMask + 1)
     return fail(NTLP_E_RANGE, "request larger than the pool (" + std::to_string(srv.cap) + " droplets) or 2^23");
   SReq* r = new SReq();
@@ -475,6 +605,8 @@ int ntlp_serve_submit(const double* in, const int64_t* in_mult, int64_t n, doubl
   r->in_mult = in_mult;
   r->out = out;
   r->out_mult = out_mult;
+  r->ld_in = ld_in;
+  r->ld_out = ld_out;
   r->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(srv.mu);
@@ -569,9 +701,23 @@ int ntlp_serve_close(ntlp_serve_totals* final_stats) {
     }
   }
   if (srv.stream) (void)hipStreamSynchronize(srv.stream);
+  if (srv.up) (void)hipStreamSynchronize(srv.up);
+  if (srv.down) (void)hipStreamSynchronize(srv.down);
   serve_release();
   ctx.serving = false;
   if (err) return fail(err, msg);
+  return NTLP_OK;
+}
+
+int ntlp_host_alloc(int64_t bytes, void** ptr) {
+  if (!ptr || bytes < 0) return fail(NTLP_E_INVALID, "bad argument");
+  *ptr = nullptr;
+  HIPCHK(hipHostMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1), hipHostMallocDefault));
+  return NTLP_OK;
+}
+
+int ntlp_host_free(void* ptr) {
+  if (ptr) HIPCHK(hipHostFree(ptr));
   return NTLP_OK;
 }
 

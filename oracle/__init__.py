@@ -78,6 +78,8 @@ def lib() -> ctypes.CDLL:
             ("or_rad_solver2", dp, [vp, vp, dp, vp]),
             ("or_gauss_newton_2d", None, [vp, vp, vp, dp, dp, vp, vp, vp, vp]),
             ("or_lv_solver", None, [vp, vp, vp, dp, dp, vp, vp, vp, vp]),
+            ("or_coupling_exchange", None, [vp, vp, vp]),
+            ("or_apply_coupling", None, [vp, vp, vp, vp, vp, vp]),
         ]:
             fn = getattr(_lib, name)
             fn.restype = res
@@ -162,6 +164,23 @@ class Oracle:
         self.L.or_coupling(ctypes.byref(self.g), ctypes.byref(self.c), ctypes.byref(self.f), _p(soa), _p(mult),
                            soa.shape[1], _p(out))
         return out
+
+    def coupling_exchange(self, src: np.ndarray) -> np.ndarray:
+        """Folds src (6, nnx+1, nny+1, nnz+2) in place and returns the x-pencil
+        arrays (6, nnz+2, nny, nnx)."""
+        g = self.grid
+        assert src.shape == (6, g.nnx + 1, g.nny + 1, g.nnz + 2) and src.flags.c_contiguous
+        out = np.empty((6, g.nnz + 2, g.nny, g.nnx))
+        self.L.or_coupling_exchange(ctypes.byref(self.g), _p(src), _p(out))
+        return out
+
+    def apply_coupling(self, xsrc: np.ndarray, T: np.ndarray, q: np.ndarray) -> None:
+        """T, q: x-pencil (nnz+2, nny, nnx+2) arrays, updated in place."""
+        g = self.grid
+        for a in (T, q):
+            assert a.shape == (g.nnz + 2, g.nny, g.nnx + 2) and a.flags.c_contiguous and a.dtype == np.float64
+        self.L.or_apply_coupling(ctypes.byref(self.g), ctypes.byref(self.c), ctypes.byref(self.f), _p(xsrc),
+                                 _p(T), _p(q))
 
     # binning -------------------------------------------------------------
     def cell_index(self, x, y, z) -> np.ndarray:

@@ -797,3 +797,51 @@ void or_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const 
         }
   }
 }
+
+/* ------------------------------------------------ coupling exchange/apply */
+
+/* particle_coupling_exchange (1066-1149) with tproc = bproc = rproc = lproc =
+ * this rank: the sendrecv pairs become in-place adds, in the same order. */
+void or_coupling_exchange(const or_grid *g, double *src, double *xsrc) {
+  const int nx = g->nnx, ny = g->nny, nz2 = g->nnz + 2;
+  const int64_t nf = (int64_t)nz2 * (ny + 1) * (nx + 1);
+  for (int f = 0; f < 6; ++f) {
+    double *s = src + f * nf;
+    /* top send (1084-1097): plane mxe+1 onto plane mxs, rows iys..iye+1 */
+    for (int iy = 1; iy <= ny + 1; ++iy)
+      for (int k = 0; k < nz2; ++k) s[src_off(g, k, iy, 1)] = s[src_off(g, k, iy, 1)] + s[src_off(g, k, iy, nx + 1)];
+    /* right send (1099-1113): row iye+1 onto row iys, columns mxs..mxe */
+    for (int ix = 1; ix <= nx; ++ix)
+      for (int k = 0; k < nz2; ++k) s[src_off(g, k, 1, ix)] = s[src_off(g, k, 1, ix)] + s[src_off(g, k, ny + 1, ix)];
+    /* ztox_trans (1117-1146) */
+    double *o = xsrc + (int64_t)f * nz2 * ny * nx;
+    for (int k = 0; k < nz2; ++k)
+      for (int iy = 1; iy <= ny; ++iy)
+        for (int ix = 1; ix <= nx; ++ix) o[(ix - 1) + (int64_t)nx * ((iy - 1) + (int64_t)ny * k)] = s[src_off(g, k, iy, ix)];
+  }
+}
+
+/* apply_particle_coupling (1278-1313) over iz = izs..ize = 1..nnz */
+void or_apply_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const double *xsrc,
+                       double *T, double *q) {
+  const int nx = g->nnx, ny = g->nny, nz = g->nnz;
+  const int64_t nf = (int64_t)(nz + 2) * ny * nx;
+  const double *sT = xsrc + 3 * nf, *sH = xsrc + 4 * nf, *sTE = xsrc + 5 * nf;
+  for (int iz = 1; iz <= nz; ++iz)
+    for (int iy = 1; iy <= ny; ++iy)
+      for (int ix = 1; ix <= nx; ++ix) {
+        const int64_t xs = (ix - 1) + (int64_t)nx * ((iy - 1) + (int64_t)ny * iz);
+        const int64_t t = (ix - 1) + (int64_t)(nx + 2) * ((iy - 1) + (int64_t)ny * iz);
+        if (f->iTcouple == 1) {
+          if (f->iexner)
+            T[t] = T[t] + c->dt * (sT[xs] / or_exner(c->surf_p, or_func_p_base(c->surf_p, c->tsfcc1, g->zz[iz], c), c));
+          else
+            T[t] = T[t] + c->dt * sT[xs];
+        }
+        if (f->iHcouple == 1) {
+          T[t] = T[t] + c->dt * sTE[xs];
+          q[t] = q[t] + c->dt * sH[xs];
+        }
+      }
+}
+

@@ -112,6 +112,18 @@ void or_update_be(const or_grid *g, const or_consts *c, const or_flags *f, const
 void or_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const double *soa,
                  const int64_t *mult, int64_t n, double *src);
 
+/* particle_coupling_exchange (1066-1149) on one periodic domain: the +1
+ * ghost plane x = nnx+1 is added onto x = 1 (all y incl. nny+1), then the
+ * ghost row y = nny+1 onto y = 1 (x = 1..nnx), then ztox_trans writes the
+ * x-pencil arrays xsrc[6][(nnz+2)][nny][nnx] (partsrc(1:nnx,1:nny,0:nnz+1,1:3),
+ * partTsrc, partHsrc, partTEsrc).  src is modified (folded) in place. */
+void or_coupling_exchange(const or_grid *g, double *src, double *xsrc);
+/* apply_particle_coupling (1278-1313): t(1:nnx,1:nny,1,1:nnz) and
+ * t(...,2,...) += dt * sources, on arrays in x-pencil layout
+ * (1:nnx+2, 1:nny, 0:nnz+1). */
+void or_apply_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const double *xsrc,
+                       double *T, double *q);
+
 /* single-droplet pieces exposed for pinning tests */
 void or_ie_vrt_nd(const or_consts *c, const double *p /* one droplet record, OR_* order */,
                   double rhoa, const double vnext[3], double tempr, double tempt, double h,

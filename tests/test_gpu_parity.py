@@ -262,3 +262,47 @@ def test_counters_count_solves():
     rows = s.step_counters()
     assert rows.shape[0] == 3 and rows[:, 6].sum() == c["solves"]
     s.close()
+
+
+@pytest.mark.parametrize("iexner", [0, 1])
+def test_coupling_exchange_and_apply_match_oracle(iexner):
+    """particle_coupling_exchange + apply_particle_coupling after a BE step:
+    the fold/transpose is bit-exact against the oracle on the device's own
+    sources; the T/q update matches to the last ulps of exner's pow; the
+    resident T and q the next step gathers are the updated ones."""
+    grid, consts, flags, fields, p, ora = small_case(seed=12, flags=Flags(iexner=iexner))
+    s = make_step(grid, consts, flags, fields, p)
+    s.particle_update_BE(it=2)
+    s.particle_coupling_update()
+    src = np.ascontiguousarray(s.sources())
+    partsrc, partT, partH, partTE = s.particle_coupling_exchange()
+    xs_o = ora.coupling_exchange(src.copy())
+    assert np.array_equal(partsrc, xs_o[:3]) and np.array_equal(partT, xs_o[3])
+    assert np.array_equal(partH, xs_o[4]) and np.array_equal(partTE, xs_o[5])
+    # T and q in the flow solver's x-pencil layout (1:nnx+2, 1:nny, 0:nnz+1)
+    u, v, w, T, q = fields
+    def xpencil(a):
+        out = np.zeros((grid.nnz + 2, grid.nny, grid.nnx + 2))
+        out[:, :, :grid.nnx] = np.transpose(a, (2, 1, 0))
+        out[:, :, grid.nnx:] = -1.0
+        return out
+    Tx, qx = xpencil(T), xpencil(q)
+    Tx_o, qx_o = Tx.copy(), qx.copy()
+    s.apply_particle_coupling(Tx, qx, LAYOUT_XPENCIL)
+    ora.apply_coupling(xs_o, Tx_o, qx_o)
+    if iexner:
+        assert np.max(np.abs(Tx - Tx_o)) <= 1e-13 * 300 and np.array_equal(qx, qx_o)
+    else:
+        assert np.array_equal(Tx, Tx_o) and np.array_equal(qx, qx_o)
+    # the resident fields: gather at the droplets and compare with the oracle's
+    T_new = np.ascontiguousarray(np.transpose(Tx_o[:, :, :grid.nnx], (2, 1, 0)))
+    q_new = np.ascontiguousarray(np.transpose(qx_o[:, :, :grid.nnx], (2, 1, 0)))
+    ora.set_fields(u, v, w, T_new, q_new)
+    got = s.get()
+    s.interp()
+    mine = s.get()
+    want = got.copy()
+    ora.interp(want.soa)
+    bad = compare(mine, want, ["Tf", "qinf"], dict(TOL64, Tf=(1e-13, 300.0)))
+    assert not bad, bad
+    s.close()

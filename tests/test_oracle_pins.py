@@ -373,6 +373,50 @@ def test_coupling_conserves_and_is_linear():
         assert np.max(np.abs(triple[f] - 3 * spread[f])) <= tol * np.max(np.abs(3 * spread[f])), f
 
 
+def test_coupling_exchange_folds_ghosts_onto_their_periodic_images():
+    """particle_coupling_exchange on one periodic domain: a deposit on the +1
+    ghost plane/row lands on its periodic image (x = nnx+1 -> 1, y = nny+1 ->
+    1, the corner -> (1, 1)); every other node keeps its value; the column
+    sums over the core equal the sums over all nodes before the fold."""
+    g = small_grid("uniform")
+    o = ora_for(g)
+    shape = (6, g.nnx + 1, g.nny + 1, g.nnz + 2)
+    src = np.random.default_rng(2).standard_normal(shape)
+    xs = o.coupling_exchange(src.copy())
+    for f in range(6):
+        assert np.allclose(xs[f].sum(axis=(1, 2)), src[f].sum(axis=(0, 1)), rtol=1e-12, atol=1e-12)
+    for (ix, iy), (jx, jy) in {(g.nnx + 1, g.nny + 1): (1, 1), (g.nnx + 1, 3): (1, 3), (4, g.nny + 1): (4, 1),
+                               (5, 6): (5, 6), (1, 1): (1, 1)}.items():
+        d = np.zeros(shape)
+        d[2, ix - 1, iy - 1, 7] = 1.5
+        x = o.coupling_exchange(d)
+        assert x[2, 7, jy - 1, jx - 1] == 1.5 and np.count_nonzero(x) == 1
+
+
+def test_apply_coupling_adds_dt_times_the_sources():
+    """apply_particle_coupling: T += dt*(S_T (/ exner) + S_TE), q += dt*S_H at
+    levels 1..nnz of the core columns; level 0, nnz+1 and the two x-pencil
+    padding columns are untouched."""
+    g = small_grid("stretched")
+    c = Consts(dt=2e-3)
+    rng = np.random.default_rng(3)
+    xs = rng.standard_normal((6, g.nnz + 2, g.nny, g.nnx))
+    for iexner in (0, 1):
+        o = ora_for(g, c, Flags(iexner=iexner))
+        T0 = 300 + rng.standard_normal((g.nnz + 2, g.nny, g.nnx + 2))
+        q0 = 0.01 + 1e-3 * rng.standard_normal((g.nnz + 2, g.nny, g.nnx + 2))
+        T, q = T0.copy(), q0.copy()
+        o.apply_coupling(xs, T, q)
+        core = (slice(1, g.nnz + 1), slice(None), slice(0, g.nnx))
+        ex = np.array([o.exner(c.surf_p, o.p_base(c.surf_p, c.tsfcc1, z)) for z in g.zz[1:g.nnz + 1]])
+        sT = xs[3][core] / (ex[:, None, None] if iexner else 1.0)
+        assert np.allclose(T[core] - T0[core], c.dt * (sT + xs[5][core]), rtol=1e-9, atol=1e-12)
+        assert np.allclose(q[core] - q0[core], c.dt * xs[4][core], rtol=1e-9, atol=1e-15)
+        for a, a0 in ((T, T0), (q, q0)):
+            assert np.array_equal(a[0], a0[0]) and np.array_equal(a[-1], a0[-1])
+            assert np.array_equal(a[:, :, g.nnx:], a0[:, :, g.nnx:])
+
+
 # ----------------------------------------------------------------- walls
 
 def test_boundary_conditions_hand_cases():

@@ -147,3 +147,27 @@ def test_admission_respects_the_hbm_budget():
     assert tot["iterations"] >= 3 * STEPS  # at most two requests fit at a time
     want = plain_result(s, *reqs[4], PREC_MIXED)
     assert np.array_equal(outs[4][0], want[0])
+
+
+def test_strided_windows_of_wide_buffers_give_the_same_results():
+    """Requests handed over as column windows of wider arrays (pinned here),
+    results written into windows of a wide result buffer."""
+    from ntlp_amd import PinnedArray
+    grid, s = serve_case("uniform", seed=9)
+    inp, mult = make_request(grid, 400, 9000, 7e-6, 0.08e-6)
+    bank = PinnedArray((6, 12000))
+    bank.array[:, 1500:10500] = inp
+    res = PinnedArray((14, 30000))
+    res.array[:] = 0.0
+    res_mult = np.zeros(30000, dtype=np.int64)
+    loop = ServeLoop(s, hbm_budget_bytes=16 << 20, steps=STEPS, precision=PREC_MIXED)
+    t1 = loop.submit(bank.array[:, 1500:10500], mult, res.array[:, 20000:29000], res_mult[20000:29000])
+    out, om = np.zeros((14, 9000)), np.zeros(9000, dtype=np.int64)
+    t2 = loop.submit(inp, mult, out, om)
+    loop.wait(t1)
+    loop.wait(t2)
+    loop.close()
+    assert np.array_equal(res.array[:, 20000:29000], out) and np.array_equal(res_mult[20000:29000], om)
+    assert not res.array[:, :20000].any() and not res.array[:, 29000:].any()
+    bank.free()
+    res.free()
