@@ -64,10 +64,11 @@ module ntlp_gpu
        ntlp_gpu_particle_update_be_prec, ntlp_gpu_particle_coupling_update, ntlp_gpu_get_sources, &
        ntlp_gpu_particle_coupling_exchange, ntlp_gpu_apply_particle_coupling, &
        ntlp_gpu_bin, ntlp_gpu_wcell_counts, ntlp_gpu_num_ucells, ntlp_gpu_step, ntlp_gpu_counters, &
+       ntlp_gpu_calc_tauc_min, ntlp_gpu_particle_xy_stats, &
        ntlp_gpu_step_counters, ntlp_gpu_sync
   public :: gpu_check, gpu_setup, gpu_set_fields, gpu_put, gpu_get, gpu_interp, gpu_crit_radius, &
        gpu_particle_update_rk3, gpu_particle_update_BE, gpu_particle_coupling_update, gpu_step, gpu_counters, &
-       gpu_particle_coupling_exchange, gpu_apply_particle_coupling
+       gpu_particle_coupling_exchange, gpu_apply_particle_coupling, gpu_calc_tauc_min, gpu_particle_xy_stats
 
   interface
      integer(c_int32_t) function ntlp_gpu_init(grid, consts, flags) bind(c, name="ntlp_gpu_init")
@@ -173,6 +174,14 @@ module ntlp_gpu
        import :: c_int32_t
        integer(c_int32_t), intent(out) :: num_array(*)
      end function ntlp_gpu_wcell_counts
+     integer(c_int32_t) function ntlp_gpu_calc_tauc_min(tauc_min) bind(c, name="ntlp_gpu_calc_tauc_min")
+       import :: c_int32_t, c_double
+       real(c_double), intent(out) :: tauc_min
+     end function ntlp_gpu_calc_tauc_min
+     integer(c_int32_t) function ntlp_gpu_particle_xy_stats(s0, s1, lwp) bind(c, name="ntlp_gpu_particle_xy_stats")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: s0, s1, lwp
+     end function ntlp_gpu_particle_xy_stats
      integer(c_int64_t) function ntlp_gpu_num_ucells() bind(c, name="ntlp_gpu_num_ucells")
        import :: c_int64_t
      end function ntlp_gpu_num_ucells
@@ -315,6 +324,19 @@ contains
     real(c_double), intent(inout) :: T(:, :, :), q(:, :, :)
     call gpu_check(ntlp_gpu_apply_particle_coupling(T, q, NTLP_LAYOUT_XPENCIL), 'apply_particle_coupling')
   end subroutine gpu_apply_particle_coupling
+
+  ! calc_tauc_min (particles.f90:3310-3368) on one domain.
+  subroutine gpu_calc_tauc_min(tauc_min)
+    real(c_double), intent(out) :: tauc_min
+    call gpu_check(ntlp_gpu_calc_tauc_min(tauc_min), 'calc_tauc_min')
+  end subroutine gpu_calc_tauc_min
+
+  ! particle_xy_stats (particles.f90:3370-3770) before its divisions:
+  ! s0(21), statsvec1(0:nnz+1, 24), LWP(1:nnx, 1:nny).
+  subroutine gpu_particle_xy_stats(s0, statsvec1, lwp)
+    real(c_double), intent(out), target :: s0(21), statsvec1(:, :), lwp(:, :)
+    call gpu_check(ntlp_gpu_particle_xy_stats(c_loc(s0), c_loc(statsvec1), c_loc(lwp)), 'particle_xy_stats')
+  end subroutine gpu_particle_xy_stats
 
   ! nsteps fused droplet steps (gather, advance, boundary conditions, coupling, sort).
   subroutine gpu_step(mode, prec, nsteps, it0)

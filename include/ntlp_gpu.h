@@ -197,6 +197,26 @@ int ntlp_gpu_bin(int32_t *cellidx, int32_t *counts, int32_t *offsets, int32_t *p
 int ntlp_gpu_wcell_counts(int32_t *num_array);
 int64_t ntlp_gpu_num_ucells(void);
 
+/* ------------------------------------------------------------- statistics */
+
+/* calc_tauc_min (particles.f90:3310-3368) on one domain: 5 times the smallest
+ * condensation time scale 1/(2 pi (nuf/Sc) 2 r_avg N_c + 1e-10) over the w
+ * cells holding droplets (1e10 when none does). */
+int ntlp_gpu_calc_tauc_min(double *tauc_min);
+/* particle_xy_stats (particles.f90:3370-3770) on one domain, before its
+ * divisions by counts:
+ *  s0[21]: numpart, numdrop, numaerosol; sums of Rep, water mass, droplet
+ *          mass, volume, radius, radius^2; maxima of radius, Tp, qinf,
+ *          vp(1:3); minima of the same (the reference's starting values
+ *          1000/0/-1000 included);
+ *  s1[24*(nnz+2)]: statsvec1(iz, 1..24), quantity-major, iz = 0..nnz+1
+ *          (partcount, vp1..3, vp1..3^2, vp1*vp3, uf1..3, uf1..3^2, Tp, Tp^2,
+ *          Tf, radius, radius^2, qinf, Tp*vp3, mult, mult*water mass, qstar);
+ *  lwp[nnx*nny]: LWP(ipt, jpt), x fastest.
+ * Any output may be NULL.  num100, numimpos, actnum and denum come from
+ * ntlp_gpu_counters. */
+int ntlp_gpu_particle_xy_stats(double *s0, double *s1, double *lwp);
+
 /* -------------------------------------------------------- fused stepping */
 
 /* nsteps whole droplet steps against the resident fields (which stay fixed),

@@ -221,6 +221,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_bin": (ctypes.c_int, [vp, vp, vp, vp]),
         "ntlp_gpu_wcell_counts": (ctypes.c_int, [vp]),
         "ntlp_gpu_num_ucells": (i64, []),
+        "ntlp_gpu_calc_tauc_min": (ctypes.c_int, [ctypes.POINTER(dp)]),
+        "ntlp_gpu_particle_xy_stats": (ctypes.c_int, [vp, vp, vp]),
         "ntlp_gpu_step": (ctypes.c_int, [i32, i32, i32, i32]),
         "ntlp_gpu_counters": (ctypes.c_int, [ctypes.POINTER(_CCounters), i32]),
         "ntlp_gpu_step_counters": (ctypes.c_int, [vp, i32]),
@@ -374,6 +376,21 @@ class DropletStep:
         out = np.empty((g.nnz, g.nny, g.nnx), dtype=np.int32)
         self._check(self.lib.ntlp_gpu_wcell_counts(_ptr(out)))
         return out
+
+    # ----------------------------------------------------------- statistics
+    def calc_tauc_min(self) -> float:
+        """calc_tauc_min (particles.f90:3310-3368)."""
+        v = ctypes.c_double()
+        self._check(self.lib.ntlp_gpu_calc_tauc_min(ctypes.byref(v)))
+        return float(v.value)
+
+    def particle_xy_stats(self):
+        """particle_xy_stats (particles.f90:3370-3770) before its divisions:
+        (s0[21], s1 (24, nnz+2), lwp (nny, nnx))."""
+        g = self.grid
+        s0, s1, lwp = np.empty(21), np.empty((24, g.nnz + 2)), np.empty((g.nny, g.nnx))
+        self._check(self.lib.ntlp_gpu_particle_xy_stats(_ptr(s0), _ptr(s1), _ptr(lwp)))
+        return s0, s1, lwp
 
     # ------------------------------------------------------------ stepping
     def step(self, mode: int = MODE_BE, precision: int = PREC_FP64, nsteps: int = 1, it0: int = 2) -> None:

@@ -1,9 +1,14 @@
 // kernels.hpp -- HIP kernels of the droplet step (gfx950, wave64).
 //
 // Device layout of the droplets (capacity `cap`, slot t):
-//   hot[c*cap + t]   14 FP64 columns the step reads and writes every step
+//   hot[hix(c, t)]   14 FP64 columns the step reads and writes every step
 //                    (kHot* below), kept in cell-sorted slot order and
-//                    ping-ponged between two buffers by the fused step;
+//                    ping-ponged between two buffers by the fused step.  The
+//                    columns are tiled by wave: the 14 columns of 64
+//                    consecutive slots form one 7 KB block, so that a wave's
+//                    column reads and writes stay within a page or two
+//                    instead of touching 14 pages (28 across the ping-pong)
+//                    whose translations would thrash the L1 TLB;
 //   mult[t], home[t] multiplicity and the droplet's put-order index;
 //   cold[c*cap + h]  the remaining FP64 record columns, in put order h, only
 //                    touched through home[t] (diagnostics, RK3 stage
@@ -25,6 +30,17 @@ enum {
 
 // Tally slots in the per-step counter rows.
 enum { kTNum100 = 0, kTNumImpos, kTActnum, kTDenum, kTGnIters, kTLmIters, kTSolves, kTSteps, kTHist0, kTallyWidth = kTHist0 + 64 };
+
+// Wave tiles of the droplet columns (see above): element (c, t) of a tiled
+// array with `cols` columns.
+constexpr int kTileRows = 64;
+__host__ __device__ __forceinline__ int64_t tix(int cols, int c, int64_t t) {
+  return ((t >> 6) * cols + c) * kTileRows + (t & (kTileRows - 1));
+}
+__host__ __device__ __forceinline__ int64_t hix(int c, int64_t t) { return tix(kNHot, c, t); }
+__host__ __device__ __forceinline__ int64_t aix(int f, int64_t t) { return tix(6, f, t); }
+// elements of a tiled array of `cols` columns holding `cap` slots
+inline int64_t tiled_elems(int64_t cap, int cols) { return (cap + kTileRows - 1) / kTileRows * kTileRows * cols; }
 
 struct DropletBufs {
   double* hot;
@@ -142,55 +158,62 @@ __global__ void k_tally_close(unsigned long long* slots, unsigned long long* row
 
 // ------------------------------------------------------------ load/store
 
-__device__ __forceinline__ Drop load_drop(const double* hot, int64_t cap, int64_t s) {
+__device__ __forceinline__ Drop load_drop(const double* hot, int64_t s) {
   Drop d;
-  d.x = hot[kHX * cap + s];
-  d.y = hot[kHY * cap + s];
-  d.z = hot[kHZ * cap + s];
-  d.vx = hot[kHVX * cap + s];
-  d.vy = hot[kHVY * cap + s];
-  d.vz = hot[kHVZ * cap + s];
-  d.Tp = hot[kHTP * cap + s];
-  d.r = hot[kHR * cap + s];
-  d.m_s = hot[kHMS * cap + s];
-  d.kappa = hot[kHKAPPA * cap + s];
-  d.rc = hot[kHRC * cap + s];
-  d.res = hot[kHRES * cap + s];
-  d.actres = hot[kHACTRES * cap + s];
-  d.numact = hot[kHNUMACT * cap + s];
+  d.x = hot[hix(kHX, s)];
+  d.y = hot[hix(kHY, s)];
+  d.z = hot[hix(kHZ, s)];
+  d.vx = hot[hix(kHVX, s)];
+  d.vy = hot[hix(kHVY, s)];
+  d.vz = hot[hix(kHVZ, s)];
+  d.Tp = hot[hix(kHTP, s)];
+  d.r = hot[hix(kHR, s)];
+  d.m_s = hot[hix(kHMS, s)];
+  d.kappa = hot[hix(kHKAPPA, s)];
+  d.rc = hot[hix(kHRC, s)];
+  d.res = hot[hix(kHRES, s)];
+  d.actres = hot[hix(kHACTRES, s)];
+  d.numact = hot[hix(kHNUMACT, s)];
   return d;
 }
 
-__device__ __forceinline__ void store_drop(double* hot, int64_t cap, int64_t s, const Drop& d) {
-  hot[kHX * cap + s] = d.x;
-  hot[kHY * cap + s] = d.y;
-  hot[kHZ * cap + s] = d.z;
-  hot[kHVX * cap + s] = d.vx;
-  hot[kHVY * cap + s] = d.vy;
-  hot[kHVZ * cap + s] = d.vz;
-  hot[kHTP * cap + s] = d.Tp;
-  hot[kHR * cap + s] = d.r;
-  hot[kHMS * cap + s] = d.m_s;
-  hot[kHKAPPA * cap + s] = d.kappa;
-  hot[kHRC * cap + s] = d.rc;
-  hot[kHRES * cap + s] = d.res;
-  hot[kHACTRES * cap + s] = d.actres;
-  hot[kHNUMACT * cap + s] = d.numact;
+__device__ __forceinline__ void store_drop(double* hot, int64_t s, const Drop& d) {
+  hot[hix(kHX, s)] = d.x;
+  hot[hix(kHY, s)] = d.y;
+  hot[hix(kHZ, s)] = d.z;
+  hot[hix(kHVX, s)] = d.vx;
+  hot[hix(kHVY, s)] = d.vy;
+  hot[hix(kHVZ, s)] = d.vz;
+  hot[hix(kHTP, s)] = d.Tp;
+  hot[hix(kHR, s)] = d.r;
+  hot[hix(kHMS, s)] = d.m_s;
+  hot[hix(kHKAPPA, s)] = d.kappa;
+  hot[hix(kHRC, s)] = d.rc;
+  hot[hix(kHRES, s)] = d.res;
+  hot[hix(kHACTRES, s)] = d.actres;
+  hot[hix(kHNUMACT, s)] = d.numact;
+}
+
+// The hot columns as plain column blocks out[c*ld + row] (serving results).
+__device__ __forceinline__ void store_drop_rows(double* out, int64_t ld, int64_t row, const Drop& d) {
+  const double v[kNHot] = {d.x, d.y, d.z, d.vx, d.vy, d.vz, d.Tp, d.r, d.m_s, d.kappa, d.rc, d.res, d.actres, d.numact};
+#pragma unroll
+  for (int c = 0; c < kNHot; ++c) out[c * ld + row] = v[c];
 }
 
 // State that moves with a droplet in the fused step but is constant in it.
-__device__ __forceinline__ void store_dynamic(double* hot, int64_t cap, int64_t s, const Drop& d) {
-  hot[kHX * cap + s] = d.x;
-  hot[kHY * cap + s] = d.y;
-  hot[kHZ * cap + s] = d.z;
-  hot[kHVX * cap + s] = d.vx;
-  hot[kHVY * cap + s] = d.vy;
-  hot[kHVZ * cap + s] = d.vz;
-  hot[kHTP * cap + s] = d.Tp;
-  hot[kHR * cap + s] = d.r;
-  hot[kHRES * cap + s] = d.res;
-  hot[kHACTRES * cap + s] = d.actres;
-  hot[kHNUMACT * cap + s] = d.numact;
+__device__ __forceinline__ void store_dynamic(double* hot, int64_t s, const Drop& d) {
+  hot[hix(kHX, s)] = d.x;
+  hot[hix(kHY, s)] = d.y;
+  hot[hix(kHZ, s)] = d.z;
+  hot[hix(kHVX, s)] = d.vx;
+  hot[hix(kHVY, s)] = d.vy;
+  hot[hix(kHVZ, s)] = d.vz;
+  hot[hix(kHTP, s)] = d.Tp;
+  hot[hix(kHR, s)] = d.r;
+  hot[hix(kHRES, s)] = d.res;
+  hot[hix(kHACTRES, s)] = d.actres;
+  hot[hix(kHNUMACT, s)] = d.numact;
 }
 
 __device__ __forceinline__ void load_rhs(const double* cold, int64_t cap, int64_t h, Diag& o) {
@@ -276,7 +299,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   int key = -1;
   if (valid) {
     const int64_t s = a.perm ? (int64_t)a.perm[t] : t;
-    Drop d = load_drop(a.src.hot, a.cap, s);
+    Drop d = load_drop(a.src.hot, s);
     const int64_t mult = a.src.mult[s];
     const int32_t home = a.src.home[s];
     Diag o;
@@ -293,7 +316,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     }
     if (SERVE && a.serve->last[home >> kServeIdxBits]) {
       const int64_t row = a.serve->out_off[home >> kServeIdxBits] + (home & kServeIdxMask);
-      store_drop(a.out, a.out_cap, row, d);
+      store_drop_rows(a.out, a.out_cap, row, d);
       a.out_mult[row] = mult;
       a.keys[t] = -1;
     } else {
@@ -301,12 +324,12 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
         double am[6];
         coupling_amounts<PREC == PREC_MIXED>(a.C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) a.amounts[k * a.cap + t] = am[k];
+        for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];
       }
-      store_dynamic(a.dst.hot, a.cap, t, d);
-      a.dst.hot[kHMS * a.cap + t] = d.m_s;
-      a.dst.hot[kHKAPPA * a.cap + t] = d.kappa;
-      a.dst.hot[kHRC * a.cap + t] = d.rc;
+      store_dynamic(a.dst.hot, t, d);
+      a.dst.hot[hix(kHMS, t)] = d.m_s;
+      a.dst.hot[hix(kHKAPPA, t)] = d.kappa;
+      a.dst.hot[hix(kHRC, t)] = d.rc;
       a.dst.mult[t] = mult;
       a.dst.home[t] = home;
       const Cell c = cell_near(a.g, s_z, s_zz, d.x, d.y, d.z, o.kpt);
@@ -337,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
   Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
   if (t < a.n) {
-    Drop d = load_drop(a.src.hot, a.cap, t);
+    Drop d = load_drop(a.src.hot, t);
     const int32_t home = a.src.home[t];
     Diag o;
     if (MODE == 1) {
@@ -350,7 +373,7 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
       store_old(a.cold, a.cap, home, o);
     }
     store_gathered(a.cold, a.cap, home, o);
-    store_dynamic(a.src.hot, a.cap, t, d);
+    store_dynamic(a.src.hot, t, d);
   }
   if (MODE == 2) flush_tally(tl, a.tally, s_red);
 }
@@ -363,7 +386,7 @@ __global__ void __launch_bounds__(kBlock) k_interp(StepArgs a, int set_vp, int s
   __syncthreads();
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
   if (t >= a.n) return;
-  const double x = a.src.hot[kHX * a.cap + t], y = a.src.hot[kHY * a.cap + t], z = a.src.hot[kHZ * a.cap + t];
+  const double x = a.src.hot[hix(kHX, t)], y = a.src.hot[hix(kHY, t)], z = a.src.hot[hix(kHZ, t)];
   const Cell c = cell_of(a.g, s_z, s_zz, x, y, z);
   Fluid f = (ILIN == 1) ? gather_lin_f64<true>(a.g, a.F, s_z, s_zz, x, y, z, c)
                         : gather6_f64(a.g, a.F, s_z, s_zz, x, y, z, c);
@@ -374,11 +397,11 @@ __global__ void __launch_bounds__(kBlock) k_interp(StepArgs a, int set_vp, int s
   a.cold[kCTF * a.cap + h] = f.T;
   a.cold[kCQINF * a.cap + h] = f.q;
   if (set_vp) {
-    a.src.hot[kHVX * a.cap + t] = f.u[0];
-    a.src.hot[kHVY * a.cap + t] = f.u[1];
-    a.src.hot[kHVZ * a.cap + t] = f.u[2];
+    a.src.hot[hix(kHVX, t)] = f.u[0];
+    a.src.hot[hix(kHVY, t)] = f.u[1];
+    a.src.hot[hix(kHVZ, t)] = f.u[2];
   }
-  if (set_tp) a.src.hot[kHTP * a.cap + t] = f.T;
+  if (set_tp) a.src.hot[hix(kHTP, t)] = f.T;
 }
 
 // Coupling amounts of the current state from the stored pre-step values
@@ -386,14 +409,14 @@ __global__ void __launch_bounds__(kBlock) k_interp(StepArgs a, int set_vp, int s
 __global__ void __launch_bounds__(kBlock) k_amounts(StepArgs a) {
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
   if (t >= a.n) return;
-  const Drop d = load_drop(a.src.hot, a.cap, t);
+  const Drop d = load_drop(a.src.hot, t);
   const int32_t h = a.src.home[t];
   const double vold[3] = {a.cold[kCVXOLD * a.cap + h], a.cold[kCVYOLD * a.cap + h], a.cold[kCVZOLD * a.cap + h]};
   double am[6];
   coupling_amounts<false>(a.C, a.fl, d, a.cold[kCTF * a.cap + h], vold, a.cold[kCTPOLD * a.cap + h],
                           a.cold[kCRADIUSOLD * a.cap + h], a.src.mult[t], am);
 #pragma unroll
-  for (int k = 0; k < 6; ++k) a.amounts[k * a.cap + t] = am[k];
+  for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];
 }
 
 // -------------------------------------------------------- coupling scatter
@@ -502,7 +525,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
   int* my_tail = s_tail + (threadIdx.x & ~63);
   Deposit dp;
   if (valid) {
-    const double x = a.hot[kHX * a.cap + t], y = a.hot[kHY * a.cap + t], z = a.hot[kHZ * a.cap + t];
+    const double x = a.hot[hix(kHX, t)], y = a.hot[hix(kHY, t)], z = a.hot[hix(kHZ, t)];
     dp = a.keys ? deposit_key(a.g, s_zz, a.keys[t], x, y, z) : deposit_of(a.g, s_z, s_zz, x, y, z);
   } else {
     dp.key = -1 - lane;
@@ -541,7 +564,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
   key_nodes(a.g, act ? skey : 0, node);
   for (int f = 0; f < 6; ++f) {
     if (!((a.fmask >> f) & 1u)) continue;
-    const double am = valid ? a.amounts[f * a.cap + t] : 0.0;
+    const double am = valid ? a.amounts[aix(f, t)] : 0.0;
     double v[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) v[c] = am * dp.w[c];
@@ -619,10 +642,10 @@ __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
         for (int p = b; p < e; ++p) {
           const int64_t s = a.perm[p];
           const double w = corner_weight(a.g, s_zz, cx[xa] + 1, cy[ya] + 1, k0, ci[xa], cj[ya], ck,
-                                         a.hot[kHX * a.cap + s], a.hot[kHY * a.cap + s], a.hot[kHZ * a.cap + s]);
+                                         a.hot[hix(kHX, s)], a.hot[hix(kHY, s)], a.hot[hix(kHZ, s)]);
 #pragma unroll
           for (int f = 0; f < 6; ++f)
-            if ((a.fmask >> f) & 1u) acc[f] += a.amounts[f * a.cap + s] * w;
+            if ((a.fmask >> f) & 1u) acc[f] += a.amounts[aix(f, s)] * w;
         }
       }
 #pragma unroll
@@ -711,7 +734,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
     while (s + 1 < nseg && s_pre[s + 1] <= p) ++s;
     const int64_t slot = a.perm[s_beg[s] + (p - s_pre[s])];
     const int key = a.keys[slot];
-    const double x = a.hot[kHX * a.cap + slot], y = a.hot[kHY * a.cap + slot], z = a.hot[kHZ * a.cap + slot];
+    const double x = a.hot[hix(kHX, slot)], y = a.hot[hix(kHY, slot)], z = a.hot[hix(kHZ, slot)];
     const int kpt = key % nz1;
     const int j0 = (key / nz1) % ny1;
     const int i0 = key / (nz1 * ny1);
@@ -735,7 +758,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
     }
     double am[6];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) am[f] = ((a.fmask >> f) & 1u) ? a.amounts[f * a.cap + slot] : 0.0;
+    for (int f = 0; f < 6; ++f) am[f] = ((a.fmask >> f) & 1u) ? a.amounts[aix(f, slot)] : 0.0;
 #pragma unroll
     for (int ci = 0; ci < 2; ++ci)
 #pragma unroll
@@ -867,7 +890,7 @@ __global__ void __launch_bounds__(kBlock) k_keys(BinArgs a) {
   const bool valid = t < a.n;
   int key = -1;
   if (valid) {
-    const Cell c = cell_of(a.g, s_z, s_zz, a.hot[kHX * a.cap + t], a.hot[kHY * a.cap + t], a.hot[kHZ * a.cap + t]);
+    const Cell c = cell_of(a.g, s_z, s_zz, a.hot[hix(kHX, t)], a.hot[hix(kHY, t)], a.hot[hix(kHZ, t)]);
     key = ukey_of(a.g, c);
     a.keys[t] = key;
     if (a.cellidx) {
@@ -887,11 +910,170 @@ __global__ void __launch_bounds__(kBlock) k_wcells(BinArgs a, int32_t* num) {
   __syncthreads();
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
   if (t >= a.n) return;
-  const double x = a.hot[kHX * a.cap + t], y = a.hot[kHY * a.cap + t], z = a.hot[kHZ * a.cap + t];
+  const double x = a.hot[hix(kHX, t)], y = a.hot[hix(kHY, t)], z = a.hot[hix(kHZ, t)];
   const Cell c = cell_of(a.g, s_z, s_zz, x, y, z);
   const int kc = wcell_of(a.g, s_z, z);
   if (c.ipt < 1 || c.ipt > a.g.nnx || c.jpt < 1 || c.jpt > a.g.nny || kc < 1 || kc > a.g.nnz) return;
   atomicAdd(&num[(c.ipt - 1) + (int64_t)a.g.nnx * ((c.jpt - 1) + (int64_t)a.g.nny * (kc - 1))], 1);
+}
+
+// ------------------------------------------------------------ statistics
+
+// calc_tauc_min (particles.f90:3310-3368), first half: radius sum, count and
+// multiplicity sum per w cell (ipt, jpt, kc); droplets outside (1:nnx,
+// 1:nny, 1:nnz) are not counted.
+__global__ void __launch_bounds__(kBlock) k_tauc_accum(BinArgs a, const int64_t* mult, double* rad, int32_t* num,
+                                                       double* nc) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  stage_grid(a.g, s_z, s_zz);
+  __syncthreads();
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  if (t >= a.n) return;
+  const double x = a.hot[hix(kHX, t)], y = a.hot[hix(kHY, t)], z = a.hot[hix(kHZ, t)];
+  const Cell c = cell_of(a.g, s_z, s_zz, x, y, z);
+  const int kc = wcell_of(a.g, s_z, z);
+  if (c.ipt < 1 || c.ipt > a.g.nnx || c.jpt < 1 || c.jpt > a.g.nny || kc < 1 || kc > a.g.nnz) return;
+  const int64_t e = (c.ipt - 1) + (int64_t)a.g.nnx * ((c.jpt - 1) + (int64_t)a.g.nny * (kc - 1));
+  atomicAdd(&rad[e], a.hot[hix(kHR, t)]);
+  atomicAdd(&num[e], 1);
+  atomicAdd(&nc[e], (double)mult[t]);
+}
+
+// Second half: the smallest 1/(denom + 1e-10) over occupied cells, as the
+// bit pattern of a positive double (ordered like the value) in *out.
+__global__ void __launch_bounds__(kBlock) k_tauc_min(DevGrid g, DevConsts C, const double* rad, const int32_t* num,
+                                                     const double* nc, unsigned long long* out) {
+  __shared__ double s_min[kBlock / 64];
+  const int64_t ncell = (int64_t)g.nnx * g.nny * g.nnz;
+  double m = 1.0e10;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ncell; e += (int64_t)gridDim.x * blockDim.x) {
+    if (num[e] != 0) {
+      const int iz = (int)(e / ((int64_t)g.nnx * g.nny)) + 1;
+      const double radavg = rad[e] / (double)num[e];
+      const double Nc = nc[e] / (g.dx * g.dy * g.dzw[iz]);
+      const double denom = C.pi2 * (C.nuf / C.Sc) * 2.0 * radavg * Nc;
+      m = fmin(m, 1.0 / (denom + 1.0e-10));
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
+  if (lane_id() == 0) s_min[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmin(m, s_min[w]);
+    atomicMin(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// particle_xy_stats (particles.f90:3370-3770): the droplet loops.  Each of
+// gridDim.x workgroups walks its own contiguous run of slots and keeps
+//  - the 0th-order statistics per thread (counts, 6 sums, 6 maxima, 6
+//    minima, starting from the reference's initial values), reduced in LDS;
+//  - quantities q0..q0+nq-1 of the 24 per-level sums of statsvec1 in LDS;
+// and writes them to its row of `part`; LWP(ipt, jpt) is added with global
+// atomics.  k_xy_reduce then sums the rows in a fixed order.
+constexpr int kStats0 = 21;
+struct StatsArgs {
+  DevGrid g;
+  DevConsts C;
+  const double* hot;
+  const int64_t* mult;
+  const int32_t* home;
+  const double* cold;
+  int64_t n, cap;
+  int q0, nq;        // per-level quantities of this pass
+  double* part0;     // [gridDim.x][kStats0] (pass 0 only)
+  double* part1;     // [gridDim.x][nq][nnz+2]
+  double* lwp;       // [nny][nnx] (pass 0 only)
+};
+
+__global__ void __launch_bounds__(kBlock) k_xy_stats(StatsArgs a) {
+  extern __shared__ double s_lvl[];  // [nq][nnz+2]
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  __shared__ double s_red[kBlock / 64][kStats0];
+  stage_grid(a.g, s_z, s_zz);
+  const int nz2 = a.g.nnz + 2;
+  for (int i = threadIdx.x; i < a.nq * nz2; i += blockDim.x) s_lvl[i] = 0.0;
+  __syncthreads();
+  const bool first = a.q0 == 0;
+  double v0[kStats0] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0, -1000.0, -1000.0, -1000.0,
+                        1000.0, 1000.0, 1000.0, 1000.0, 1000.0, 1000.0};
+  const int64_t chunk = (a.n + gridDim.x - 1) / gridDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * chunk, t1 = min(a.n, t0 + chunk);
+  const double pi = 4.0 * atan(1.0);
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+    const Drop d = load_drop(a.hot, t);
+    const int64_t h = a.home[t];
+    const double uf0 = a.cold[kCUFX * a.cap + h], uf1 = a.cold[kCUFY * a.cap + h], uf2 = a.cold[kCUFZ * a.cap + h];
+    const double Tf = a.cold[kCTF * a.cap + h], qinf = a.cold[kCQINF * a.cap + h];
+    const double m = (double)a.mult[t];
+    const double mw = a.C.rhow * 4.0 / 3.0 * pi * pow(d.r, 3);
+    if (first) {
+      v0[0] += 1.0;
+      if (d.r > d.rc) v0[1] += 1.0;
+      else v0[2] += 1.0;
+      const double Volp = a.C.pi2 * 2.0 / 3.0 * pow(d.r, 3);
+      const double rhop = (d.m_s + Volp * a.C.rhow) / Volp;
+      const double d0 = d.vx - uf0, d1 = d.vy - uf1, d2 = d.vz - uf2;
+      v0[3] += 2.0 * d.r * sqrt(d0 * d0 + d1 * d1 + d2 * d2) / a.C.nuf;
+      v0[4] += Volp * a.C.rhow;
+      v0[5] += Volp * rhop;
+      v0[6] += Volp;
+      v0[7] += d.r;
+      v0[8] += d.r * d.r;
+      const double mm[6] = {d.r, d.Tp, qinf, d.vx, d.vy, d.vz};
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        v0[9 + k] = fmax(v0[9 + k], mm[k]);
+        v0[15 + k] = fmin(v0[15 + k], mm[k]);
+      }
+    }
+    const Cell c = cell_of(a.g, s_z, s_zz, d.x, d.y, d.z);
+    const int kc = wcell_of(a.g, s_z, d.z);
+    if (kc >= 0) {
+      const double q[24] = {1.0, d.vx, d.vy, d.vz, d.vx * d.vx, d.vy * d.vy, d.vz * d.vz, d.vx * d.vz,
+                            uf0, uf1, uf2, uf0 * uf0, uf1 * uf1, uf2 * uf2, d.Tp, d.Tp * d.Tp, Tf,
+                            d.r, d.r * d.r, qinf, d.Tp * d.vz, m, m * mw, a.cold[kCQSTAR * a.cap + h]};
+#pragma unroll
+      for (int k = 0; k < 24; ++k)
+        if (k >= a.q0 && k < a.q0 + a.nq) atomicAdd(&s_lvl[(k - a.q0) * nz2 + kc], q[k]);
+    }
+    if (first && c.ipt >= 1 && c.ipt <= a.g.nnx && c.jpt >= 1 && c.jpt <= a.g.nny)
+      atomicAdd(&a.lwp[(c.ipt - 1) + (int64_t)a.g.nnx * (c.jpt - 1)], m * mw);
+  }
+  if (first) {
+#pragma unroll
+    for (int k = 0; k < kStats0; ++k) {
+      double v = v0[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = (k < 9) ? v + w : (k < 15 ? fmax(v, w) : fmin(v, w));
+      }
+      if (lane_id() == 0) s_red[threadIdx.x >> 6][k] = v;
+    }
+  }
+  __syncthreads();
+  if (first && threadIdx.x < kStats0) {
+    const int k = threadIdx.x;
+    double v = s_red[0][k];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      v = (k < 9) ? v + s_red[w][k] : (k < 15 ? fmax(v, s_red[w][k]) : fmin(v, s_red[w][k]));
+    a.part0[(int64_t)blockIdx.x * kStats0 + k] = v;
+  }
+  for (int i = threadIdx.x; i < a.nq * nz2; i += blockDim.x) a.part1[(int64_t)blockIdx.x * a.nq * nz2 + i] = s_lvl[i];
+}
+
+// Sums (maxima, minima) of the workgroup rows in row order.
+__global__ void k_xy_reduce(const double* part, int rows, int width, double* out, int kind_split0, int kind_split1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= width) return;
+  double v = part[i];
+  for (int r = 1; r < rows; ++r) {
+    const double w = part[(int64_t)r * width + i];
+    v = (i < kind_split0) ? v + w : (i < kind_split1 ? fmax(v, w) : fmin(v, w));
+  }
+  out[i] = v;
 }
 
 // Exclusive scan of the u-cell histogram: per-block sums, one-block scan of
@@ -1111,8 +1293,7 @@ __global__ void __launch_bounds__(kBlock) k_crit_radius(StepArgs a) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.n) return;
   double* hot = a.src.hot;
-  hot[kHRC * a.cap + t] =
-      crit_radius_of(a.C, a.fl, hot[kHMS * a.cap + t], hot[kHKAPPA * a.cap + t], hot[kHTP * a.cap + t]);
+  hot[hix(kHRC, t)] = crit_radius_of(a.C, a.fl, hot[hix(kHMS, t)], hot[hix(kHKAPPA, t)], hot[hix(kHTP, t)]);
 }
 
 // Admission of one request into a serving pool: create_particle's state for
@@ -1167,7 +1348,7 @@ __global__ void __launch_bounds__(kBlock) k_serve_admit(AdmitArgs a) {
     d.actres = 0.0;
     d.numact = 0.0;
     const int64_t s = a.base + t;
-    store_drop(a.hot, a.cap, s, d);
+    store_drop(a.hot, s, d);
     a.mult[s] = a.in_mult[t];
     a.home[s] = a.tag | (int32_t)t;
     key = ukey_of(a.g, c);
@@ -1176,11 +1357,17 @@ __global__ void __launch_bounds__(kBlock) k_serve_admit(AdmitArgs a) {
   count_key(a.counts, key, valid);
 }
 
-// slot-order column -> put order via home
-__global__ void k_scatter_col_f64(const double* col, const int32_t* home, int64_t n, double* out) {
+// a plain column into hot column c (slot order)
+__global__ void k_pack_hot(const double* col, int c, int64_t n, double* hot) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) out[home[t]] = col[t];
+  if (t < n) hot[hix(c, t)] = col[t];
 }
+// hot column c -> put order (home == nullptr: slot order is put order)
+__global__ void k_unpack_hot(const double* hot, int c, const int32_t* home, int64_t n, double* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[home ? home[t] : t] = hot[hix(c, t)];
+}
+// slot-order column -> put order via home
 __global__ void k_scatter_col_i64(const int64_t* col, const int32_t* home, int64_t n, int64_t* out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) out[home[t]] = col[t];

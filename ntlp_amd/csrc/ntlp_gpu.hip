@@ -644,14 +644,14 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
   HIPCHK(hipStreamSynchronize(ctx.stream));
   if (n > ctx.cap) {
     for (int b = 0; b < 2; ++b) {
-      CHK(dalloc(ctx.hot[b], kNHot * n, "hot state"));
+      CHK(dalloc(ctx.hot[b], tiled_elems(n, kNHot), "hot state"));
       CHK(dalloc(ctx.mult[b], n, "mult"));
       CHK(dalloc(ctx.home[b], n, "home"));
     }
     CHK(dalloc(ctx.cold, kNCold * n, "cold state"));
     CHK(dalloc(ctx.pidx, n, "pidx"));
     CHK(dalloc(ctx.procidx, n, "procidx"));
-    CHK(dalloc(ctx.amounts, 6 * n, "coupling amounts"));
+    CHK(dalloc(ctx.amounts, tiled_elems(n, 6), "coupling amounts"));
     CHK(dalloc(ctx.keys[0], n, "keys"));
     CHK(dalloc(ctx.keys[1], n, "keys"));
     CHK(dalloc(ctx.perm, n, "perm"));
@@ -665,8 +665,17 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
   ctx.keys_valid = false;
   if (n == 0) return NTLP_OK;
   for (int c = 0; c < NTLP_NREAL; ++c) {
-    double* dst = hot_of(c) >= 0 ? ctx.hot[0] + (int64_t)hot_of(c) * ctx.cap : ctx.cold + (int64_t)cold_of(c) * ctx.cap;
-    HIPCHK(hipMemcpyAsync(dst, soa + (int64_t)c * n, sizeof(double) * n, hipMemcpyHostToDevice, ctx.stream));
+    if (hot_of(c) >= 0) {
+      double* tmp = ctx.col_tmp;
+      double* hot = ctx.hot[0];
+      const int hc = hot_of(c);
+      HIPCHK(hipMemcpyAsync(tmp, soa + (int64_t)c * n, sizeof(double) * n, hipMemcpyHostToDevice, ctx.stream));
+      CHK(launch("put", [&] { k_pack_hot<<<grid_flat(n), kBlock, 0, ctx.stream>>>(tmp, hc, n, hot); }));
+      HIPCHK(hipStreamSynchronize(ctx.stream));  // col_tmp reuse
+    } else {
+      HIPCHK(hipMemcpyAsync(ctx.cold + (int64_t)cold_of(c) * ctx.cap, soa + (int64_t)c * n, sizeof(double) * n,
+                            hipMemcpyHostToDevice, ctx.stream));
+    }
   }
   HIPCHK(hipMemcpyAsync(ctx.mult[0], mult, sizeof(int64_t) * n, hipMemcpyHostToDevice, ctx.stream));
   if (pidx) HIPCHK(hipMemcpyAsync(ctx.pidx, pidx, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx.stream));
@@ -687,13 +696,12 @@ int ntlp_gpu_particles_get(double* soa, int64_t* mult, int32_t* pidx, int32_t* p
   if (soa) {
     for (int c = 0; c < NTLP_NREAL; ++c) {
       if (hot_of(c) >= 0) {
-        const double* col = ctx.hot[ctx.cur] + (int64_t)hot_of(c) * cap;
-        if (!ctx.home_identity) {
-          double* tmp = ctx.col_tmp;
-          CHK(launch("get", [&] { k_scatter_col_f64<<<grid_flat(n), kBlock, 0, ctx.stream>>>(col, home, n, tmp); }));
-          col = tmp;
-        }
-        HIPCHK(hipMemcpyAsync(soa + (int64_t)c * n, col, sizeof(double) * n, hipMemcpyDeviceToHost, ctx.stream));
+        const double* hot = ctx.hot[ctx.cur];
+        const int hc = hot_of(c);
+        double* tmp = ctx.col_tmp;
+        const int32_t* hm = ctx.home_identity ? nullptr : home;
+        CHK(launch("get", [&] { k_unpack_hot<<<grid_flat(n), kBlock, 0, ctx.stream>>>(hot, hc, hm, n, tmp); }));
+        HIPCHK(hipMemcpyAsync(soa + (int64_t)c * n, tmp, sizeof(double) * n, hipMemcpyDeviceToHost, ctx.stream));
         HIPCHK(hipStreamSynchronize(ctx.stream));  // col_tmp reuse
       } else {
         HIPCHK(hipMemcpyAsync(soa + (int64_t)c * n, ctx.cold + (int64_t)cold_of(c) * cap, sizeof(double) * n,
@@ -988,6 +996,100 @@ int ntlp_gpu_wcell_counts(int32_t* num_array) {
     rc = fail(NTLP_E_HIP, "copy failed");
   if (hipStreamSynchronize(ctx.stream) != hipSuccess && rc == 0) rc = fail(NTLP_E_HIP, "sync failed");
   dfree(d);
+  return rc;
+}
+
+int ntlp_gpu_calc_tauc_min(double* tauc_min) {
+  CHK(need_drops());
+  if (!tauc_min) return fail(NTLP_E_INVALID, "null output");
+  const int64_t ncell = (int64_t)ctx.g.nnx * ctx.g.nny * ctx.g.nnz;
+  double *rad = nullptr, *nc = nullptr;
+  int32_t* num = nullptr;
+  unsigned long long* res = nullptr;
+  int rc = dalloc(rad, ncell, "tauc sums");
+  if (!rc) rc = dalloc(nc, ncell, "tauc sums");
+  if (!rc) rc = dalloc(num, ncell, "tauc counts");
+  if (!rc) rc = dalloc(res, 1, "tauc result");
+  const double start = 1.0e10;
+  if (!rc && (hipMemsetAsync(rad, 0, sizeof(double) * ncell, ctx.stream) != hipSuccess ||
+              hipMemsetAsync(nc, 0, sizeof(double) * ncell, ctx.stream) != hipSuccess ||
+              hipMemsetAsync(num, 0, sizeof(int32_t) * ncell, ctx.stream) != hipSuccess ||
+              hipMemcpyAsync(res, &start, sizeof(double), hipMemcpyHostToDevice, ctx.stream) != hipSuccess))
+    rc = fail(NTLP_E_HIP, "tauc_min set-up failed");
+  if (!rc) {
+    BinArgs b;
+    b.g = ctx.g;
+    b.hot = ctx.hot[ctx.cur];
+    b.n = ctx.n;
+    b.cap = ctx.cap;
+    const int64_t* mult = ctx.mult[ctx.cur];
+    rc = launch("tauc", [&] { k_tauc_accum<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(b, mult, rad, num, nc); });
+  }
+  if (!rc)
+    rc = launch("tauc", [&] { k_tauc_min<<<grid_flat(ncell), kBlock, 0, ctx.stream>>>(ctx.g, ctx.C, rad, num, nc, res); });
+  double m = 0.0;
+  if (!rc && (hipMemcpyAsync(&m, res, sizeof(double), hipMemcpyDeviceToHost, ctx.stream) != hipSuccess ||
+              hipStreamSynchronize(ctx.stream) != hipSuccess))
+    rc = fail(NTLP_E_HIP, "tauc_min failed");
+  dfree(rad);
+  dfree(nc);
+  dfree(num);
+  dfree(res);
+  if (rc) return rc;
+  *tauc_min = m * 5.0;
+  return NTLP_OK;
+}
+
+int ntlp_gpu_particle_xy_stats(double* s0, double* s1, double* lwp) {
+  CHK(need_drops());
+  const int nz2 = ctx.g.nnz + 2;
+  const int rows = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (ctx.n + kBlock - 1) / kBlock));
+  const int nq_max = std::max(1, std::min(24, 6144 / nz2));
+  double *part0 = nullptr, *part1 = nullptr, *out0 = nullptr, *out1 = nullptr, *lw = nullptr;
+  int rc = dalloc(part0, (int64_t)rows * kStats0, "statistics rows");
+  if (!rc) rc = dalloc(part1, (int64_t)rows * nq_max * nz2, "statistics rows");
+  if (!rc) rc = dalloc(out0, kStats0, "statistics");
+  if (!rc) rc = dalloc(out1, 24 * nz2, "statistics");
+  if (!rc) rc = dalloc(lw, (int64_t)ctx.g.nnx * ctx.g.nny, "LWP");
+  if (!rc && hipMemsetAsync(lw, 0, sizeof(double) * ctx.g.nnx * ctx.g.nny, ctx.stream) != hipSuccess)
+    rc = fail(NTLP_E_HIP, "statistics set-up failed");
+  for (int q0 = 0; q0 < 24 && !rc; q0 += nq_max) {
+    StatsArgs a;
+    a.g = ctx.g;
+    a.C = ctx.C;
+    a.hot = ctx.hot[ctx.cur];
+    a.mult = ctx.mult[ctx.cur];
+    a.home = ctx.home[ctx.cur];
+    a.cold = ctx.cold;
+    a.n = ctx.n;
+    a.cap = ctx.cap;
+    a.q0 = q0;
+    a.nq = std::min(nq_max, 24 - q0);
+    a.part0 = part0;
+    a.part1 = part1;
+    a.lwp = lw;
+    const size_t lds = sizeof(double) * a.nq * nz2;
+    const int width = a.nq * nz2;
+    double* dst = out1 + (int64_t)q0 * nz2;
+    rc = launch("xy_stats", [&] { k_xy_stats<<<rows, kBlock, lds, ctx.stream>>>(a); });
+    if (!rc)
+      rc = launch("xy_stats", [&] {
+        k_xy_reduce<<<(width + kBlock - 1) / kBlock, kBlock, 0, ctx.stream>>>(part1, rows, width, dst, width, width);
+      });
+    if (!rc && q0 == 0)
+      rc = launch("xy_stats", [&] { k_xy_reduce<<<1, kBlock, 0, ctx.stream>>>(part0, rows, kStats0, out0, 9, 15); });
+  }
+  if (!rc && ((s0 && hipMemcpyAsync(s0, out0, sizeof(double) * kStats0, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess) ||
+              (s1 && hipMemcpyAsync(s1, out1, sizeof(double) * 24 * nz2, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess) ||
+              (lwp && hipMemcpyAsync(lwp, lw, sizeof(double) * ctx.g.nnx * ctx.g.nny, hipMemcpyDeviceToHost,
+                                     ctx.stream) != hipSuccess) ||
+              hipStreamSynchronize(ctx.stream) != hipSuccess))
+    rc = fail(NTLP_E_HIP, "statistics copy failed");
+  dfree(part0);
+  dfree(part1);
+  dfree(out0);
+  dfree(out1);
+  dfree(lw);
   return rc;
 }
 

@@ -545,7 +545,7 @@ int ntlp_serve_open(const ntlp_serve_config* cfg, int64_t* capacity) {
     srv.in_ring.size = cap;
     srv.out_ring.size = cap;
     for (int b = 0; b < 2; ++b) {
-      CHK(dalloc(srv.hot[b], kNHot * cap, "serving pool"));
+      CHK(dalloc(srv.hot[b], tiled_elems(cap, kNHot), "serving pool"));
       CHK(dalloc(srv.mult[b], cap, "serving pool"));
       CHK(dalloc(srv.home[b], cap, "serving pool"));
       CHK(dalloc(srv.keys[b], cap, "serving pool"));

@@ -79,6 +79,8 @@ def lib() -> ctypes.CDLL:
             ("or_gauss_newton_2d", None, [vp, vp, vp, dp, dp, vp, vp, vp, vp]),
             ("or_lv_solver", None, [vp, vp, vp, dp, dp, vp, vp, vp, vp]),
             ("or_coupling_exchange", None, [vp, vp, vp]),
+            ("or_calc_tauc_min", dp, [vp, vp, vp, vp, i64]),
+            ("or_xy_stats", None, [vp, vp, vp, vp, i64, vp, vp, vp]),
             ("or_apply_coupling", None, [vp, vp, vp, vp, vp, vp]),
         ]:
             fn = getattr(_lib, name)
@@ -181,6 +183,19 @@ class Oracle:
             assert a.shape == (g.nnz + 2, g.nny, g.nnx + 2) and a.flags.c_contiguous and a.dtype == np.float64
         self.L.or_apply_coupling(ctypes.byref(self.g), ctypes.byref(self.c), ctypes.byref(self.f), _p(xsrc),
                                  _p(T), _p(q))
+
+    def calc_tauc_min(self, soa: np.ndarray, mult: np.ndarray) -> float:
+        mult = np.ascontiguousarray(mult, dtype=np.int64)
+        return self.L.or_calc_tauc_min(ctypes.byref(self.g), ctypes.byref(self.c), _p(soa), _p(mult), soa.shape[1])
+
+    def xy_stats(self, soa: np.ndarray, mult: np.ndarray):
+        """(s0[21], s1 (24, nnz+2), lwp (nny, nnx)) of particle_xy_stats."""
+        g = self.grid
+        mult = np.ascontiguousarray(mult, dtype=np.int64)
+        s0, s1, lwp = np.empty(21), np.empty((24, g.nnz + 2)), np.empty((g.nny, g.nnx))
+        self.L.or_xy_stats(ctypes.byref(self.g), ctypes.byref(self.c), _p(soa), _p(mult), soa.shape[1], _p(s0),
+                           _p(s1), _p(lwp))
+        return s0, s1, lwp
 
     # binning -------------------------------------------------------------
     def cell_index(self, x, y, z) -> np.ndarray:

@@ -845,3 +845,105 @@ void or_apply_coupling(const or_grid *g, const or_consts *c, const or_flags *f, 
       }
 }
 
+/* ------------------------------------------------------------ statistics */
+
+/* calc_tauc_min (3310-3368).  A droplet whose cell lies outside
+ * (1:nnx, 1:nny, 1:nnz) is not counted (the reference would index outside
+ * its arrays). */
+double or_calc_tauc_min(const or_grid *g, const or_consts *c, const double *soa, const int64_t *mult, int64_t n) {
+  const int nx = g->nnx, ny = g->nny, nz = g->nnz;
+  const int64_t nc = (int64_t)nx * ny * nz;
+  double *rad = calloc((size_t)nc, sizeof(double));
+  double *ncs = calloc((size_t)nc, sizeof(double));
+  int32_t *num = calloc((size_t)nc, sizeof(int32_t));
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t ci[5];
+    or_cell_index(g, soa[OR_X * n + i], soa[OR_Y * n + i], soa[OR_Z * n + i], ci);
+    const int ipt = ci[0], jpt = ci[1], kpt = ci[4];
+    if (ipt < 1 || ipt > nx || jpt < 1 || jpt > ny || kpt < 1 || kpt > nz) continue;
+    const int64_t e = (ipt - 1) + (int64_t)nx * ((jpt - 1) + (int64_t)ny * (kpt - 1));
+    rad[e] = rad[e] + soa[OR_RADIUS * n + i];
+    num[e] = num[e] + 1;
+    ncs[e] = ncs[e] + (double)mult[i];
+  }
+  double tauc_min = 1.0e10;
+  for (int iz = 1; iz <= nz; ++iz)
+    for (int iy = 1; iy <= ny; ++iy)
+      for (int ix = 1; ix <= nx; ++ix) {
+        const int64_t e = (ix - 1) + (int64_t)nx * ((iy - 1) + (int64_t)ny * (iz - 1));
+        if (num[e] != 0) {
+          const double radavg = rad[e] / (double)num[e];
+          const double Nc = ncs[e] / (g->dx * g->dy * g->dzw[iz]);
+          const double denom = c->pi2 * (c->nuf / c->Sc) * 2.0 * radavg * Nc;
+          const double t = 1.0 / (denom + 1.0e-10);
+          if (t < tauc_min) tauc_min = t;
+        }
+      }
+  free(rad);
+  free(ncs);
+  free(num);
+  return tauc_min * 5.0;
+}
+
+/* particle_xy_stats (3370-3770): the loops over the droplet list. */
+void or_xy_stats(const or_grid *g, const or_consts *c, const double *soa, const int64_t *mult, int64_t n,
+                 double *s0, double *s1, double *lwp) {
+  const int nx = g->nnx, ny = g->nny, nz2 = g->nnz + 2;
+  double cnt[3] = {0, 0, 0}, sum[6] = {0, 0, 0, 0, 0, 0};
+  double mx[6] = {0.0, 0.0, 0.0, -1000.0, -1000.0, -1000.0};
+  double mn[6] = {1000.0, 1000.0, 1000.0, 1000.0, 1000.0, 1000.0};
+  memset(s1, 0, sizeof(double) * 24 * nz2);
+  memset(lwp, 0, sizeof(double) * nx * ny);
+  const double pi = 4.0 * atan(1.0);
+  for (int64_t i = 0; i < n; ++i) {
+    double p[OR_NREAL];
+    for (int k = 0; k < OR_NREAL; ++k) p[k] = soa[(int64_t)k * n + i];
+    /* 0th order (3425-3468) */
+    cnt[0] += 1.0;
+    if (p[OR_RADIUS] > p[OR_RC]) cnt[1] += 1.0;
+    else cnt[2] += 1.0;
+    sum[4] = sum[4] + p[OR_RADIUS];
+    sum[5] = sum[5] + pow(p[OR_RADIUS], 2);
+    const double v0[6] = {p[OR_RADIUS], p[OR_TP], p[OR_QINF], p[OR_VX], p[OR_VY], p[OR_VZ]};
+    for (int k = 0; k < 6; ++k) {
+      mx[k] = v0[k] > mx[k] ? v0[k] : mx[k];
+      mn[k] = v0[k] < mn[k] ? v0[k] : mn[k];
+    }
+    const double Volp = c->pi2 * 2.0 / 3.0 * pow(p[OR_RADIUS], 3);
+    const double rhop0 = (p[OR_M_S] + Volp * c->rhow) / Volp;
+    const double d0 = p[OR_VX] - p[OR_UFX], d1 = p[OR_VY] - p[OR_UFY], d2 = p[OR_VZ] - p[OR_UFZ];
+    const double diffnorm = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double Rep = 2.0 * p[OR_RADIUS] * diffnorm / c->nuf;
+    sum[0] = sum[0] + Rep;
+    sum[1] = sum[1] + Volp * c->rhow;
+    sum[2] = sum[2] + Volp * rhop0;
+    sum[3] = sum[3] + Volp;
+    /* 1st order (3580-3636) */
+    int32_t ci[5];
+    or_cell_index(g, p[OR_X], p[OR_Y], p[OR_Z], ci);
+    const int ipt = ci[0], jpt = ci[1], kpt = ci[4];
+    const double m = (double)mult[i];
+    const double mw = c->rhow * 4.0 / 3.0 * pi * pow(p[OR_RADIUS], 3);
+    if (kpt >= 0 && kpt < nz2) {
+      const double q[24] = {1.0,
+                            p[OR_VX], p[OR_VY], p[OR_VZ],
+                            pow(p[OR_VX], 2), pow(p[OR_VY], 2), pow(p[OR_VZ], 2),
+                            p[OR_VX] * p[OR_VZ],
+                            p[OR_UFX], p[OR_UFY], p[OR_UFZ],
+                            pow(p[OR_UFX], 2), pow(p[OR_UFY], 2), pow(p[OR_UFZ], 2),
+                            p[OR_TP], pow(p[OR_TP], 2), p[OR_TF],
+                            p[OR_RADIUS], pow(p[OR_RADIUS], 2), p[OR_QINF],
+                            p[OR_TP] * p[OR_VZ], m, m * mw, p[OR_QSTAR]};
+      for (int k = 0; k < 24; ++k) s1[k * nz2 + kpt] = s1[k * nz2 + kpt] + q[k];
+    }
+    if (ipt >= 1 && ipt <= nx && jpt >= 1 && jpt <= ny) {
+      const int64_t e = (ipt - 1) + (int64_t)nx * (jpt - 1);
+      lwp[e] = lwp[e] + m * mw;
+    }
+  }
+  for (int k = 0; k < 3; ++k) s0[k] = cnt[k];
+  for (int k = 0; k < 6; ++k) s0[3 + k] = sum[k];
+  for (int k = 0; k < 6; ++k) s0[9 + k] = mx[k];
+  for (int k = 0; k < 6; ++k) s0[15 + k] = mn[k];
+}
+

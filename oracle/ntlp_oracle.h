@@ -124,6 +124,18 @@ void or_coupling_exchange(const or_grid *g, double *src, double *xsrc);
 void or_apply_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const double *xsrc,
                        double *T, double *q);
 
+/* calc_tauc_min (3310-3368) on one domain: 5 * the smallest condensation
+ * time scale over the w cells (ipt, jpt, kc) holding droplets. */
+double or_calc_tauc_min(const or_grid *g, const or_consts *c, const double *soa, const int64_t *mult, int64_t n);
+/* particle_xy_stats (3370-3770) on one domain, before the per-level
+ * divisions: s0[21] = numpart, numdrop, numaerosol, then the sums of Rep,
+ * water mass, droplet mass, volume, radius, radius^2, then the maxima of
+ * radius, Tp, qinf, vp(1:3) and the minima of the same (with the reference's
+ * starting values); s1[24][nnz+2] = statsvec1(iz, 1..24) for iz = 0..nnz+1;
+ * lwp[nny][nnx] = LWP(ipt, jpt). */
+void or_xy_stats(const or_grid *g, const or_consts *c, const double *soa, const int64_t *mult, int64_t n,
+                 double *s0, double *s1, double *lwp);
+
 /* single-droplet pieces exposed for pinning tests */
 void or_ie_vrt_nd(const or_consts *c, const double *p /* one droplet record, OR_* order */,
                   double rhoa, const double vnext[3], double tempr, double tempt, double h,

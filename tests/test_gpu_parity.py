@@ -306,3 +306,23 @@ def test_coupling_exchange_and_apply_match_oracle(iexner):
     bad = compare(mine, want, ["Tf", "qinf"], dict(TOL64, Tf=(1e-13, 300.0)))
     assert not bad, bad
     s.close()
+
+
+@pytest.mark.parametrize("kind", ["uniform", "stretched"])
+def test_statistics_match_oracle(kind):
+    """calc_tauc_min and particle_xy_stats on the device's own state after
+    two fused BE steps: counts, maxima and minima exact; sums within 1e-12
+    (summation order)."""
+    grid, consts, flags, fields, p, ora = small_case(seed=13, kind=kind)
+    s = make_step(grid, consts, flags, fields, p)
+    s.step(MODE_BE, PREC_FP64, nsteps=2, it0=2)
+    st = s.get()
+    assert abs(s.calc_tauc_min() / ora.calc_tauc_min(st.soa, st.mult) - 1) < 1e-12
+    s0, s1, lwp = s.particle_xy_stats()
+    w0, w1, wl = ora.xy_stats(st.soa, st.mult)
+    assert np.array_equal(s0[:3], w0[:3]) and np.array_equal(s0[9:], w0[9:])
+    assert np.allclose(s0[3:9], w0[3:9], rtol=1e-12, atol=0)
+    assert np.array_equal(s1[0], w1[0])
+    assert np.allclose(s1, w1, rtol=1e-12, atol=1e-300)
+    assert np.allclose(lwp, wl, rtol=1e-12, atol=0)
+    s.close()

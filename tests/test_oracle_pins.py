@@ -417,6 +417,79 @@ def test_apply_coupling_adds_dt_times_the_sources():
             assert np.array_equal(a[:, :, g.nnx:], a0[:, :, g.nnx:])
 
 
+# ------------------------------------------------------------ statistics
+
+def stats_population(g, n=3000, seed=6):
+    rng = np.random.default_rng(seed)
+    p = Particles.empty(n)
+    p["x"], p["y"], p["z"] = rng.uniform(0, g.xl, n), rng.uniform(0, g.yl, n), rng.uniform(0, g.zl, n)
+    for c in ("vx", "vy", "vz", "ufx", "ufy", "ufz"):
+        p[c] = 0.1 * rng.standard_normal(n)
+    p["Tp"], p["Tf"] = 300 + rng.standard_normal(n), 300 + rng.standard_normal(n)
+    p["radius"] = np.exp(rng.uniform(np.log(1e-6), np.log(2e-5), n))
+    p["rc"] = 1.5e-6
+    p["m_s"], p["qinf"], p["qstar"] = 1e-18, 0.02 + 1e-3 * rng.standard_normal(n), 0.02
+    p.mult[:] = rng.integers(1, 100, n)
+    return p
+
+
+def test_tauc_min_single_cell_closed_form():
+    """calc_tauc_min with every droplet in one w cell: 5/(2 pi (nuf/Sc) 2 r
+    N mult/(dx dy dzw) + 1e-10)."""
+    g = small_grid("stretched")
+    c = Consts()
+    o = ora_for(g, c)
+    n, r, m = 7, 4e-6, 30
+    p = Particles.empty(n)
+    p["x"], p["y"], p["z"], p["radius"] = 2.5 * g.dx, 3.5 * g.dy, 0.5 * (g.z[4] + g.z[5]), r
+    p.mult[:] = m
+    Nc = n * m / (g.dx * g.dy * g.dzw[5])
+    want = 5.0 / (2 * np.pi * (c.nuf / c.Sc) * 2 * r * Nc + 1e-10)
+    assert abs(o.calc_tauc_min(p.soa, p.mult) / want - 1) < 1e-13
+    assert o.calc_tauc_min(p.soa[:, :0], p.mult[:0]) == 5.0e10
+
+
+def test_statistics_match_numpy_bincounts():
+    """calc_tauc_min and particle_xy_stats against numpy: cells from
+    searchsorted on z, sums by np.add.at, starting values of the maxima and
+    minima as the reference sets them."""
+    g = small_grid("stretched")
+    c = Consts()
+    o = ora_for(g, c)
+    p = stats_population(g)
+    ix, iy = np.floor(p["x"] / g.dx).astype(int), np.floor(p["y"] / g.dy).astype(int)
+    kc = np.searchsorted(g.z, p["z"], side="right")
+    ok = (ix < g.nnx) & (iy < g.nny) & (kc >= 1) & (kc <= g.nnz)
+    cell = (ix + g.nnx * (iy + g.nny * (kc - 1)))[ok]
+    nc = g.nnx * g.nny * g.nnz
+    num = np.bincount(cell, minlength=nc)
+    rad = np.bincount(cell, weights=p["radius"][ok], minlength=nc)
+    mult = np.bincount(cell, weights=p.mult[ok].astype(float), minlength=nc)
+    occ = num > 0
+    dzw = g.dzw[1:g.nnz + 1][np.arange(nc) // (g.nnx * g.nny)]
+    denom = c.pi2 * (c.nuf / c.Sc) * 2 * (rad[occ] / num[occ]) * (mult[occ] / (g.dx * g.dy * dzw[occ]))
+    assert abs(o.calc_tauc_min(p.soa, p.mult) / (5 * np.min(1 / (denom + 1e-10))) - 1) < 1e-12
+    s0, s1, lwp = o.xy_stats(p.soa, p.mult)
+    r = p["radius"]
+    volp = 4.0 / 3.0 * np.pi * r ** 3
+    rep = 2 * r * np.sqrt((p["vx"] - p["ufx"]) ** 2 + (p["vy"] - p["ufy"]) ** 2 + (p["vz"] - p["ufz"]) ** 2) / c.nuf
+    want0 = [p.n, np.sum(r > p["rc"]), np.sum(r <= p["rc"]), rep.sum(), (volp * c.rhow).sum(),
+             (volp * c.rhow + p["m_s"]).sum(), volp.sum(), r.sum(), (r * r).sum(),
+             max(r.max(), 0), max(p["Tp"].max(), 0), max(p["qinf"].max(), 0), max(p["vx"].max(), -1000),
+             max(p["vy"].max(), -1000), max(p["vz"].max(), -1000), min(r.min(), 1000), min(p["Tp"].min(), 1000),
+             min(p["qinf"].min(), 1000), min(p["vx"].min(), 1000), min(p["vy"].min(), 1000), min(p["vz"].min(), 1000)]
+    assert np.allclose(s0, want0, rtol=1e-12, atol=0)
+    m = p.mult.astype(float)
+    qs = [np.ones(p.n), p["vx"], p["vy"], p["vz"], p["vx"] ** 2, p["vy"] ** 2, p["vz"] ** 2, p["vx"] * p["vz"],
+          p["ufx"], p["ufy"], p["ufz"], p["ufx"] ** 2, p["ufy"] ** 2, p["ufz"] ** 2, p["Tp"], p["Tp"] ** 2, p["Tf"],
+          r, r * r, p["qinf"], p["Tp"] * p["vz"], m, m * c.rhow * volp, p["qstar"]]
+    for k, q in enumerate(qs):
+        assert np.allclose(s1[k], np.bincount(kc, weights=q, minlength=g.nnz + 2), rtol=1e-12, atol=1e-300), k
+    inxy = (ix < g.nnx) & (iy < g.nny)
+    want_lwp = np.bincount((ix + g.nnx * iy)[inxy], weights=(m * c.rhow * volp)[inxy], minlength=g.nnx * g.nny)
+    assert np.allclose(lwp.ravel(), want_lwp, rtol=1e-12, atol=0)
+
+
 # ----------------------------------------------------------------- walls
 
 def test_boundary_conditions_hand_cases():
