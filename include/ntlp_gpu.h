@@ -143,6 +143,13 @@ int ntlp_gpu_particles_get(double *real_soa, int64_t *mult, int32_t *pidx, int32
 int ntlp_gpu_particles_put_records(const void *records, int64_t n);
 int ntlp_gpu_particles_get_records(void *records, int64_t n);
 int64_t ntlp_gpu_particles_count(void);
+/* save_particles / read_part_res (particles.f90:1988-2146) for one rank: the
+ * file is an int32 droplet count followed by one 352-byte record per droplet.
+ * Reading keeps the records with 0 < x < xl and 0 < y < yl (the reference's
+ * strict bounds test) and replaces the droplets on the device; *kept (may be
+ * NULL) receives their number. */
+int ntlp_gpu_save_particles(const char *path);
+int ntlp_gpu_read_part_res(const char *path, int64_t *kept);
 
 /* ------------------------------------------------- reference entry points */
 

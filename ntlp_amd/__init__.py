@@ -208,6 +208,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_particles_put_records": (ctypes.c_int, [vp, i64]),
         "ntlp_gpu_particles_get_records": (ctypes.c_int, [vp, i64]),
         "ntlp_gpu_particles_count": (i64, []),
+        "ntlp_gpu_save_particles": (ctypes.c_int, [ctypes.c_char_p]),
+        "ntlp_gpu_read_part_res": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(i64)]),
         "ntlp_gpu_interp": (ctypes.c_int, [i32, i32]),
         "ntlp_gpu_crit_radius": (ctypes.c_int, []),
         "ntlp_gpu_particle_update_rk3": (ctypes.c_int, [i32, i32]),
@@ -297,6 +299,17 @@ class DropletStep:
                       np.empty(n, dtype=np.int32))
         self._check(self.lib.ntlp_gpu_particles_get(_ptr(p.soa), _ptr(p.mult), _ptr(p.pidx), _ptr(p.procidx), n))
         return p
+
+    def save_particles(self, path: str) -> None:
+        """save_particles (particles.f90:1988-2046): int32 count + 352-byte records."""
+        self._check(self.lib.ntlp_gpu_save_particles(os.fsencode(path)))
+
+    def read_part_res(self, path: str) -> int:
+        """read_part_res (particles.f90:2048-2146): returns the droplets kept."""
+        k = ctypes.c_int64()
+        self._check(self.lib.ntlp_gpu_read_part_res(os.fsencode(path), ctypes.byref(k)))
+        self.n = int(k.value)
+        return self.n
 
     def put_records(self, records: np.ndarray) -> None:
         rec = np.ascontiguousarray(records)

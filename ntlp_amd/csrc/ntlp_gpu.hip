@@ -782,6 +782,68 @@ int ntlp_gpu_particles_get_records(void* records, int64_t n) {
   return NTLP_OK;
 }
 
+// save_particles (particles.f90:1988-2046) on one rank: the int32 droplet
+// count at offset 0, then one 352-byte record per droplet.
+int ntlp_gpu_save_particles(const char* path) {
+  CHK(need_init());
+  if (!path) return fail(NTLP_E_INVALID, "null path");
+  if (ctx.n > (int64_t)INT32_MAX) return fail(NTLP_E_RANGE, "the file's count is an int32");
+  std::vector<unsigned char> rec((size_t)ctx.n * NTLP_RECORD_BYTES);
+  CHK(ntlp_gpu_particles_get_records(rec.data(), ctx.n));
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(NTLP_E_INVALID, std::string("cannot open ") + path + " for writing");
+  const int32_t count = (int32_t)ctx.n;
+  bool ok = fwrite(&count, 4, 1, f) == 1;
+  if (ok && !rec.empty()) ok = fwrite(rec.data(), 1, rec.size(), f) == rec.size();
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) return fail(NTLP_E_INVALID, std::string("write to ") + path + " failed");
+  return NTLP_OK;
+}
+
+// read_part_res (particles.f90:2048-2146) on one rank: reads the count and
+// the records in chunks of 5 million and keeps those inside the rank's open
+// box xmin < x < xmax, ymin < y < ymax (here 0 < x < xl, 0 < y < yl), as the
+// reference does; the records advance by 352 bytes, the writer's stride.
+int ntlp_gpu_read_part_res(const char* path, int64_t* kept) {
+  CHK(need_init());
+  if (!path) return fail(NTLP_E_INVALID, "null path");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(NTLP_E_INVALID, std::string("cannot open ") + path);
+  int32_t count = 0;
+  if (fread(&count, 4, 1, f) != 1 || count < 0) {
+    fclose(f);
+    return fail(NTLP_E_INVALID, std::string(path) + ": no droplet count");
+  }
+  std::vector<unsigned char> keep;
+  const int64_t chunk = 5000000;
+  std::vector<unsigned char> buf;
+  int64_t left = count, nkeep = 0;
+  constexpr int kX = 16 + 3 * 8, kY = 16 + 4 * 8;  // xp(1), xp(2) follow vp(1:3)
+  while (left > 0) {
+    const int64_t m = std::min(left, chunk);
+    buf.resize((size_t)m * NTLP_RECORD_BYTES);
+    if (fread(buf.data(), 1, buf.size(), f) != buf.size()) {
+      fclose(f);
+      return fail(NTLP_E_INVALID, std::string(path) + ": fewer records than its count");
+    }
+    for (int64_t i = 0; i < m; ++i) {
+      const unsigned char* r = buf.data() + i * NTLP_RECORD_BYTES;
+      double x, y;
+      memcpy(&x, r + kX, 8);
+      memcpy(&y, r + kY, 8);
+      if (y > 0.0 && y < ctx.g.yl && x > 0.0 && x < ctx.g.xl) {
+        keep.insert(keep.end(), r, r + NTLP_RECORD_BYTES);
+        ++nkeep;
+      }
+    }
+    left -= m;
+  }
+  fclose(f);
+  CHK(ntlp_gpu_particles_put_records(keep.data(), nkeep));
+  if (kept) *kept = nkeep;
+  return NTLP_OK;
+}
+
 int ntlp_gpu_crit_radius(void) {
   CHK(need_init());
   if (ctx.n <= 0) return fail(NTLP_E_STATE, "no droplets");

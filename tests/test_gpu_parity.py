@@ -120,6 +120,30 @@ def test_put_get_and_records_roundtrip():
     s.close()
 
 
+def test_restart_file_roundtrip(tmp_path):
+    """save_particles writes an int32 count and 352-byte records; read_part_res
+    reads them back, keeping only 0 < x < xl, 0 < y < yl like the reference."""
+    grid, consts, flags, fields, p, ora = small_case(seed=14, n=1500)
+    p["x"][:3] = [0.0, grid.xl * 0.5, grid.xl * 0.25]
+    p["y"][:3] = [grid.yl * 0.5, 0.0, grid.yl]
+    s = make_step(grid, consts, flags, fields, p)
+    s.particle_update_BE(it=2)
+    path = str(tmp_path / "part.res")
+    s.save_particles(path)
+    raw = np.fromfile(path, dtype=np.uint8)
+    assert raw.size == 4 + 352 * p.n and int(raw[:4].view(np.int32)[0]) == p.n
+    assert np.array_equal(raw[4:], s.get_records())
+    saved = s.get()
+    kept = s.read_part_res(path)
+    inside = (saved["x"] > 0) & (saved["x"] < grid.xl) & (saved["y"] > 0) & (saved["y"] < grid.yl)
+    assert kept == int(inside.sum()) and kept <= p.n - 3
+    back = s.get()
+    for c in ("x", "y", "z", "vx", "vy", "vz", "Tp", "radius", "m_s", "kappa_s", "rc", "res", "numact", "qstar"):
+        assert np.array_equal(back[c], saved[c][inside]), c
+    assert np.array_equal(back.mult, saved.mult[inside]) and np.array_equal(back.pidx, saved.pidx[inside])
+    s.close()
+
+
 @pytest.mark.parametrize("kind", ["uniform", "stretched"])
 def test_particle_update_be_fp64(kind):
     grid, consts, flags, fields, p, ora = small_case(seed=6, kind=kind)
