@@ -126,6 +126,7 @@ def test_restart_file_roundtrip(tmp_path):
     grid, consts, flags, fields, p, ora = small_case(seed=14, n=1500)
     p["x"][:3] = [0.0, grid.xl * 0.5, grid.xl * 0.25]
     p["y"][:3] = [grid.yl * 0.5, 0.0, grid.yl]
+    p["vx"][0] = p["vy"][1] = p["vy"][2] = 0.0  # x += dt*vp_old keeps them on the bounds
     s = make_step(grid, consts, flags, fields, p)
     s.particle_update_BE(it=2)
     path = str(tmp_path / "part.res")
