@@ -84,6 +84,7 @@ struct StepArgs {
   int it;
   int istage;
   int write_cold;        // write the record diagnostics (final step of a fused call)
+  double* diag;          // [kNCold][cap] diagnostics of the final step in dst slot order
 };
 
 // Workgroup -> droplet chunk mapping that keeps consecutive chunks on one XCD:
@@ -336,10 +337,10 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
       key = ukey_of(a.g, c);
       a.keys[t] = key;
     }
-    if (!SERVE && a.write_cold) {
-      store_gathered(a.cold, a.cap, home, o);
-      store_old(a.cold, a.cap, home, o);
-      if (MODE == 1) store_rhs(a.cold, a.cap, home, o);
+    if (!SERVE && a.write_cold) {  // coalesced in slot order; put order is restored on demand
+      store_gathered(a.diag, a.cap, t, o);
+      store_old(a.diag, a.cap, t, o);
+      if (MODE == 1) store_rhs(a.diag, a.cap, t, o);
     }
   }
   count_key(a.counts, key, valid);
@@ -759,19 +760,22 @@ __global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
     double am[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) am[f] = ((a.fmask >> f) & 1u) ? a.amounts[aix(f, slot)] : 0.0;
+    // Neighbouring lanes hold droplets of the same or adjacent cells, whose
+    // corners are the same LDS words; each lane starts at a different corner
+    // so that one atomic instruction rarely hits a word twice.
+    const int rot = lane_id() & 7;
 #pragma unroll
-    for (int ci = 0; ci < 2; ++ci)
+    for (int s8 = 0; s8 < 8; ++s8) {
+      const int cc = (s8 + rot) & 7;
+      const int ci = cc >> 2, cj = (cc >> 1) & 1, ck = cc & 1;
+      const int li = ci ? lx[1] : lx[0], lj = cj ? ly[1] : ly[0], lk = ck ? lz[1] : lz[0];
+      if (li < 0 || lj < 0 || lk < 0) continue;
+      const double w = (ci ? wtx[1] : wtx[0]) * (cj ? wty[1] : wty[0]) * (ck ? wtz[1] : wtz[0]) / dV;
+      const int l = (li * TJ + lj) * TK + lk;
 #pragma unroll
-      for (int cj = 0; cj < 2; ++cj)
-#pragma unroll
-        for (int ck = 0; ck < 2; ++ck) {
-          if (lx[ci] < 0 || ly[cj] < 0 || lz[ck] < 0) continue;
-          const double w = wtx[ci] * wty[cj] * wtz[ck] / dV;
-          const int l = (lx[ci] * TJ + ly[cj]) * TK + lz[ck];
-#pragma unroll
-          for (int f = 0; f < 6; ++f)
-            if ((a.fmask >> f) & 1u) atomicAdd(&s_acc[f * kNodes + l], am[f] * w);
-        }
+      for (int f = 0; f < 6; ++f)
+        if ((a.fmask >> f) & 1u) atomicAdd(&s_acc[f * kNodes + l], am[f] * w);
+    }
   }
   __syncthreads();
   // store the box: z fastest, so consecutive threads write consecutive levels
@@ -1367,6 +1371,16 @@ __global__ void k_unpack_hot(const double* hot, int c, const int32_t* home, int6
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) out[home ? home[t] : t] = hot[hix(c, t)];
 }
+// Diagnostic columns (mask bit c = cold column c) from slot order to put order.
+__global__ void k_flush_diag(const double* diag, const int32_t* home, int64_t n, int64_t cap, unsigned mask,
+                             double* cold) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t h = home[t];
+  for (int c = 0; c < kNCold; ++c)
+    if ((mask >> c) & 1u) cold[c * cap + h] = diag[c * cap + t];
+}
+
 // slot-order column -> put order via home
 __global__ void k_scatter_col_i64(const int64_t* col, const int32_t* home, int64_t n, int64_t* out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -70,6 +70,9 @@ struct Ctx {
   int32_t* pidx = nullptr;
   int32_t* procidx = nullptr;
   double* amounts = nullptr;
+  bool amounts_valid = false;  // amounts belong to the last step of the current slots
+  double* diag = nullptr;      // last step's diagnostics in slot order (kNCold columns)
+  unsigned diag_mask = 0;      // cold columns waiting in diag
   int32_t* keys[2] = {};
   int32_t* perm = nullptr;
   bool perm_valid = false;  // perm sorts the current slots by u cell
@@ -245,6 +248,7 @@ void release_all() {
   dfree(ctx.pidx);
   dfree(ctx.procidx);
   dfree(ctx.amounts);
+  dfree(ctx.diag);
   dfree(ctx.keys[0]);
   dfree(ctx.keys[1]);
   dfree(ctx.perm);
@@ -296,6 +300,7 @@ StepArgs base_args() {
   a.perm = nullptr;
   a.cold = ctx.cold;
   a.amounts = ctx.amounts;
+  a.diag = ctx.diag;
   a.keys_in = nullptr;
   a.keys = ctx.keys[ctx.cur];
   a.counts = ctx.counts;
@@ -386,7 +391,7 @@ int pull_sorted() {
 }
 
 int scatter_pull() {
-  CHK(bin_current(nullptr, nullptr));
+  if (!ctx.perm_valid || !ctx.keys_valid) CHK(bin_current(nullptr, nullptr));
   return pull_sorted();
 }
 
@@ -418,8 +423,30 @@ int tile_sorted() {
 }
 
 int scatter_tile() {
-  CHK(bin_current(nullptr, nullptr));
+  if (!ctx.perm_valid || !ctx.keys_valid) CHK(bin_current(nullptr, nullptr));
   return tile_sorted();
+}
+
+// cold columns of the diagnostics the last fused step left in slot order
+constexpr unsigned kDiagBE = (1u << kCUFX) | (1u << kCUFY) | (1u << kCUFZ) | (1u << kCTF) | (1u << kCQINF) |
+                             (1u << kCQSTAR) | (1u << kCVXOLD) | (1u << kCVYOLD) | (1u << kCVZOLD) |
+                             (1u << kCTPOLD) | (1u << kCRADIUSOLD);
+constexpr unsigned kDiagRK3 = kDiagBE | (1u << kCXRHSX) | (1u << kCXRHSY) | (1u << kCXRHSZ) | (1u << kCVRHSX) |
+                              (1u << kCVRHSY) | (1u << kCVRHSZ) | (1u << kCTPRHS_S) | (1u << kCTPRHS_L) |
+                              (1u << kCRADRHS);
+
+// puts the pending diagnostics into the put-order cold columns
+int flush_diag() {
+  if (!ctx.diag_mask) return 0;
+  const unsigned mask = ctx.diag_mask;
+  const double* diag = ctx.diag;
+  const int32_t* home = ctx.home[ctx.cur];
+  double* cold = ctx.cold;
+  CHK(launch("flush_diag", [&] {
+    k_flush_diag<<<grid_flat(ctx.n), kBlock, 0, ctx.stream>>>(diag, home, ctx.n, ctx.cap, mask, cold);
+  }));
+  ctx.diag_mask = 0;
+  return 0;
 }
 
 template <int MODE, int PREC, int ILIN>
@@ -461,6 +488,37 @@ int close_step() {
   unsigned long long* row = ctx.tally + (ctx.step_row % kTallyRows) * kTallyWidth;
   unsigned long long* slots = ctx.tally_slots;
   return launch("tally", [&] { k_tally_close<<<1, 128, 0, ctx.stream>>>(slots, row, 1); });
+}
+
+// nsteps fused steps: each reads the droplets through the cell sort, writes
+// them in slot order with their new keys, coupling amounts and histogram,
+// then sorts for the next step and (scatter) deposits the sources.  The last
+// step leaves its diagnostics in slot order (put order is restored when a
+// caller needs the cold columns).
+int run_fused(int mode, int prec, int nsteps, int it0, bool scatter) {
+  const unsigned mask = mode == NTLP_MODE_RK3 ? kDiagRK3 : kDiagBE;
+  if (ctx.diag_mask & ~mask) CHK(flush_diag());
+  if (!ctx.perm_valid || !ctx.keys_valid) CHK(bin_current(nullptr, nullptr));
+  for (int k = 0; k < nsteps; ++k) {
+    StepArgs a = base_args();
+    a.perm = ctx.perm;
+    a.keys_in = ctx.keys[ctx.cur];
+    a.keys = ctx.keys[1 - ctx.cur];
+    a.it = it0 + k;
+    a.write_cold = (k == nsteps - 1) ? 1 : 0;
+    if (mode == NTLP_MODE_RK3) CHK(dispatch_step<1>(a, prec));
+    else CHK(dispatch_step<2>(a, prec));
+    ctx.cur = 1 - ctx.cur;
+    ctx.home_identity = false;
+    ctx.keys_valid = true;
+    CHK(close_step());
+    CHK(finish_bin(nullptr));
+    if (scatter) CHK(tile_sorted());
+  }
+  ctx.diag_mask = mask;
+  ctx.amounts_valid = true;
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return 0;
 }
 
 }  // namespace
@@ -652,6 +710,7 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
     CHK(dalloc(ctx.pidx, n, "pidx"));
     CHK(dalloc(ctx.procidx, n, "procidx"));
     CHK(dalloc(ctx.amounts, tiled_elems(n, 6), "coupling amounts"));
+    CHK(dalloc(ctx.diag, kNCold * n, "step diagnostics"));
     CHK(dalloc(ctx.keys[0], n, "keys"));
     CHK(dalloc(ctx.keys[1], n, "keys"));
     CHK(dalloc(ctx.perm, n, "perm"));
@@ -661,6 +720,8 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
   ctx.n = n;
   ctx.cur = 0;
   ctx.home_identity = true;
+  ctx.diag_mask = 0;
+  ctx.amounts_valid = false;
   ctx.perm_valid = false;
   ctx.keys_valid = false;
   if (n == 0) return NTLP_OK;
@@ -690,6 +751,7 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
 int ntlp_gpu_particles_get(double* soa, int64_t* mult, int32_t* pidx, int32_t* procidx, int64_t n) {
   CHK(need_init());
   if (n != ctx.n) return fail(NTLP_E_INVALID, "n does not match the droplet count on the device");
+  if (soa) CHK(flush_diag());
   if (n == 0) return NTLP_OK;
   const int64_t cap = ctx.cap;
   const int32_t* home = ctx.home[ctx.cur];
@@ -859,6 +921,8 @@ int ntlp_gpu_crit_radius(void) {
 
 int ntlp_gpu_interp(int32_t set_vp, int32_t set_tp) {
   CHK(need_drops());
+  CHK(flush_diag());
+  if (set_vp || set_tp) ctx.amounts_valid = false;
   StepArgs a = base_args();
   const bool lin = ctx.fl.ilin == 1;
   CHK(launch("interp", [&] {
@@ -878,7 +942,9 @@ int ntlp_gpu_particle_update_rk3_prec(int32_t istage, int32_t it, int32_t prec) 
   CHK(need_drops());
   CHK(check_prec(prec));
   if (istage < 1 || istage > 3) return fail(NTLP_E_INVALID, "istage must be 1, 2 or 3");
+  CHK(flush_diag());
   ctx.keys_valid = false;
+  ctx.amounts_valid = false;
   StepArgs a = base_args();
   a.istage = istage;
   a.it = it;
@@ -891,24 +957,24 @@ int ntlp_gpu_particle_update_rk3(int32_t istage, int32_t it) {
   return ntlp_gpu_particle_update_rk3_prec(istage, it, PREC_FP64);
 }
 
+// One BE step with the fused machinery (cell sorted, coupling amounts kept for
+// particle_coupling_update); the droplets keep their put order to callers.
 int ntlp_gpu_particle_update_be_prec(int32_t it, int32_t prec) {
   CHK(need_drops());
   CHK(check_prec(prec));
-  ctx.keys_valid = false;
-  StepArgs a = base_args();
-  a.it = it;
-  CHK(dispatch_update<2>(a, prec));
-  CHK(close_step());
-  HIPCHK(hipStreamSynchronize(ctx.stream));
-  return NTLP_OK;
+  return run_fused(NTLP_MODE_BE, prec, 1, it, false);
 }
 
 int ntlp_gpu_particle_update_be(int32_t it) { return ntlp_gpu_particle_update_be_prec(it, PREC_FP64); }
 
 int ntlp_gpu_particle_coupling_update(double* src_out, int32_t deterministic) {
   CHK(need_drops());
-  StepArgs a = base_args();
-  CHK(launch("amounts", [&] { k_amounts<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(a); }));
+  if (!ctx.amounts_valid) {  // amounts from the state and its old values in cold
+    CHK(flush_diag());
+    StepArgs a = base_args();
+    CHK(launch("amounts", [&] { k_amounts<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(a); }));
+    ctx.amounts_valid = true;
+  }
   if (deterministic) CHK(scatter_pull());
   else CHK(scatter_tile());
   if (src_out)
@@ -1104,6 +1170,7 @@ int ntlp_gpu_calc_tauc_min(double* tauc_min) {
 
 int ntlp_gpu_particle_xy_stats(double* s0, double* s1, double* lwp) {
   CHK(need_drops());
+  CHK(flush_diag());
   const int nz2 = ctx.g.nnz + 2;
   const int rows = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (ctx.n + kBlock - 1) / kBlock));
   const int nq_max = std::max(1, std::min(24, 6144 / nz2));
@@ -1162,26 +1229,7 @@ int ntlp_gpu_step(int32_t mode, int32_t prec, int32_t nsteps, int32_t it0) {
   CHK(check_prec(prec));
   if (mode != NTLP_MODE_RK3 && mode != NTLP_MODE_BE) return fail(NTLP_E_INVALID, "unknown mode");
   if (nsteps < 1) return fail(NTLP_E_INVALID, "nsteps must be >= 1");
-  // the first step of a call reads the droplets cell sorted too
-  if (!ctx.perm_valid || !ctx.keys_valid) CHK(bin_current(nullptr, nullptr));
-  for (int k = 0; k < nsteps; ++k) {
-    StepArgs a = base_args();
-    a.perm = ctx.perm;
-    a.keys_in = ctx.keys[ctx.cur];
-    a.keys = ctx.keys[1 - ctx.cur];
-    a.it = it0 + k;
-    a.write_cold = (k == nsteps - 1) ? 1 : 0;
-    if (mode == NTLP_MODE_RK3) CHK(dispatch_step<1>(a, prec));
-    else CHK(dispatch_step<2>(a, prec));
-    ctx.cur = 1 - ctx.cur;
-    ctx.home_identity = false;
-    ctx.keys_valid = true;
-    CHK(close_step());
-    CHK(finish_bin(nullptr));
-    CHK(tile_sorted());
-  }
-  HIPCHK(hipStreamSynchronize(ctx.stream));
-  return NTLP_OK;
+  return run_fused(mode, prec, nsteps, it0, true);
 }
 
 int ntlp_gpu_counters(ntlp_counters* out, int32_t reset) {
