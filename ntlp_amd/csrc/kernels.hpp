@@ -84,6 +84,7 @@ struct StepArgs {
   int it;
   int istage;
   int write_cold;        // write the record diagnostics (final step of a fused call)
+  int inplace;           // update slot t in place (src == dst, no perm, keys == keys_in)
   double* diag;          // [kNCold][cap] diagnostics of the final step in dst slot order
 };
 
@@ -282,6 +283,9 @@ __device__ __forceinline__ void count_key(int* counts, int key, bool valid) {
 // conditions + coupling amounts + new u-cell key and histogram.  Reads the
 // droplet through perm from the source buffers and writes it to slot t of
 // the destination buffers, so that the destination is cell sorted.
+// inplace: a step between two such reorderings updates slot t where it is
+// (the slots stay nearly cell sorted for many steps: a droplet moves a small
+// fraction of a cell per step); only the state that changes is written back.
 //
 // SERVE: the step of a serving pool.  The droplet count comes from the control
 // block; a droplet whose request takes its last step here is written to the
@@ -298,11 +302,12 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   const bool valid = t < (SERVE ? a.serve->n_live : a.n);
   Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
   int key = -1;
+  const bool inplace = !SERVE && a.inplace;
   if (valid) {
-    const int64_t s = a.perm ? (int64_t)a.perm[t] : t;
+    const int64_t s = (a.perm && !inplace) ? (int64_t)a.perm[t] : t;
     Drop d = load_drop(a.src.hot, s);
     const int64_t mult = a.src.mult[s];
-    const int32_t home = a.src.home[s];
+    const int32_t home = inplace ? 0 : a.src.home[s];
     Diag o;
     const int kg = a.keys_in ? a.keys_in[s] % (a.g.nnz + 1) : -1;
     if (MODE == 1) {  // three particle_update_rk3 stages; stage registers stay in VGPRs
@@ -328,11 +333,13 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
         for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];
       }
       store_dynamic(a.dst.hot, t, d);
-      a.dst.hot[hix(kHMS, t)] = d.m_s;
-      a.dst.hot[hix(kHKAPPA, t)] = d.kappa;
-      a.dst.hot[hix(kHRC, t)] = d.rc;
-      a.dst.mult[t] = mult;
-      a.dst.home[t] = home;
+      if (!inplace) {
+        a.dst.hot[hix(kHMS, t)] = d.m_s;
+        a.dst.hot[hix(kHKAPPA, t)] = d.kappa;
+        a.dst.hot[hix(kHRC, t)] = d.rc;
+        a.dst.mult[t] = mult;
+        a.dst.home[t] = home;
+      }
       const Cell c = cell_near(a.g, s_z, s_zz, d.x, d.y, d.z, o.kpt);
       key = ukey_of(a.g, c);
       a.keys[t] = key;
@@ -715,12 +722,12 @@ __global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
   const int ncx = s_ncx, ncy = s_ncy, nseg = ncx * ncy;
   const int klo = max(0, K0 - 1), khi = min(nnz, K1);
   const int nz1 = nnz + 1, ny1 = nny + 1;
-  if (threadIdx.x < nseg) {
-    const int cx = s_cx[threadIdx.x / ncy], cy = s_cy[threadIdx.x % ncy];
+  for (int sg = threadIdx.x; sg < nseg; sg += blockDim.x) {
+    const int cx = s_cx[sg / ncy], cy = s_cy[sg % ncy];
     const int base = nz1 * (cy + ny1 * cx);
     const int beg = a.offsets[base + klo], end = a.offsets[base + khi + 1];
-    s_beg[threadIdx.x] = beg;
-    s_pre[threadIdx.x + 1] = end - beg;
+    s_beg[sg] = beg;
+    s_pre[sg + 1] = end - beg;
   }
   __syncthreads();
   if (threadIdx.x == 0) {

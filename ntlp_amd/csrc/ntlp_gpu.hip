@@ -76,6 +76,9 @@ struct Ctx {
   int32_t* keys[2] = {};
   int32_t* perm = nullptr;
   bool perm_valid = false;  // perm sorts the current slots by u cell
+  bool perm_canonical = false;  // ... and within a cell by droplet (home)
+  bool slots_sorted = false;    // the slots were last written in cell order
+  int since_reorder = 0;        // fused steps since then
   bool keys_valid = false;  // keys[cur] are the cells of the current positions
   double* col_tmp = nullptr;
   // bins
@@ -321,16 +324,33 @@ int scan_counts() {
 
 // keys + counts of the current slot order must be in place; leaves perm
 // sorted by u cell (ties by `tie`, or by slot) and counts zeroed.
-int finish_bin(const int32_t* tie) {
+// canonical: order the droplets within each cell by `tie` (or by slot), so
+// that the permutation does not depend on the order of the scatter's
+// atomics.  The fused step does not need that (a droplet's arithmetic does
+// not depend on its slot); the bin queries and the deterministic coupling do.
+int finish_bin(const int32_t* tie, bool canonical = true) {
   CHK(scan_counts());
   CHK(launch("bin_scatter", [&] {
     k_bin_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(ctx.keys[ctx.cur], ctx.n, ctx.offsets, ctx.counts, ctx.perm,
                                                               nullptr);
   }));
-  CHK(launch("bin_cellsort", [&] {
-    k_cell_sort<<<grid_flat(ctx.ncell), kBlock, 0, ctx.stream>>>(ctx.offsets, ctx.ncell, ctx.perm, tie);
-  }));
+  if (canonical)
+    CHK(launch("bin_cellsort", [&] {
+      k_cell_sort<<<grid_flat(ctx.ncell), kBlock, 0, ctx.stream>>>(ctx.offsets, ctx.ncell, ctx.perm, tie);
+    }));
   ctx.perm_valid = true;
+  ctx.perm_canonical = canonical && tie == ctx.home[ctx.cur];
+  return 0;
+}
+
+// orders the current permutation within cells by droplet identity (home)
+int canonical_perm() {
+  if (ctx.perm_canonical) return 0;
+  const int32_t* home = ctx.home[ctx.cur];
+  CHK(launch("bin_cellsort", [&] {
+    k_cell_sort<<<grid_flat(ctx.ncell), kBlock, 0, ctx.stream>>>(ctx.offsets, ctx.ncell, ctx.perm, home);
+  }));
+  ctx.perm_canonical = true;
   return 0;
 }
 
@@ -392,11 +412,12 @@ int pull_sorted() {
 
 int scatter_pull() {
   if (!ctx.perm_valid || !ctx.keys_valid) CHK(bin_current(nullptr, nullptr));
+  CHK(canonical_perm());  // a fixed summation order
   return pull_sorted();
 }
 
 // node boxes of the tiled scatter
-constexpr int kTileI = 4, kTileJ = 4, kTileK = 32;
+constexpr int kTileI = 8, kTileJ = 8, kTileK = 8;
 
 // tiled scatter over the current cell sort (keys, perm and offsets must
 // describe the current slots); writes every node of the 6 fields
@@ -495,24 +516,41 @@ int close_step() {
 // then sorts for the next step and (scatter) deposits the sources.  The last
 // step leaves its diagnostics in slot order (put order is restored when a
 // caller needs the cold columns).
+// Steps between two reorderings of the slots into cell order.
+constexpr int kReorderEvery = 16;
+
 int run_fused(int mode, int prec, int nsteps, int it0, bool scatter) {
   const unsigned mask = mode == NTLP_MODE_RK3 ? kDiagRK3 : kDiagBE;
   if (ctx.diag_mask & ~mask) CHK(flush_diag());
   if (!ctx.perm_valid || !ctx.keys_valid) CHK(bin_current(nullptr, nullptr));
   for (int k = 0; k < nsteps; ++k) {
+    const bool reorder = !ctx.slots_sorted || ctx.since_reorder + 1 >= kReorderEvery;
     StepArgs a = base_args();
-    a.perm = ctx.perm;
-    a.keys_in = ctx.keys[ctx.cur];
-    a.keys = ctx.keys[1 - ctx.cur];
     a.it = it0 + k;
     a.write_cold = (k == nsteps - 1) ? 1 : 0;
+    if (reorder) {  // read through the sort, write slot t of the other buffer
+      a.perm = ctx.perm;
+      a.keys_in = ctx.keys[ctx.cur];
+      a.keys = ctx.keys[1 - ctx.cur];
+    } else {
+      a.inplace = 1;
+      a.dst = a.src;
+      a.keys_in = ctx.keys[ctx.cur];
+      a.keys = ctx.keys[ctx.cur];
+    }
     if (mode == NTLP_MODE_RK3) CHK(dispatch_step<1>(a, prec));
     else CHK(dispatch_step<2>(a, prec));
-    ctx.cur = 1 - ctx.cur;
-    ctx.home_identity = false;
+    if (reorder) {
+      ctx.cur = 1 - ctx.cur;
+      ctx.home_identity = false;
+      ctx.slots_sorted = true;
+      ctx.since_reorder = 0;
+    } else {
+      ctx.since_reorder++;
+    }
     ctx.keys_valid = true;
     CHK(close_step());
-    CHK(finish_bin(nullptr));
+    CHK(finish_bin(nullptr, false));
     if (scatter) CHK(tile_sorted());
   }
   ctx.diag_mask = mask;
@@ -722,6 +760,7 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
   ctx.home_identity = true;
   ctx.diag_mask = 0;
   ctx.amounts_valid = false;
+  ctx.slots_sorted = false;
   ctx.perm_valid = false;
   ctx.keys_valid = false;
   if (n == 0) return NTLP_OK;

@@ -251,7 +251,9 @@ void serve_launch_step(const StepArgs& a, unsigned grid) {
 
 // The per-iteration kernels for a pool of at most `size` slots: counting sort
 // of the current buffer (the counts were gathered by the previous step and
-// the admissions), then the step.  Sizes come from the control block.
+// the admissions; the order within a cell is left to the atomics, since no
+// droplet's arithmetic depends on it), then the step.  Sizes come from the
+// control block.
 void serve_enqueue_step(int64_t size, int cur) {
   const int64_t nb = (ctx.ncell + kScanTile - 1) / kScanTile;
   k_scan_reduce<<<(unsigned)nb, kScanThreads, 0, srv.stream>>>(srv.counts, ctx.ncell, srv.bsum);
@@ -259,7 +261,6 @@ void serve_enqueue_step(int64_t size, int cur) {
   k_scan_down<<<(unsigned)nb, kScanThreads, 0, srv.stream>>>(srv.counts, ctx.ncell, srv.bsum, srv.offsets);
   k_bin_scatter<<<grid8(size), kBlock, 0, srv.stream>>>(srv.keys[cur], size, srv.offsets, srv.counts, srv.perm,
                                                         &srv.ctl_dev->hw);
-  k_cell_sort<<<grid_flat(ctx.ncell), kBlock, 0, srv.stream>>>(srv.offsets, ctx.ncell, srv.perm, nullptr);
   const StepArgs a = serve_step_args(cur);
   if (srv.prec == PREC_FP64) serve_launch_step<PREC_FP64>(a, grid8(size));
   else if (srv.prec == PREC_MIXED) serve_launch_step<PREC_MIXED>(a, grid8(size));

@@ -260,6 +260,27 @@ def test_fused_steps_match_single_steps(mode):
     s.close()
 
 
+def test_fused_steps_across_reorderings_match_oracle():
+    """20 fused FP64 BE steps: in-place steps between the periodic
+    reorderings of the slots into cell order, and the reorderings themselves,
+    leave the same droplets as 20 oracle steps."""
+    grid, consts, flags, fields, p, ora = small_case(seed=16, n=4000, kind="stretched")
+    s = make_step(grid, consts, flags, fields, p)
+    s.step(mode=MODE_BE, precision=PREC_FP64, nsteps=12, it0=2)
+    s.step(mode=MODE_BE, precision=PREC_FP64, nsteps=8, it0=14)
+    got = s.get()
+    want = p.copy()
+    for it in range(2, 22):
+        ora.update_be(want.soa, it)
+    tol = dict(TOL64)
+    tol.update({"x": (1e-12, 1.0), "y": (1e-12, 1.0), "z": (1e-12, 1.0), "vx": (1e-9, 0.1), "vy": (1e-9, 0.1),
+                "vz": (1e-9, 0.1), "Tp": (1e-10, 300.0), "radius": (1e-8, 1e-6)})
+    bad = compare(got, want, ["x", "y", "z", "vx", "vy", "vz", "Tp", "radius", "res", "numact"], tol)
+    assert not bad, bad
+    assert np.array_equal(got.mult, p.mult) and np.array_equal(got.pidx, p.pidx)
+    s.close()
+
+
 @pytest.mark.parametrize("kind", ["uniform", "stretched"])
 def test_fused_mixed_steps_match_single_mixed_steps(kind):
     """Dense droplets (about 7 per cell), mixed precision: three fused steps
