@@ -686,7 +686,11 @@ struct TileArgs {
 
 template <int TI, int TJ, int TK>
 __global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
-  constexpr int kNodes = TI * TJ * TK;
+  // node (li, lj, lk) of the box at (li*TJ + lj)*TKP + lk: the odd column
+  // pitch puts the 4 node columns a droplet touches at one level in 4
+  // different LDS banks (8-byte words, 16 per bank row)
+  constexpr int TKP = TK + 1 - (TK & 1);
+  constexpr int kNodes = TI * TJ * TKP;
   constexpr int kMaxSeg = (TI + 2) * (TJ + 2);
   __shared__ double s_acc[6 * kNodes];
   __shared__ double s_zz[kMaxLevels];
@@ -778,7 +782,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
       const int li = ci ? lx[1] : lx[0], lj = cj ? ly[1] : ly[0], lk = ck ? lz[1] : lz[0];
       if (li < 0 || lj < 0 || lk < 0) continue;
       const double w = (ci ? wtx[1] : wtx[0]) * (cj ? wty[1] : wty[0]) * (ck ? wtz[1] : wtz[0]) / dV;
-      const int l = (li * TJ + lj) * TK + lk;
+      const int l = (li * TJ + lj) * TKP + lk;
 #pragma unroll
       for (int f = 0; f < 6; ++f)
         if ((a.fmask >> f) & 1u) atomicAdd(&s_acc[f * kNodes + l], am[f] * w);
@@ -791,7 +795,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
   for (int l = threadIdx.x; l < ni * nj * nk; l += blockDim.x) {
     const int lk = l % nk, lj = (l / nk) % nj, li = l / (nk * nj);
     const int64_t node = (int64_t)(K0 + lk) + sj * (Y0 + lj - 1) + si * (X0 + li - 1);
-    const int ll = (li * TJ + lj) * TK + lk;
+    const int ll = (li * TJ + lj) * TKP + lk;
 #pragma unroll
     for (int f = 0; f < 6; ++f) a.src[(int64_t)f * a.nsrc + node] = s_acc[f * kNodes + ll];
   }
