@@ -33,30 +33,34 @@ def log(*a):
 
 
 def kernel_bytes(name: str, n: int, grid, precision: int) -> float | None:
-    """Algorithmic HBM bytes of one launch of a step kernel over n droplets
-    (a step that is not the last of a call, so no cold-column write-back).
+    """Algorithmic HBM bytes of one launch of a fused-step kernel over n
+    droplets (each byte the kernel must move once).
 
-    be_advance: reads perm (4), the 14 FP64 hot columns, mult (8), home (4)
-      and the previous u-cell key (4) of each droplet; writes the 14 hot
-      columns, mult and home in slot order plus the hand-over to be_solve
-      (8 FP64 + the u level, 68); reads once the gathered fields over the ext
-      array (u, v, w in FP32 for mixed precision, else FP64; T, q FP64).
-    be_solve: reads the 14 hot columns, mult and the hand-over; writes 7 hot
-      columns, the 6 FP64 coupling amounts and the new u-cell key.
-    scatter: reads x, y, z, the key and the 6 amounts of each droplet; reads
-      and writes each of the 6 FP64 source fields once.
+    step_be (in place): reads the 14 FP64 hot columns, mult (8) and the u-cell
+      key (4); writes the 11 columns that change, the 6 FP64 coupling amounts
+      and the new key; reads once the gathered fields over the ext array (u, v,
+      w in FP32 for mixed precision, else FP64; T, q FP64).
+    step_be_sorted (the step that reorders into cell order): reads also the
+      sort permutation (4) and home (4), writes all 14 hot columns, mult and
+      home into the other buffer.
+    *_last: the last step of a call also writes the 11 BE diagnostics (the
+      gathered fluid state and the pre-step values), 88 bytes per droplet.
+    scatter: reads the sort permutation, the key, x, y, z and the 6 amounts of
+      each droplet; writes each of the 6 FP64 source fields once.
     """
     ext = (grid.nnz + 2) * (grid.nny + 5) * (grid.nnx + 5)
     nodes = (grid.nnz + 2) * (grid.nny + 1) * (grid.nnx + 1)
     per_cell = (3 * 4 + 2 * 8) if precision == 1 else 5 * 8
-    if name == "be_advance":
-        return float(n) * ((4 + 14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4 + 8 * 8 + 4)) + float(ext) * per_cell
-    if name == "be_solve":
-        return float(n) * ((14 * 8 + 8 + 8 * 8 + 4) + (7 * 8 + 6 * 8 + 4))
+    if name.startswith("step_be"):
+        if "_sorted" in name:
+            per = (4 + 14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4 + 6 * 8 + 4)
+        else:
+            per = (14 * 8 + 8 + 4) + (11 * 8 + 6 * 8 + 4)
+        if name.endswith("_last"):
+            per += 11 * 8
+        return float(n) * per + float(ext) * per_cell
     if name == "scatter":
-        return float(n) * (3 * 8 + 4 + 6 * 8) + 2.0 * 6 * 8 * nodes
-    if name == "step_be":
-        return float(n) * ((4 + 14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4 + 6 * 8 + 4)) + float(ext) * per_cell
+        return float(n) * (4 + 4 + 3 * 8 + 6 * 8) + 6 * 8 * float(nodes)
     return None
 
 
@@ -152,10 +156,21 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
     `n_templates` droplet ensembles generated up front in pinned memory, each
     with its own seed, radius median drawn from 2-20 um and aerosol median
     from 0.02-0.2 um (generating every request in Python at 200 req/s would
-    measure numpy, not the server).  Results land in a pinned FIFO ring.
-    Reports p50/p99 latency (submission to result in host memory), then the
-    throughput of a saturated stream against the plain batched step on the
-    pool size the saturated stream kept."""
+    measure numpy, not the server).  Results land in a pinned FIFO ring of
+    3 x 4M rows.  One loop serves three phases:
+
+      stream     Poisson arrivals; p50/p99 latency from submission to result
+                 in host memory;
+      saturated  a second set of requests all arriving at t = 0 (a closed
+                 loop bounded by the result ring): sustained throughput;
+      static     the same requests in static batches (each filling the
+                 result ring, submitted together, all waited for before the
+                 next): the batched throughput the continuous loop is
+                 compared with.
+
+    The device-resident batched step (no host transfers) on the saturated
+    pool size is reported beside them: results of 120 bytes per droplet over
+    the host link bound any served throughput well below it."""
     import queue
     import threading
 
@@ -170,7 +185,6 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
     t = time.time()
     bank = PinnedArray((n_templates, 6, nmax))
     bmult = PinnedArray((n_templates, nmax), np.int64)
-    meta = []
     for k in range(n_templates):
         rng = np.random.default_rng(1000 * seed + k)
         r_med, a_med = rng.uniform(2e-6, 20e-6), rng.uniform(0.02e-6, 0.2e-6)
@@ -182,7 +196,6 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
         b[4] = synth.dry_mass(synth.lognormal(rng, nmax, a_med, 0.5))
         b[5] = 1.2
         bmult.array[k] = 1
-        meta.append((r_med, a_med))
     log(f"[serve] request bank: {n_templates} ensembles x {nmax} droplets in {time.time() - t:.1f}s")
     ring_rows = 3 * nmax
     outbuf = PinnedArray((14, ring_rows))
@@ -196,8 +209,7 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
             with self.cv:
                 while True:
                     start = self.head if self.head + n <= ring_rows else 0
-                    busy = [(a, b) for a, b in self.blocks]
-                    if all(start + n <= a or start >= b for a, b in busy):
+                    if all(start + n <= a or start >= b for a, b in self.blocks):
                         self.blocks.append((start, start + n))
                         self.head = start + n
                         return start
@@ -208,7 +220,15 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
                 self.blocks.pop(0)
                 self.cv.notify_all()
 
-    def run_stream(loop, arrivals, sizes, rng):
+    loop = ServeLoop(s, budget_bytes, steps=steps, precision=PREC_MIXED)
+
+    def submit(ring, i, n, o):
+        k = i % n_templates
+        row = ring.take(n)
+        return loop.submit(bank.array[k][:, o:o + n], bmult.array[k][o:o + n], outbuf.array[:, row:row + n],
+                           outmult.array[row:row + n])
+
+    def run_stream(arrivals, sizes, offs):
         ring = OutRing()
         done_q = queue.Queue()
         lat, qd, comp = [], [], []
@@ -218,8 +238,7 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
                 item = done_q.get()
                 if item is None:
                     return
-                tk, n = item
-                a, b = loop.wait(tk)
+                a, b = loop.wait(item)
                 lat.append(a)
                 qd.append(b)
                 comp.append(time.perf_counter())
@@ -228,46 +247,68 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
         th = threading.Thread(target=waiter)
         th.start()
         t0 = time.perf_counter()
-        for i, (ta, n) in enumerate(zip(arrivals, sizes)):
+        for i, (ta, n, o) in enumerate(zip(arrivals, sizes, offs)):
             now = time.perf_counter() - t0
             if ta > now:
                 time.sleep(ta - now)
-            k = i % n_templates
-            o = int(rng.integers(0, nmax - n + 1))
-            row = ring.take(n)
-            tk = loop.submit(bank.array[k][:, o:o + n], bmult.array[k][o:o + n], outbuf.array[:, row:row + n],
-                             outmult.array[row:row + n])
-            done_q.put((tk, n))
+            done_q.put(submit(ring, i, int(n), int(o)))
         done_q.put(None)
         th.join()
         return np.array(lat), np.array(qd), t0, max(comp)
 
+    def run_static(sizes, offs):
+        ring = OutRing()
+        t0 = time.perf_counter()
+        i, lat = 0, []
+        while i < len(sizes):
+            batch, rows = [], 0
+            while i < len(sizes) and rows + sizes[i] <= ring_rows:
+                batch.append(submit(ring, i, int(sizes[i]), int(offs[i])))
+                rows += int(sizes[i])
+                i += 1
+            for tk in batch:
+                lat.append(loop.wait(tk)[0])
+                ring.give()
+        return np.array(lat), t0, time.perf_counter()
+
+    def delta(a, b):
+        return {k: b[k] - a[k] for k in ("iterations", "droplet_steps", "graph_launches")}
+
+    def draw(n):
+        sz = np.exp(rng.uniform(np.log(1e3), np.log(4e6), n)).astype(np.int64)
+        return sz, np.array([rng.integers(0, nmax - m + 1) for m in sz])
+
     rng = np.random.default_rng(seed + 77)
     nreq = max(1, int(rate * duration_s))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    sizes = np.exp(rng.uniform(np.log(1e3), np.log(4e6), nreq)).astype(np.int64)
-    loop = ServeLoop(s, budget_bytes, steps=steps, precision=PREC_MIXED)
-    lat, qd, t0, t1 = run_stream(loop, arrivals, sizes, rng)
-    tot = loop.close()
+    sizes, offs = draw(nreq)
+    st0 = loop.stats()
+    lat, qd, t0, t1 = run_stream(arrivals, sizes, offs)
+    st1 = loop.stats()
+    d = delta(st0, st1)
     work = float(sizes.sum()) * steps
     stream = {"requests": int(nreq), "rate_req_per_s": rate, "duration_s": float(arrivals[-1]),
               "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
               "p50_queue_ms": float(np.percentile(qd, 50)), "p99_queue_ms": float(np.percentile(qd, 99)),
               "throughput_du_s": work / (t1 - t0), "offered_du_s": rate * float(sizes.mean()) * steps,
-              "mean_pool": tot["droplet_steps"] / max(tot["iterations"], 1), "iterations": tot["iterations"],
-              "graphs": tot["graphs"], "capacity": tot["capacity"]}
+              "mean_pool": d["droplet_steps"] / max(d["iterations"], 1), "iterations": d["iterations"]}
     log(f"[serve] stream: p50 {stream['p50_ms']:.1f} ms, p99 {stream['p99_ms']:.1f} ms, "
         f"{stream['throughput_du_s']:.3g} du/s")
 
-    # saturated: arrivals all at t = 0 (a closed loop bounded by the result ring)
     nsat = max(16, nreq // 4)
-    sizes_s = np.exp(rng.uniform(np.log(1e3), np.log(4e6), nsat)).astype(np.int64)
-    loop = ServeLoop(s, budget_bytes, steps=steps, precision=PREC_MIXED)
-    lat_s, _, t0, t1 = run_stream(loop, np.zeros(nsat), sizes_s, rng)
-    tot_s = loop.close()
+    sizes_s, offs_s = draw(nsat)
+    lat_s, _, t0, t1 = run_stream(np.zeros(nsat), sizes_s, offs_s)
+    st2 = loop.stats()
+    d = delta(st1, st2)
     sat_tp = float(sizes_s.sum()) * steps / (t1 - t0)
-    mean_pool = tot_s["droplet_steps"] / max(tot_s["iterations"], 1)
-    # the plain batched step on a pool of that size
+    mean_pool = d["droplet_steps"] / max(d["iterations"], 1)
+    lat_b, t0, t1 = run_static(sizes_s, offs_s)
+    st3 = loop.stats()
+    static_tp = float(sizes_s.sum()) * steps / (t1 - t0)
+    tot = loop.close()
+    log(f"[serve] saturated {sat_tp:.3g} du/s (mean pool {mean_pool:.3g}) vs static batches {static_tp:.3g} du/s")
+
+    # the device-resident batched step on a pool of the saturated size
     nb = int(min(max(mean_pool, 1e4), nmax))
     p = Particles.empty(nb)
     for c, name in enumerate(SERVE_IN):
@@ -280,11 +321,17 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
     tb = time.perf_counter()
     s.step(MODE_BE, PREC_MIXED, nsteps=steps, it0=4)
     s.sync()
-    batched = nb * steps / (time.perf_counter() - tb)
+    device = nb * steps / (time.perf_counter() - tb)
     saturated = {"requests": int(nsat), "throughput_du_s": sat_tp, "mean_pool": mean_pool,
+                 "iterations": d["iterations"],
                  "p50_ms": float(np.percentile(lat_s, 50)), "p99_ms": float(np.percentile(lat_s, 99)),
-                 "batched_du_s": batched, "batched_droplets": nb, "sustained_over_batched": sat_tp / batched}
-    log(f"[serve] saturated {sat_tp:.3g} du/s vs batched {batched:.3g} du/s on {nb} droplets")
+                 "static_batches_du_s": static_tp, "static_p50_ms": float(np.percentile(lat_b, 50)),
+                 "static_p99_ms": float(np.percentile(lat_b, 99)),
+                 "sustained_over_batched": sat_tp / static_tp,
+                 "device_batched_du_s": device, "device_batched_droplets": nb,
+                 "sustained_over_device_batched": sat_tp / device,
+                 "host_link_bound_du_s": steps * 56e9 / (14 * 8 + 8)}
+    log(f"[serve] device-resident batched {device:.3g} du/s on {nb} droplets; engine thread (ms) {tot['engine_ms']}")
     s.close()
     bank.free()
     bmult.free()
@@ -292,7 +339,8 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
     outmult.free()
     return {"workload": "128^3 resident fields, Poisson arrivals, sizes log-uniform 1e3-4e6, "
                         f"{steps} BE steps each, mixed precision, HBM budget {budget_bytes >> 30} GiB",
-            "stream": stream, "saturated": saturated}
+            "stream": stream, "saturated": saturated, "graphs": tot["graphs"], "capacity": tot["capacity"],
+            "newton": tot["newton"]}
 
 
 def main():

@@ -287,6 +287,10 @@ typedef struct ntlp_serve_totals {
   int64_t submitted, completed, queued;
   int64_t iterations, droplet_steps, max_pool, graph_launches, graphs, capacity;
   ntlp_counters newton; /* Newton bookkeeping over every served droplet-step */
+  /* host time of the engine thread (ms): its whole run, inside the copy
+   * calls that start uploads and downloads, and blocked on a download, an
+   * upload or a control block still in use by the device */
+  double ms_engine, ms_copy_calls, ms_wait_download, ms_wait_upload, ms_wait_control;
 } ntlp_serve_totals;
 int ntlp_serve_open(const ntlp_serve_config *cfg, int64_t *capacity);
 /* Queues a request (1 <= n <= capacity, n <= 2^23) and returns at once.
@@ -300,7 +304,9 @@ int ntlp_serve_poll(int64_t ticket);
  * latency from submission to result and the queueing delay before admission.
  * The ticket is released. */
 int ntlp_serve_wait(int64_t ticket, double *times_ms);
-/* Submitted/completed/queued counts so far (the rest is filled at close). */
+/* The loop's totals so far: request counts, and the engine's iteration
+ * counters and host times as of its last iteration (newton is filled only at
+ * close). */
 int ntlp_serve_stats(ntlp_serve_totals *out);
 /* Serves every queued request to the end, stops the loop and releases the
  * pool; final_stats (may be NULL) receives the loop's totals. */

@@ -180,7 +180,9 @@ class _CServeConfig(ctypes.Structure):
 class _CServeTotals(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "completed", "queued", "iterations", "droplet_steps",
                                               "max_pool", "graph_launches", "graphs", "capacity")] + \
-               [("newton", _CCounters)]
+               [("newton", _CCounters)] + \
+               [(n, ctypes.c_double) for n in ("ms_engine", "ms_copy_calls", "ms_wait_download", "ms_wait_upload",
+                                               "ms_wait_control")]
 
 
 _lib_handle = None
@@ -496,21 +498,30 @@ class ServeLoop:
         self._held.pop(ticket, None)
         return float(times[0]), float(times[1])
 
+    @staticmethod
+    def _totals(t: _CServeTotals, newton: bool) -> dict:
+        d = {n: int(getattr(t, n)) for n in ("submitted", "completed", "queued", "iterations", "droplet_steps",
+                                             "max_pool", "graph_launches", "graphs", "capacity")}
+        if newton:
+            c = t.newton
+            d["newton"] = {n: int(getattr(c, n)) for n in
+                           ("num100", "numimpos", "actnum", "denum", "gn_iters", "lm_iters", "solves", "steps")}
+        d["engine_ms"] = {n[3:]: float(getattr(t, n)) for n in
+                          ("ms_engine", "ms_copy_calls", "ms_wait_download", "ms_wait_upload", "ms_wait_control")}
+        return d
+
     def stats(self) -> dict:
+        """Totals so far (the engine's counters as of its last iteration;
+        no Newton counts until close)."""
         t = _CServeTotals()
         self._check(self.lib.ntlp_serve_stats(ctypes.byref(t)))
-        return {"submitted": int(t.submitted), "completed": int(t.completed), "queued": int(t.queued)}
+        return self._totals(t, newton=False)
 
     def close(self) -> dict:
         t = _CServeTotals()
         self._check(self.lib.ntlp_serve_close(ctypes.byref(t)))
         self._held.clear()
-        d = {n: int(getattr(t, n)) for n in ("submitted", "completed", "queued", "iterations", "droplet_steps",
-                                             "max_pool", "graph_launches", "graphs", "capacity")}
-        c = t.newton
-        d["newton"] = {n: int(getattr(c, n)) for n in
-                       ("num100", "numimpos", "actnum", "denum", "gn_iters", "lm_iters", "solves", "steps")}
-        return d
+        return self._totals(t, newton=True)
 
 
 class PinnedArray:

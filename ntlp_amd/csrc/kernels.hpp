@@ -1291,12 +1291,29 @@ __device__ __forceinline__ double crit_radius_of(const DevConsts& C, const DevFl
   const double A = 2 * C.Mw * C.Gam / C.Ru / C.rhow / Tf;
   const double B = kappa * m_s;
   const double D = C.rhos * C.pi2 * 2.0 / 3.0;
+  // A/r - B/(D r^3) rises to its single maximum at r* = sqrt(3B/(D A)) and
+  // falls after it, by about 1% of its value per table step, so the first
+  // maximum over the table lies within a few entries of r*'s position in the
+  // table (radval(i) = 10^(-8 + 5(i-1)/1024)); scanning that window with the
+  // reference's expression and first-maximum rule gives the full scan's
+  // answer.  B = 0: the curve only falls, the answer is the first entry.
+  int i0 = 0, i1 = 1023;
+  if (B == 0.0) {
+    i1 = 9;
+  } else if (B > 0.0 && A > 0.0) {
+    const double fi = (log10(sqrt(3.0 * B / (D * A))) + 8.0) * (1024.0 / 5.0);
+    if (isfinite(fi)) {  // else keep the full scan
+      const int ic = (int)floor(fmin(fmax(fi, -16.0), 1040.0));
+      i0 = min(max(ic - 4, 0), 1014);
+      i1 = max(min(ic + 5, 1023), 9);
+    }
+  }
   double best = 0.0;
-  int ib = 0;
-  for (int i = 0; i < 1024; ++i) {
+  int ib = i0;
+  for (int i = i0; i <= i1; ++i) {
     const double rv = c_radval[i];
     const double ss = A / rv - B / (D * (rv * rv * rv));
-    if (i == 0 || ss > best) {
+    if (i == i0 || ss > best) {
       best = ss;
       ib = i;
     }

@@ -491,11 +491,17 @@ int dispatch_update(const StepArgs& a, int prec) {
   });
 }
 
+// Launch names by variant: reading through the cell sort ("_sorted") or in
+// place, and the last step of a call that also writes the diagnostics
+// ("_last"), since their memory traffic differs.
 template <int MODE>
 int dispatch_step(const StepArgs& a, int prec) {
   const unsigned gr = grid8(a.n);
   const bool lin = ctx.fl.ilin == 1;
-  const char* name = MODE == 1 ? "step_rk3" : "step_be";
+  static const char* const names[2][4] = {
+      {"step_rk3", "step_rk3_last", "step_rk3_sorted", "step_rk3_sorted_last"},
+      {"step_be", "step_be_last", "step_be_sorted", "step_be_sorted_last"}};
+  const char* name = names[MODE == 1 ? 0 : 1][(a.inplace ? 0 : 2) + (a.write_cold ? 1 : 0)];
   return launch(name, [&] {
     if (prec == PREC_FP64) lin ? launch_step_t<MODE, 0, 1>(a, gr) : launch_step_t<MODE, 0, 0>(a, gr);
     else if (prec == PREC_MIXED) lin ? launch_step_t<MODE, 1, 1>(a, gr) : launch_step_t<MODE, 1, 0>(a, gr);

@@ -164,6 +164,17 @@ struct Serve {
   std::string err_msg;
   // statistics
   int64_t submitted = 0, completed = 0, iterations = 0, droplet_steps = 0, max_pool = 0, graph_launches = 0;
+  double ms_engine = 0, ms_copy_calls = 0, ms_wait_download = 0, ms_wait_upload = 0, ms_wait_control = 0;
+  ntlp_serve_totals snap = {};  // the engine's counters as of its last iteration (under mu)
+  Clock::time_point t_engine0;
+};
+
+// Adds the time since construction to a host-time total (engine thread).
+struct HostTimer {
+  double& acc;
+  Clock::time_point t0 = Clock::now();
+  explicit HostTimer(double& a) : acc(a) {}
+  ~HostTimer() { acc += std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
 };
 
 Serve srv;
@@ -305,6 +316,7 @@ int serve_retire(bool block) {
     const hipError_t q = hipEventQuery(p.ev);
     if (q == hipErrorNotReady) {
       if (!block && srv.pending.size() <= 8) break;
+      HostTimer w(srv.ms_wait_download);
       SHIP(hipEventSynchronize(p.ev));
     } else if (q != hipSuccess) {
       return serve_fail(NTLP_E_HIP, std::string("serving download failed: ") + hipGetErrorString(q));
@@ -338,6 +350,7 @@ int serve_retire(bool block) {
 
 // Starts the uploads of newly accepted requests.
 int serve_upload(std::vector<SReq*>& accepted) {
+  HostTimer w(srv.ms_copy_calls);
   for (SReq* r : accepted) {
     SHIP(hipMemcpy2DAsync(srv.in + 6 * r->in_row, sizeof(double) * r->n, r->in, sizeof(double) * r->ld_in,
                           sizeof(double) * r->n, 6, hipMemcpyHostToDevice, srv.up));
@@ -357,6 +370,7 @@ int serve_admit() {
     const hipError_t q = hipEventQuery(r->ev_in);
     if (q == hipErrorNotReady) {
       if (!srv.active.empty()) break;
+      HostTimer w(srv.ms_wait_upload);
       SHIP(hipEventSynchronize(r->ev_in));
     } else if (q != hipSuccess) {
       return serve_fail(NTLP_E_HIP, std::string("serving upload failed: ") + hipGetErrorString(q));
@@ -397,11 +411,31 @@ int serve_admit() {
   return 0;
 }
 
+// Copies the engine thread's counters where ntlp_serve_stats reads them.
+void serve_publish() {
+  std::lock_guard<std::mutex> lk(srv.mu);
+  ntlp_serve_totals& t = srv.snap;
+  t.iterations = srv.iterations;
+  t.droplet_steps = srv.droplet_steps;
+  t.max_pool = srv.max_pool;
+  t.graph_launches = srv.graph_launches;
+  t.graphs = (int64_t)srv.graphs.size();
+  t.capacity = srv.cap;
+  t.ms_engine = std::chrono::duration<double, std::milli>(Clock::now() - srv.t_engine0).count();
+  t.ms_copy_calls = srv.ms_copy_calls;
+  t.ms_wait_download = srv.ms_wait_download;
+  t.ms_wait_upload = srv.ms_wait_upload;
+  t.ms_wait_control = srv.ms_wait_control;
+}
+
 // One step of the pool and the downloads of the requests it finishes.
 int serve_step() {
   const int ci = srv.ctl_i;
   srv.ctl_i = (srv.ctl_i + 1) % kServeCtlRing;
-  SHIP(hipEventSynchronize(srv.ctl_ev[ci]));
+  {
+    HostTimer w(srv.ms_wait_control);
+    SHIP(hipEventSynchronize(srv.ctl_ev[ci]));
+  }
   ServeCtl* c = srv.ctl_host[ci];
   memset(c, 0, sizeof(ServeCtl));
   c->hw = srv.hw;
@@ -426,6 +460,7 @@ int serve_step() {
   SHIP(hipEventRecord(srv.ctl_ev[ci], srv.stream));
   CHK(serve_run_step(srv.hw));
   if (!leaving.empty()) {
+    HostTimer w(srv.ms_copy_calls);
     hipEvent_t stepped, landed;
     SHIP(hipEventCreateWithFlags(&stepped, hipEventDisableTiming));
     SHIP(hipEventRecord(stepped, srv.stream));
@@ -459,6 +494,7 @@ int serve_step() {
   srv.cur = 1 - srv.cur;
   srv.iterations++;
   srv.droplet_steps += n_step;
+  serve_publish();
   return 0;
 }
 
@@ -467,6 +503,8 @@ void serve_loop() {
     serve_fail(NTLP_E_HIP, "hipSetDevice in the serving thread failed");
     return;
   }
+  HostTimer whole(srv.ms_engine);
+  srv.t_engine0 = whole.t0;
   for (;;) {
     std::vector<SReq*> accepted;
     {
@@ -654,13 +692,13 @@ int ntlp_serve_wait(int64_t ticket, double* times_ms) {
 int ntlp_serve_stats(ntlp_serve_totals* out) {
   if (!srv.open) return fail(NTLP_E_STATE, "the serving loop is not open");
   if (!out) return fail(NTLP_E_INVALID, "null output");
-  memset(out, 0, sizeof(*out));
-  {
-    std::lock_guard<std::mutex> lk(srv.mu);
-    out->submitted = srv.submitted;
-    out->completed = srv.completed;
-    out->queued = (int64_t)srv.queue.size();
-  }
+  std::lock_guard<std::mutex> lk(srv.mu);
+  *out = srv.snap;
+  out->submitted = srv.submitted;
+  out->completed = srv.completed;
+  out->queued = (int64_t)srv.queue.size();
+  out->capacity = srv.cap;
+  memset(&out->newton, 0, sizeof(out->newton));
   return NTLP_OK;
 }
 
@@ -684,6 +722,11 @@ int ntlp_serve_close(ntlp_serve_totals* final_stats) {
     final_stats->graph_launches = srv.graph_launches;
     final_stats->graphs = (int64_t)srv.graphs.size();
     final_stats->capacity = srv.cap;
+    final_stats->ms_engine = srv.ms_engine;
+    final_stats->ms_copy_calls = srv.ms_copy_calls;
+    final_stats->ms_wait_download = srv.ms_wait_download;
+    final_stats->ms_wait_upload = srv.ms_wait_upload;
+    final_stats->ms_wait_control = srv.ms_wait_control;
     std::vector<unsigned long long> t((size_t)kTallySlots * kTallyWidth);
     if (hipMemcpy(t.data(), srv.tally, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost) == hipSuccess) {
       int64_t v[kTallyWidth] = {};

@@ -142,8 +142,11 @@ def test_admission_respects_the_hbm_budget():
         s.set_fields(*[np.zeros((grid.nnx, grid.nny, grid.nnz + 2))] * 5)
     reqs = [make_request(grid, 300 + i, 2500, 5e-6, 0.05e-6) for i in range(5)]  # 12500 > 6000
     outs = run_served(loop, reqs)
+    st = loop.stats()
+    assert st["completed"] == 5 and st["queued"] == 0 and st["capacity"] == 6000
+    assert st["iterations"] >= 3 * STEPS and st["droplet_steps"] == STEPS * 12500
     tot = loop.close()
-    assert tot["max_pool"] <= 6000 and tot["completed"] == 5
+    assert tot["max_pool"] <= 6000 and tot["completed"] == 5 and tot["iterations"] == st["iterations"]
     assert tot["iterations"] >= 3 * STEPS  # at most two requests fit at a time
     want = plain_result(s, *reqs[4], PREC_MIXED)
     assert np.array_equal(outs[4][0], want[0])
