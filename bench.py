@@ -68,7 +68,7 @@ def pmc_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE, scaled by the factors a plain copy of
     known size gives), when that summary covers this workload."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    path = os.path.join(ROOT, "profiles", "step_scatter_20M_256x256x128_hbm_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:

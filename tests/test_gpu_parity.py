@@ -107,6 +107,29 @@ def test_crit_radius_bit_exact():
     s.close()
 
 
+def test_crit_radius_bit_exact_across_the_table():
+    """The device searches only the table entries around the analytic maximum
+    of the Koehler curve: the same radius as the full 1024-entry scan for
+    solute from none (m_s = 0) to far past either end of the table, at cold
+    and warm Tp."""
+    grid, consts, flags, fields, p, ora = small_case(seed=14, n=20000)
+    rng = np.random.default_rng(14)
+    q = p.copy()
+    q["m_s"] = 10.0 ** rng.uniform(-32, -6, q.n)
+    q["m_s"][::53] = 0.0
+    q["kappa_s"] = rng.choice([0.0, 0.1, 0.6, 1.2, 1.28], q.n)
+    q["Tp"] = rng.uniform(230.0, 320.0, q.n)
+    q["rc"] = 0.0
+    want = ora.crit_radius(q.soa, COL)
+    s = make_step(grid, consts, flags, fields, q)
+    s.crit_radius()
+    got = s.get()["rc"]
+    assert np.array_equal(got, want), np.flatnonzero(got != want)[:10]
+    # both ends of the table are reached
+    assert want.min() == 1e-8 and want.max() > 9.8e-4
+    s.close()
+
+
 def test_put_get_and_records_roundtrip():
     grid, consts, flags, fields, p, ora = small_case(seed=5, n=777)
     s = make_step(grid, consts, flags, fields, p)
