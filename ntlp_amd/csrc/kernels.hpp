@@ -662,14 +662,19 @@ __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
 
 // Tiled coupling scatter without global atomics.  Each workgroup owns a box
 // of TI x TJ x TK source nodes (node columns ix, iy; levels kz) and keeps its
-// 6 fields in LDS.  It walks the droplets of every u cell whose 8 nodes can
-// touch the box -- cells ix-2..ix-1 along x (plus cell nnx for the periodic
-// image of node 2), likewise along y, and kz-1..kz along z -- through the
-// fresh cell sort (offsets + perm), adds each corner that falls inside the
-// box into LDS, and finally stores the box with plain coalesced writes.
-// Every node belongs to exactly one box, so the fields need no clearing.  A
-// droplet near a box face is read by the neighbouring boxes too: the read
-// amplification is (TI+1)(TJ+1)(TK+1)/(TI TJ TK).
+// 6 fields in LDS.  The droplets that can reach the box are those of the u
+// cells ix-2..ix-1 along x (plus cell nnx for the periodic image of node 2),
+// likewise along y, and kz-1..kz along z; in the fresh cell sort (offsets +
+// perm) the cells kz-1..kz of one (x, y) cell column are one contiguous run,
+// so the box reads (TI+2)(TJ+2) runs at most.  The runs are laid end to end
+// and dealt to the threads kU droplets at a time: a thread first loads the
+// kU slots from perm, then the kU droplets' positions, keys and amounts, and
+// only then adds their in-box corners into LDS, so that its loads overlap
+// instead of forming a chain of dependent misses per droplet.  The box is
+// finally stored with plain coalesced writes.  Every node belongs to exactly
+// one box, so the fields need no clearing.  A droplet near a box face is read
+// by the neighbouring boxes too: the read amplification is
+// (TI+1)(TJ+1)(TK+1)/(TI TJ TK).
 struct TileArgs {
   DevGrid g;
   const double* hot;       // positions (slot order)
@@ -684,19 +689,42 @@ struct TileArgs {
   int ntx, nty, ntz;  // boxes along x, y, z
 };
 
+// Exclusive prefix over the block of one int per thread (kBlock threads);
+// returns the thread's prefix and leaves the block total in *total.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int* total) {
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) s_wave[w] = inc;
+  __syncthreads();
+  int base = 0, sum = 0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; ++k) {
+    base += (k < w) ? s_wave[k] : 0;
+    sum += s_wave[k];
+  }
+  *total = sum;
+  return base + inc - v;
+}
+
 template <int TI, int TJ, int TK>
-__global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
+__global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
   // node (li, lj, lk) of the box at (li*TJ + lj)*TKP + lk: the odd column
   // pitch puts the 4 node columns a droplet touches at one level in 4
-  // different LDS banks (8-byte words, 16 per bank row)
+  // different LDS banks
   constexpr int TKP = TK + 1 - (TK & 1);
   constexpr int kNodes = TI * TJ * TKP;
   constexpr int kMaxSeg = (TI + 2) * (TJ + 2);
+  constexpr int kU = 2;
+  static_assert(kMaxSeg <= kBlock, "one cell column per thread in the run table");
   __shared__ double s_acc[6 * kNodes];
   __shared__ double s_zz[kMaxLevels];
   __shared__ int s_beg[kMaxSeg], s_pre[kMaxSeg + 1];
-  __shared__ int s_cx[TI + 2], s_cy[TJ + 2];
-  __shared__ int s_ncx, s_ncy;
+  __shared__ int s_wave[kBlock / 64];
   const int64_t b = xcd_block();
   if (b >= (int64_t)a.ntx * a.nty * a.ntz) return;  // whole block
   const int kt = (int)(b % a.ntz);
@@ -708,92 +736,115 @@ __global__ void __launch_bounds__(kBlock) k_scatter_tile(TileArgs a) {
   const int K0 = kt * TK, K1 = min(K0 + TK - 1, nnz + 1);
   for (int k = threadIdx.x; k < nnz + 2; k += blockDim.x) s_zz[k] = a.g.zz[k];
   for (int l = threadIdx.x; l < 6 * kNodes; l += blockDim.x) s_acc[l] = 0.0;
-  if (threadIdx.x == 0) {
-    // x cells whose nodes reach [X0, X1]: node ix takes cells ix-1 and ix-2;
-    // node 2 also takes cell nnx through the periodic image nnx+2
-    int n = 0;
-    const int lo = max(0, X0 - 2), hi = min(nnx, X1 - 1);
-    for (int c = lo; c <= hi; ++c) s_cx[n++] = c;
-    if (X0 <= 2 && X1 >= 2 && hi < nnx) s_cx[n++] = nnx;
-    s_ncx = n;
-    n = 0;
-    const int lo_y = max(0, Y0 - 2), hi_y = min(nny, Y1 - 1);
-    for (int c = lo_y; c <= hi_y; ++c) s_cy[n++] = c;
-    if (Y0 <= 2 && Y1 >= 2 && hi_y < nny) s_cy[n++] = nny;
-    s_ncy = n;
-  }
-  __syncthreads();
-  const int ncx = s_ncx, ncy = s_ncy, nseg = ncx * ncy;
+  // x cells whose nodes reach [X0, X1]: node ix takes cells ix-1 and ix-2;
+  // node 2 also takes cell nnx through the periodic image nnx+2
+  const int xlo = max(0, X0 - 2), xhi = min(nnx, X1 - 1);
+  const int ylo = max(0, Y0 - 2), yhi = min(nny, Y1 - 1);
+  const int ncx = xhi - xlo + 1 + ((X0 <= 2 && X1 >= 2 && xhi < nnx) ? 1 : 0);
+  const int ncy = yhi - ylo + 1 + ((Y0 <= 2 && Y1 >= 2 && yhi < nny) ? 1 : 0);
+  const int nseg = ncx * ncy;
   const int klo = max(0, K0 - 1), khi = min(nnz, K1);
   const int nz1 = nnz + 1, ny1 = nny + 1;
-  for (int sg = threadIdx.x; sg < nseg; sg += blockDim.x) {
-    const int cx = s_cx[sg / ncy], cy = s_cy[sg % ncy];
+  int len = 0, beg = 0;
+  if (threadIdx.x < nseg) {
+    const int ci = threadIdx.x / ncy, cj = threadIdx.x % ncy;
+    const int cx = (ci <= xhi - xlo) ? xlo + ci : nnx;
+    const int cy = (cj <= yhi - ylo) ? ylo + cj : nny;
     const int base = nz1 * (cy + ny1 * cx);
-    const int beg = a.offsets[base + klo], end = a.offsets[base + khi + 1];
-    s_beg[sg] = beg;
-    s_pre[sg + 1] = end - beg;
+    beg = a.offsets[base + klo];
+    len = a.offsets[base + khi + 1] - beg;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    s_pre[0] = 0;
-    for (int s = 1; s <= nseg; ++s) s_pre[s] += s_pre[s - 1];
+  int total;
+  const int pre = block_exclusive_scan(len, s_wave, &total);
+  if (threadIdx.x < nseg) {
+    s_beg[threadIdx.x] = beg;
+    s_pre[threadIdx.x] = pre;
   }
+  if (threadIdx.x == 0) s_pre[nseg] = total;
   __syncthreads();
-  const int total = s_pre[nseg];
   const double dx = a.g.dx, dy = a.g.dy;
-  int s = 0;  // segment of p; p only grows
-  for (int p = threadIdx.x; p < total; p += blockDim.x) {
-    while (s + 1 < nseg && s_pre[s + 1] <= p) ++s;
-    const int64_t slot = a.perm[s_beg[s] + (p - s_pre[s])];
-    const int key = a.keys[slot];
-    const double x = a.hot[hix(kHX, slot)], y = a.hot[hix(kHY, slot)], z = a.hot[hix(kHZ, slot)];
-    const int kpt = key % nz1;
-    const int j0 = (key / nz1) % ny1;
-    const int i0 = key / (nz1 * ny1);
-    // the same expression as corner_weight (particles.f90:1197-1205)
-    const double dzu1 = a.g.dzu[kpt + 1];
-    const double dV = dx * dy * dzu1;
-    double wtx[2], wty[2], wtz[2];
-    int lx[2], ly[2], lz[2];
+  const double rdx = 1.0 / dx, rdy = 1.0 / dy;
+  const int rot = lane_id() & 7;
+  for (int p0 = threadIdx.x; p0 < total; p0 += kU * kBlock) {
+    int32_t slot[kU];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      wtx[c] = 1.0 - fabs(x - dx * (double)(c + i0)) / dx;
-      wty[c] = 1.0 - fabs(y - dy * (double)(c + j0)) / dy;
-      wtz[c] = 1.0 - fabs(z - s_zz[kpt + c]) / dzu1;
-      int xi = i0 + 1 + c, yj = j0 + 1 + c;
-      if (xi > nnx + 1) xi -= nnx;
-      if (yj > nny + 1) yj -= nny;
-      lx[c] = (xi >= X0 && xi <= X1) ? xi - X0 : -1;
-      ly[c] = (yj >= Y0 && yj <= Y1) ? yj - Y0 : -1;
-      const int kz = kpt + c;
-      lz[c] = (kz >= K0 && kz <= K1) ? kz - K0 : -1;
+    for (int u = 0; u < kU; ++u) {
+      const int p = p0 + u * kBlock;
+      slot[u] = -1;
+      if (p < total) {
+        // the run of p: s_pre[lo] <= p < s_pre[lo + 1] (runs of length 0
+        // repeat a prefix value and are never chosen)
+        int lo = 0, hi = nseg;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_pre[mid] <= p) lo = mid;
+          else hi = mid;
+        }
+        slot[u] = a.perm[s_beg[lo] + (p - s_pre[lo])];
+      }
     }
-    double am[6];
+    int key[kU];
+    double x[kU], y[kU], z[kU], am[kU][6];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) am[f] = ((a.fmask >> f) & 1u) ? a.amounts[aix(f, slot)] : 0.0;
-    // Neighbouring lanes hold droplets of the same or adjacent cells, whose
-    // corners are the same LDS words; each lane starts at a different corner
-    // so that one atomic instruction rarely hits a word twice.
-    const int rot = lane_id() & 7;
+    for (int u = 0; u < kU; ++u) {
+      if (slot[u] < 0) continue;
+      const int64_t t = slot[u];
+      key[u] = a.keys[t];
+      x[u] = a.hot[hix(kHX, t)];
+      y[u] = a.hot[hix(kHY, t)];
+      z[u] = a.hot[hix(kHZ, t)];
 #pragma unroll
-    for (int s8 = 0; s8 < 8; ++s8) {
-      const int cc = (s8 + rot) & 7;
-      const int ci = cc >> 2, cj = (cc >> 1) & 1, ck = cc & 1;
-      const int li = ci ? lx[1] : lx[0], lj = cj ? ly[1] : ly[0], lk = ck ? lz[1] : lz[0];
-      if (li < 0 || lj < 0 || lk < 0) continue;
-      const double w = (ci ? wtx[1] : wtx[0]) * (cj ? wty[1] : wty[0]) * (ck ? wtz[1] : wtz[0]) / dV;
-      const int l = (li * TJ + lj) * TKP + lk;
+      for (int f = 0; f < 6; ++f) am[u][f] = ((a.fmask >> f) & 1u) ? a.amounts[aix(f, t)] : 0.0;
+    }
 #pragma unroll
-      for (int f = 0; f < 6; ++f)
-        if ((a.fmask >> f) & 1u) atomicAdd(&s_acc[f * kNodes + l], am[f] * w);
+    for (int u = 0; u < kU; ++u) {
+      if (slot[u] < 0) continue;
+      const int kpt = key[u] % nz1;
+      const int j0 = (key[u] / nz1) % ny1;
+      const int i0 = key[u] / (nz1 * ny1);
+      // corner weights of particles.f90:1197-1205 (divisions by dx, dy,
+      // dzu(kpt+1) and dV as products with their reciprocals)
+      const double dzu1 = a.g.dzu[kpt + 1];
+      const double rdz = 1.0 / dzu1;
+      const double rdV = 1.0 / (dx * dy * dzu1);
+      double wtx[2], wty[2], wtz[2];
+      int lx[2], ly[2], lz[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        wtx[c] = 1.0 - fabs(x[u] - dx * (double)(c + i0)) * rdx;
+        wty[c] = 1.0 - fabs(y[u] - dy * (double)(c + j0)) * rdy;
+        wtz[c] = 1.0 - fabs(z[u] - s_zz[kpt + c]) * rdz;
+        int xi = i0 + 1 + c, yj = j0 + 1 + c;
+        if (xi > nnx + 1) xi -= nnx;
+        if (yj > nny + 1) yj -= nny;
+        lx[c] = (xi >= X0 && xi <= X1) ? xi - X0 : -1;
+        ly[c] = (yj >= Y0 && yj <= Y1) ? yj - Y0 : -1;
+        const int kz = kpt + c;
+        lz[c] = (kz >= K0 && kz <= K1) ? kz - K0 : -1;
+      }
+      // Neighbouring lanes hold droplets of the same or adjacent cells, whose
+      // corners are the same LDS words; each lane starts at a different
+      // corner so that one atomic instruction rarely hits a word twice.
+#pragma unroll
+      for (int s8 = 0; s8 < 8; ++s8) {
+        const int cc = (s8 + rot) & 7;
+        const int ci = cc >> 2, cj = (cc >> 1) & 1, ck = cc & 1;
+        const int li = ci ? lx[1] : lx[0], lj = cj ? ly[1] : ly[0], lk = ck ? lz[1] : lz[0];
+        if (li < 0 || lj < 0 || lk < 0) continue;
+        const double w = (ci ? wtx[1] : wtx[0]) * (cj ? wty[1] : wty[0]) * (ck ? wtz[1] : wtz[0]) * rdV;
+        const int l = (li * TJ + lj) * TKP + lk;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+          if ((a.fmask >> f) & 1u) atomicAdd(&s_acc[f * kNodes + l], am[u][f] * w);
+      }
     }
   }
   __syncthreads();
   // store the box: z fastest, so consecutive threads write consecutive levels
-  const int nk = K1 - K0 + 1, nj = Y1 - Y0 + 1, ni = X1 - X0 + 1;
+  const int bk = K1 - K0 + 1, nj = Y1 - Y0 + 1, ni = X1 - X0 + 1;
   const int64_t sj = nnz + 2, si = (int64_t)(nnz + 2) * (nny + 1);
-  for (int l = threadIdx.x; l < ni * nj * nk; l += blockDim.x) {
-    const int lk = l % nk, lj = (l / nk) % nj, li = l / (nk * nj);
+  for (int l = threadIdx.x; l < ni * nj * bk; l += blockDim.x) {
+    const int lk = l % bk, lj = (l / bk) % nj, li = l / (bk * nj);
     const int64_t node = (int64_t)(K0 + lk) + sj * (Y0 + lj - 1) + si * (X0 + li - 1);
     const int ll = (li * TJ + lj) * TKP + lk;
 #pragma unroll

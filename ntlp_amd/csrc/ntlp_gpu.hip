@@ -440,7 +440,7 @@ int tile_sorted() {
   t.ntz = (ctx.g.nnz + 2 + kTileK - 1) / kTileK;
   const int64_t nt = (int64_t)t.ntx * t.nty * t.ntz;
   const unsigned grid = (unsigned)((nt + 7) / 8 * 8);
-  return launch("scatter", [&] { k_scatter_tile<kTileI, kTileJ, kTileK><<<grid, kBlock, 0, ctx.stream>>>(t); });
+  return launch("scatter", [&] { k_scatter_box<kTileI, kTileJ, kTileK><<<grid, kBlock, 0, ctx.stream>>>(t); });
 }
 
 int scatter_tile() {
