@@ -27,6 +27,11 @@ HIPCC_FLAGS = [
     "-fPIC",
     # keep hipcc from forming packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fused only within one source expression, before any loop
+    # hoisting: every kernel that inlines the same droplet arithmetic (the
+    # one-droplet step, the per-box fused step, the serving step) rounds it
+    # identically
+    "-ffp-contract=on",
 ]
 
 

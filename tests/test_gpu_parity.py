@@ -283,6 +283,41 @@ def test_fused_steps_match_single_steps(mode):
     s.close()
 
 
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_MIXED])
+def test_fused_step_sources_match_oracle_deposit(prec):
+    """The sources a fused step leaves (per-droplet amounts written by the step
+    kernel, deposited by the box scatter through the fresh cell sort) equal the
+    oracle's particle_coupling_update on the device's own post-step state (its
+    positions, the gathered Tf and the pre-step velocity, Tp and radius).
+    Droplets placed just short of the 8-cell box faces and of x = xl, moving
+    across them, exercise the boxes' shared faces and the periodic wrap."""
+    grid, consts, flags, fields, p, ora = small_case(seed=21, n=6000)
+    q = p.copy()
+    rng = np.random.default_rng(21)
+    pick = rng.permutation(q.n)
+    k = q.n // 8
+    face_x, face_y = 8 * grid.dx, 8 * grid.dy
+    q["x"][pick[:k]] = face_x - 2e-5
+    q["vx"][pick[:k]] = 0.2
+    q["x"][pick[k:2 * k]] = face_x + 2e-5
+    q["vx"][pick[k:2 * k]] = -0.2
+    q["y"][pick[2 * k:3 * k]] = face_y - 2e-5
+    q["vy"][pick[2 * k:3 * k]] = 0.2
+    q["x"][pick[3 * k:3 * k + 50]] = grid.xl - 1e-5
+    q["vx"][pick[3 * k:3 * k + 50]] = 0.1
+    s = make_step(grid, consts, flags, fields, q)
+    s.step(mode=MODE_BE, precision=prec, nsteps=3, it0=2)
+    src = s.sources()
+    got = s.get()
+    src_o = ora.coupling(got.soa, got.mult)
+    for f in range(6):
+        scale = np.max(np.abs(src_o[f])) + 1e-300
+        # field 5 is -A*Tp + A*Tf: its two terms cancel to 1/3000
+        tol = 1e-12 if f < 5 else 1e-9
+        assert np.max(np.abs(src[f] - src_o[f])) <= tol * scale, (f, np.max(np.abs(src[f] - src_o[f])) / scale)
+    s.close()
+
+
 def test_fused_steps_across_reorderings_match_oracle():
     """20 fused FP64 BE steps: in-place steps between the periodic
     reorderings of the slots into cell order, and the reorderings themselves,
