@@ -35,6 +35,22 @@ struct DevConsts {
   double Mw_Ru;           // Mw/Ru
   double Mw2Gam_Ru_rhow;  // 2 Mw Gam / (Ru rhow)
   double isol;            // 1 / ((2/3) pi2 rhos)
+  // constants of the mixed-precision step and the closed-form solve, formed
+  // once on the host (reciprocals and products that the FP64 reference-order
+  // path does not use)
+  double pi2_23;          // pi2*2.0/3.0 (evaluated left to right)
+  double inuf;            // 1/nuf
+  double idt;             // 1/dt
+  double Lv3_Cpp;         // 3 Lv / Cpp
+  double Cpv_Cpa;         // Cpv / Cpa
+  double hA0;             // 0.5 dt nuf / (Sc rhow)
+  double hB0;             // 1.5 dt (Cpa/Cpp) nuf / (Pra rhow)
+  double Ru_Mw;           // Ru / Mw
+  double irhos;           // 1 / rhos
+  double nuf18;           // 18 nuf
+  double isurf_rho;       // 1 / surf_rho
+  double Mw2Gam_Ru_rhow_um;  // Mw2Gam_Ru_rhow in micrometre units (x 1e6)
+  double isol_um3;        // isol in micrometre units (x 1e18)
 };
 
 struct DevFlags {

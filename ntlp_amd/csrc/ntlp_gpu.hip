@@ -653,6 +653,19 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   C.Mw_Ru = c->Mw / c->Ru;
   C.Mw2Gam_Ru_rhow = 2. * c->Mw * c->Gam / (c->Ru * c->rhow);
   C.isol = 1.0 / ((2.0 / 3.0) * c->pi2 * c->rhos);
+  C.pi2_23 = c->pi2 * 2.0 / 3.0;
+  C.inuf = 1.0 / c->nuf;
+  C.idt = 1.0 / c->dt;
+  C.Lv3_Cpp = 3. * c->Lv / c->Cpp;
+  C.Cpv_Cpa = c->Cpv / c->Cpa;
+  C.hA0 = 0.5 * c->dt * c->nuf / (c->Sc * c->rhow);
+  C.hB0 = 1.5 * c->dt * c->CpaCpp * c->nuf / (c->Pra * c->rhow);
+  C.Ru_Mw = c->Ru / c->Mw;
+  C.irhos = 1.0 / c->rhos;
+  C.nuf18 = 18.0 * c->nuf;
+  C.isurf_rho = 1.0 / c->surf_rho;
+  C.Mw2Gam_Ru_rhow_um = C.Mw2Gam_Ru_rhow * 1e6;
+  C.isol_um3 = C.isol * 1e18;
   DevFlags& fl = ctx.fl;
   fl.ilin = f->ilin;
   fl.icouple = f->icouple;

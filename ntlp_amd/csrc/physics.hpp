@@ -86,6 +86,16 @@ __device__ __forceinline__ double rcp64(double x) {
   return fma(r, e, r);
 }
 
+// FP32 quotient through the hardware reciprocal (a few ulp; the mixed step's
+// FP32 drag, Koehler guess and q* carry FP32 rounding anyway).
+__device__ __forceinline__ float fdiv32(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// mod_magnus with the quotient through rcp64 (closed-form and mixed steps).
+__device__ __forceinline__ double mod_magnus_fast(double T) {
+  const double t = T - 273.15;
+  return 610.94 * exp(17.6257 * t * rcp64(243.04 + t));
+}
+
 // Inputs of ie_vrt_nd that stay fixed during one solve.
 struct NewtonIn {
   double r, Tp, Tf, qinf, m_s, kappa, rhoa, esa, taup0, h, diffnorm;
@@ -338,10 +348,11 @@ __device__ __forceinline__ double rad_solver2(const DevConsts& C, double Tf, dou
 __device__ __forceinline__ double rad_solver2_f32(const DevConsts& C, double Tf, double rhoa, double qinf,
                                                   double kappa, double m_s, double r, double esa, int& mflag) {
   mflag = 0;
-  const double Sm1 = (C.Ru * Tf * rhoa * qinf) / (C.Mw * esa) - 1.0;
+  const double Sm1 = C.Ru_Mw * Tf * rhoa * qinf * rcp64(esa) - 1.0;
   const float L = log1pf((float)Sm1);
-  const float a = -(float)(C.Mw2Gam_Ru_rhow * 1e6 / Tf) / L;
-  const float c = (float)(kappa * m_s * C.isol * 1e18) / L;
+  const float iL = __builtin_amdgcn_rcpf(L);
+  const float a = -(float)(C.Mw2Gam_Ru_rhow_um * rcp64(Tf)) * iL;
+  const float c = (float)(kappa * m_s * C.isol_um3) * iL;
   const float Q = (a * a) * (1.0f / 9.0f);
   const float R = (2.0f * (a * a * a) + 27.0f * c) * (1.0f / 54.0f);
   const float Q3 = Q * Q * Q;
@@ -349,13 +360,13 @@ __device__ __forceinline__ double rad_solver2_f32(const DevConsts& C, double Tf,
   const float pi2 = (float)C.pi2;
   float guess;
   if (M < 0.0f) {
-    const float theta = acosf(R / sqrtf(Q3));
+    const float theta = acosf(R * __builtin_amdgcn_rsqf(Q3));
     const float sQ = sqrtf(Q);
     guess = -(2.0f * sQ * cosf((theta - pi2) * (1.0f / 3.0f))) - a * (1.0f / 3.0f);
     if (guess < 0.0f) guess = -(2.0f * sQ * cosf((theta + pi2) * (1.0f / 3.0f))) - a * (1.0f / 3.0f);
   } else {
     const float S = -copysignf(1.0f, R) * cbrtf(fabsf(R) + sqrtf(M));
-    guess = S + Q / S - a * (1.0f / 3.0f);
+    guess = S + fdiv32(Q, S) - a * (1.0f / 3.0f);
     if (guess < 0.0f) {
       mflag = 1;
       return r;
@@ -381,9 +392,12 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
     double rt_start[2], rt_zeroes[2];
     double guess = (PREC == PREC_MIXED) ? rad_solver2_f32(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag)
                                         : rad_solver2(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
+    // reciprocals of the closed-form solve (the reference-order FP64 solve divides)
+    const double ir = (PREC == PREC_FP64) ? 0.0 : rcp64(d.r);
+    const double iTp = (PREC == PREC_FP64) ? 0.0 : rcp64(d.Tp);
     if (mflag == 0) {
-      rt_start[0] = guess / d.r;
-      rt_start[1] = Tf / d.Tp;
+      rt_start[0] = (PREC == PREC_FP64) ? guess / d.r : guess * ir;
+      rt_start[1] = (PREC == PREC_FP64) ? Tf / d.Tp : Tf * iTp;
     } else {
       rt_start[0] = 1.0;
       rt_start[1] = 1.0;
@@ -426,18 +440,19 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
       P.Tp = d.Tp;
       P.Tf = Tf;
       P.qinf = qinf;
-      P.K = C.Mw_Ru * esa / rhoa;
+      const double irhoa = (fl.iexner == 1) ? rcp64(rhoa) : C.isurf_rho;  // rhoa = surf_rho unless iexner
+      P.K = C.Mw_Ru * esa * irhoa;
       P.csol = d.kappa * d.m_s * C.isol;
-      P.LMRiTf = C.LvMw_Ru / Tf;
-      P.sq = sqrt(2.0 * diffnorm / C.nuf);
-      P.hA = 0.5 * dt * rhoa * C.nuf / (C.Sc * C.rhow * d.r);
-      P.hB = 1.5 * dt * C.CpaCpp * rhoa * C.nuf / (C.Pra * C.rhow * d.Tp);
-      P.Cc = C.Lv3 * d.r / (C.Cpp * d.Tp);
+      P.LMRiTf = C.LvMw_Ru * rcp64(Tf);
+      P.sq = sqrt(diffnorm * (2.0 * C.inuf));
+      P.hA = C.hA0 * rhoa * ir;
+      P.hB = C.hB0 * rhoa * iTp;
+      P.Cc = C.Lv3_Cpp * d.r * iTp;
       P.sc6 = 0.6 * C.Sc13;
       P.pr6 = 0.6 * C.Pra13;
       const AJEval ev{C, P};
       if (PREC == PREC_MIXED) {
-        const double alt[2] = {1.0, Tf / d.Tp};
+        const double alt[2] = {1.0, Tf * iTp};
         double fa[2], fb[2];
         ev.f(rt_start[0], rt_start[1], fa);
         ev.f(alt[0], alt[1], fb);
@@ -499,11 +514,10 @@ __device__ __forceinline__ double qstar_f64(const DevConsts& C, const Drop& d, d
 __device__ __forceinline__ double qstar_f32(const DevConsts& C, const Drop& d, double Tf, double rhoa,
                                             double einf) {
   const float r = (float)d.r, Tp = (float)d.Tp;
-  const float V = (float)(C.pi2 * 2.0 / 3.0) * r * r * r;
-  const float E = (float)(C.Lv * C.Mw / C.Ru) * ((float)(d.Tp - Tf) / ((float)Tf * Tp)) +
-                  (float)((2. * C.Mw * C.Gam) / (C.Ru * C.rhow)) / (r * Tp) -
-                  (float)(d.kappa * d.m_s / C.rhos) / V;
-  return (double)((float)(C.Mw / C.Ru) / (Tp * (float)rhoa) * (float)einf * __expf(E));
+  const float V = (float)C.pi2_23 * r * r * r;
+  const float E = (float)C.LvMw_Ru * fdiv32((float)(d.Tp - Tf), (float)Tf * Tp) +
+                  fdiv32((float)C.Mw2Gam_Ru_rhow, r * Tp) - fdiv32((float)(d.kappa * d.m_s * C.irhos), V);
+  return (double)(fdiv32((float)C.Mw_Ru, Tp * (float)rhoa) * (float)einf * __expf(E));
 }
 
 // ---------------------------------------------------------- BE droplet
@@ -558,17 +572,17 @@ __device__ __forceinline__ BeHand be_advance(const DevGrid& g, const DevConsts& 
   if (PREC == PREC_MIXED) {
     const float rf = (float)r;
     const float a0 = (float)(d.vx - f.u[0]), a1 = (float)(d.vy - f.u[1]), a2 = (float)(d.vz - f.u[2]);
-    const float Volp = (float)(C.pi2 * 2.0 / 3.0) * rf * rf * rf;
-    const float rhopf = ((float)d.m_s + Volp * (float)C.rhow) / Volp;
-    const float tpi = (float)(18.0 * h.rhoa * C.nuf) / rhopf / ((2.0f * rf) * (2.0f * rf));
-    h.Rep = (double)(2.0f * rf * sqrtf(a0 * a0 + a1 * a1 + a2 * a2) / (float)C.nuf);
+    const float Volp = (float)C.pi2_23 * rf * rf * rf;
+    const float rhopf = fdiv32((float)d.m_s + Volp * (float)C.rhow, Volp);
+    const float tpi = fdiv32((float)(h.rhoa * C.nuf18), rhopf * ((2.0f * rf) * (2.0f * rf)));
+    h.Rep = (double)(2.0f * rf * sqrtf(a0 * a0 + a1 * a1 + a2 * a2) * (float)C.inuf);
     d.x = d.x + dt * d.vx;
     d.y = d.y + dt * d.vy;
     d.z = d.z + dt * d.vz;
-    const float dtf = (float)dt, den = 1.0f + dtf * tpi;
-    const float nx = ((float)d.vx + tpi * dtf * (float)f.u[0] + dtf * (float)C.g[0]) / den;
-    const float ny = ((float)d.vy + tpi * dtf * (float)f.u[1] + dtf * (float)C.g[1]) / den;
-    const float nz = ((float)d.vz + tpi * dtf * (float)f.u[2] + dtf * (float)C.g[2]) / den;
+    const float dtf = (float)dt, iden = __builtin_amdgcn_rcpf(1.0f + dtf * tpi);
+    const float nx = ((float)d.vx + tpi * dtf * (float)f.u[0] + dtf * (float)C.g[0]) * iden;
+    const float ny = ((float)d.vy + tpi * dtf * (float)f.u[1] + dtf * (float)C.g[1]) * iden;
+    const float nz = ((float)d.vz + tpi * dtf * (float)f.u[2] + dtf * (float)C.g[2]) * iden;
     d.vx = nx;
     d.vy = ny;
     d.vz = nz;
@@ -606,7 +620,7 @@ __device__ __forceinline__ void be_finish(const DevGrid& g, const DevConsts& C, 
                                           const double* s_z, Drop& d, const BeHand& h, double& qstar,
                                           Tally& tl) {
   const double dt = C.dt;
-  const double esa = mod_magnus(h.Tf);
+  const double esa = (PREC == PREC_FP64) ? mod_magnus(h.Tf) : mod_magnus_fast(h.Tf);
   be_microphysics<PREC>(C, fl, d, h.Tf, h.qinf, h.rhoa, h.diffnorm, h.Rep, esa, tl);
   qstar = (PREC == PREC_MIXED) ? qstar_f32(C, d, h.Tf, h.rhoa, esa) : qstar_f64(C, d, h.Tf, h.rhoa, esa);
   d.res = d.res + dt;
@@ -674,20 +688,19 @@ __device__ __forceinline__ void rk3_droplet(const DevGrid& g, const DevConsts& C
   if (PREC == PREC_MIXED) {
     const float rf = (float)r;
     const float a0 = (float)(d.vx - f.u[0]), a1 = (float)(d.vy - f.u[1]), a2 = (float)(d.vz - f.u[2]);
-    const float Rep = 2.0f * rf * sqrtf(a0 * a0 + a1 * a1 + a2 * a2) / (float)C.nuf;
-    const float Volp = (float)(C.pi2 * 2.0 / 3.0) * rf * rf * rf;
+    const float Rep = 2.0f * rf * sqrtf(a0 * a0 + a1 * a1 + a2 * a2) * (float)C.inuf;
+    const float Volp = (float)C.pi2_23 * rf * rf * rf;
     const float rhop = ((float)d.m_s + Volp * (float)C.rhow) / Volp;
-    const float taup_i = (float)(18.0 * rhoa * C.nuf) / rhop / ((2.0f * rf) * (2.0f * rf));
+    const float taup_i = fdiv32((float)(rhoa * C.nuf18), rhop * ((2.0f * rf) * (2.0f * rf)));
     const float corrfac = 1.0f + 0.15f * __powf(Rep, 0.687f);
     const float sRe = sqrtf(Rep);
     const float Nup = 2.0f + 0.6f * sRe * (float)C.Pra13;
     const float Shp = 2.0f + 0.6f * sRe * (float)C.Sc13;
     const float einf = (float)mod_magnus(Tf);
     const float Tp = (float)d.Tp;
-    const float E = (float)(C.Mw * C.Lv / C.Ru) * ((float)(d.Tp - Tf) / ((float)Tf * Tp)) +
-                    (float)(2.0 * C.Mw * C.Gam / (C.Ru * C.rhow)) / (rf * Tp) -
-                    (float)(d.kappa * d.m_s / C.rhos) / Volp;
-    const float qstar = (float)(C.Mw / C.Ru) * einf * __expf(E) / Tp / (float)rhoa;
+    const float E = (float)C.LvMw_Ru * fdiv32((float)(d.Tp - Tf), (float)Tf * Tp) +
+                    fdiv32((float)C.Mw2Gam_Ru_rhow, rf * Tp) - fdiv32((float)(d.kappa * d.m_s * C.irhos), Volp);
+    const float qstar = fdiv32((float)C.Mw_Ru * einf * __expf(E), Tp * (float)rhoa);
     o.qstar = qstar;
     const float ct = corrfac * taup_i;
     o.vrhs[0] = (double)(ct * (float)(f.u[0] - d.vx) + (float)C.g[0]);
@@ -698,7 +711,7 @@ __device__ __forceinline__ void rk3_droplet(const DevGrid& g, const DevConsts& C
                              : 0.0f;
     o.radrhs = radrhs;
     o.Tprhs_s = (double)(-(Nup / 3.0f / (float)C.Pra * (float)C.CpaCpp * rhop / (float)C.rhow * taup_i * (float)(d.Tp - Tf)));
-    o.Tprhs_L = (double)((float)(3.0 * C.Lv / C.Cpp) / rf * radrhs);
+    o.Tprhs_L = (double)(fdiv32((float)C.Lv3_Cpp, rf) * radrhs);
   } else {
     const double a0 = d.vx - f.u[0], a1 = d.vy - f.u[1], a2 = d.vz - f.u[2];
     const double diffnorm = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
@@ -753,17 +766,17 @@ __device__ __forceinline__ void coupling_amounts(const DevConsts& C, const DevFl
                                                  double Tf, const double vold[3], double Tpold,
                                                  double rold, int64_t mult, double a[6]) {
   const double dt = C.dt;
-  const double idt = FAST ? rcp64(dt) : 0.0;
+  const double idt = FAST ? C.idt : 0.0;
   const double r = d.r;
   double vrhs0, vrhs1, vrhs2, radrhs, Tprhs_L, Tprhs_s, rhoa, rhop, taup_i;
-  const double V = C.pi2 * 2.0 / 3.0 * (r * r * r);
+  const double V = C.pi2_23 * (r * r * r);  // C.pi2 * 2.0 / 3.0 * (r * r * r)
   if (FAST) {
     vrhs0 = (d.vx - vold[0]) * idt;
     vrhs1 = (d.vy - vold[1]) * idt;
     vrhs2 = (d.vz - vold[2]) * idt;
     radrhs = (r - rold) * idt;
     const double ir = rcp64(r);
-    Tprhs_L = C.Lv3 / C.Cpp * ir * radrhs;
+    Tprhs_L = C.Lv3_Cpp * ir * radrhs;
     Tprhs_s = (d.Tp - Tpold) * idt - Tprhs_L;
     rhoa = (fl.iexner == 1) ? func_p_base(C, d.z) / C.Rd / Tf : C.surf_rho;
     rhop = d.m_s * rcp64(V) + C.rhow;
@@ -779,8 +792,8 @@ __device__ __forceinline__ void coupling_amounts(const DevConsts& C, const DevFl
     rhop = (d.m_s + V * C.rhow) / V;
     taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
   }
-  const double irhoa = FAST ? rcp64(rhoa) : 1.0 / rhoa;
-  const double partmass = rhop * 2.0 / 3.0 * C.pi2 * (r * r * r);
+  const double irhoa = FAST ? ((fl.iexner == 1) ? rcp64(rhoa) : C.isurf_rho) : 1.0 / rhoa;
+  const double partmass = FAST ? rhop * V : rhop * 2.0 / 3.0 * C.pi2 * (r * r * r);
   const double rm = (double)mult;
   const double vap = C.rhow * irhoa * C.pi2 * 2 * (r * r) * radrhs;
   for (int k = 0; k < 6; ++k) a[k] = 0.0;
@@ -795,7 +808,8 @@ __device__ __forceinline__ void coupling_amounts(const DevConsts& C, const DevFl
   }
   if (fl.iHcouple == 1) {
     a[4] = -vap * rm;
-    a[5] = (-vap * C.Cpv / C.Cpa * d.Tp + vap * C.Cpv / C.Cpa * Tf) * rm;
+    a[5] = FAST ? (-vap * C.Cpv_Cpa * d.Tp + vap * C.Cpv_Cpa * Tf) * rm
+                : (-vap * C.Cpv / C.Cpa * d.Tp + vap * C.Cpv / C.Cpa * Tf) * rm;
   }
 }
 
