@@ -357,19 +357,20 @@ def main():
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--no-copy", action="store_true",
+                    help="skip the device-copy bandwidth measurement")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
+        from ntlp_amd import Consts, DropletStep, Flags
+        from ntlp_amd.grid import Grid
 
-        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": x.numel() * 8, "copy_probe_bytes_written": y.numel() * 8,
-                          "launches": 3}))
+        s = DropletStep(Grid.uniform(8, 8, 8, 0.01, 0.01, 0.01), Consts(), Flags())
+        gbps = s.copy_bandwidth(1.0, 3)  # 1 GiB read + 1 GiB written per launch (k_copy16), 4 launches
+        s.close()
+        print(json.dumps({"copy_probe_bytes_read": 1 << 30, "copy_probe_bytes_written": 1 << 30, "launches": 4,
+                          "GBps": gbps}))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (the droplet step shards by "
@@ -451,6 +452,12 @@ def main():
         ach = traffic_alg / (avg_ms * 1e-3) / 1e9
         roofline.update({"achieved": ach, "frac": ach / HBM_PEAK_GBS})
     roofline["traffic"] = pmc_traffic(dom_name, wl.name)
+    if not args.no_copy:
+        copy_gbps = s.copy_bandwidth(2.0, 10)
+        log(f"[bench] device copy {copy_gbps:.0f} GB/s")
+        roofline["copy_GBps"] = copy_gbps  # achievable: a plain 2 GiB device copy (16-byte accesses) on this GPU
+        if roofline["achieved"]:
+            roofline["frac_of_copy"] = roofline["achieved"] / copy_gbps
     roofline["kernels"] = per_kernel
 
     cpu = None

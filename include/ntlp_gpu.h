@@ -247,6 +247,12 @@ int ntlp_gpu_step_counters(int64_t *rows, int32_t max_rows);
  * entries of name (32 bytes each), launches and total milliseconds. */
 int ntlp_gpu_timing(int32_t enable);
 int ntlp_gpu_kernel_times(char *names, int64_t *launches, double *total_ms, int32_t cap);
+/* Measurement helper with no counterpart in the reference: the achievable HBM
+ * bandwidth of a plain device copy of `bytes` bytes (read + write, 16-byte
+ * vector accesses) on the library's stream, the median of `reps` timed copies,
+ * in GB/s of bytes read plus bytes written.  Allocates and frees its two
+ * buffers. */
+int ntlp_gpu_copy_bandwidth(int64_t bytes, int32_t reps, double *gbps);
 /* Waits for all work of the context. */
 int ntlp_gpu_sync(void);
 

@@ -232,6 +232,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_step_counters": (ctypes.c_int, [vp, i32]),
         "ntlp_gpu_timing": (ctypes.c_int, [i32]),
         "ntlp_gpu_kernel_times": (ctypes.c_int, [vp, vp, vp, i32]),
+        "ntlp_gpu_copy_bandwidth": (ctypes.c_int, [i64, i32, vp]),
         "ntlp_gpu_sync": (ctypes.c_int, []),
         "ntlp_serve_open": (ctypes.c_int, [ctypes.POINTER(_CServeConfig), ctypes.POINTER(i64)]),
         "ntlp_serve_submit": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, vp, ctypes.POINTER(i64)]),
@@ -426,6 +427,13 @@ class DropletStep:
 
     def timing(self, enable: bool) -> None:
         self._check(self.lib.ntlp_gpu_timing(int(enable)))
+
+    def copy_bandwidth(self, gib: float = 2.0, reps: int = 10) -> float:
+        """GB/s (read + write) of a plain device copy of `gib` GiB: the
+        achievable HBM bandwidth the kernels are compared with."""
+        out = np.zeros(1)
+        self._check(self.lib.ntlp_gpu_copy_bandwidth(int(gib * (1 << 30)), int(reps), _ptr(out)))
+        return float(out[0])
 
     def kernel_times(self) -> dict:
         cap = 64

@@ -1279,6 +1279,12 @@ __global__ void __launch_bounds__(kBlock) k_cell_sort(const int32_t* offsets, in
 
 // ------------------------------------------------------------ utilities
 
+// plain device copy, one 16-byte element per thread (bandwidth reference)
+__global__ void __launch_bounds__(kBlock) k_copy16(const double2* __restrict__ a, double2* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 __global__ void k_fill_f64(double* p, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }

@@ -1359,6 +1359,39 @@ int ntlp_gpu_kernel_times(char* names, int64_t* launches, double* total_ms, int3
   return k;
 }
 
+int ntlp_gpu_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps) {
+  CHK(need_init());
+  if (bytes < (1 << 20) || bytes > ((int64_t)1 << 36) || reps < 1 || !gbps)
+    return fail(NTLP_E_INVALID, "copy_bandwidth needs 1 MiB to 64 GiB and reps >= 1");
+  const int64_t n = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  int rc = dalloc(a, n, "copy source");
+  if (!rc) rc = dalloc(b, n, "copy destination");
+  std::vector<float> ms;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(NTLP_E_HIP, "hipEventCreate failed");
+  const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+  for (int r = -1; r < reps && !rc; ++r) {  // r = -1: warm-up
+    if (hipEventRecord(e0, ctx.stream) != hipSuccess) rc = fail(NTLP_E_HIP, "hipEventRecord failed");
+    k_copy16<<<grid, kBlock, 0, ctx.stream>>>(a, b, n);
+    if (!rc && hipGetLastError() != hipSuccess) rc = fail(NTLP_E_HIP, "copy kernel launch failed");
+    if (!rc && hipEventRecord(e1, ctx.stream) != hipSuccess) rc = fail(NTLP_E_HIP, "hipEventRecord failed");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(NTLP_E_HIP, "copy kernel failed");
+    float t = 0.f;
+    if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = fail(NTLP_E_HIP, "hipEventElapsedTime failed");
+    if (!rc && r >= 0) ms.push_back(t);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(a);
+  dfree(b);
+  if (rc) return rc;
+  std::sort(ms.begin(), ms.end());
+  *gbps = 2.0 * 16.0 * (double)n / (ms[ms.size() / 2] * 1e-3) / 1e9;
+  return NTLP_OK;
+}
+
 int ntlp_gpu_sync(void) {
   CHK(need_init());
   HIPCHK(hipStreamSynchronize(ctx.stream));
