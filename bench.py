@@ -37,9 +37,10 @@ def kernel_bytes(name: str, n: int, grid, precision: int) -> float | None:
     droplets (each byte the kernel must move once).
 
     step_be (in place): reads the 14 FP64 hot columns, mult (8) and the u-cell
-      key (4); writes the 11 columns that change, the 6 FP64 coupling amounts
-      and the new key; reads once the gathered fields over the ext array (u, v,
-      w in FP32 for mixed precision, else FP64; T, q FP64).
+      key (4); writes the 10 columns that change every step (numact only when
+      a droplet activates), the 6 FP64 coupling amounts and the new key; reads
+      once the gathered fields over the ext array (u, v, w in FP32 for mixed
+      precision, else FP64; T, q FP64).
     step_be_sorted (the step that reorders into cell order): reads also the
       sort permutation (4) and home (4), writes all 14 hot columns, mult and
       home into the other buffer.
@@ -55,7 +56,7 @@ def kernel_bytes(name: str, n: int, grid, precision: int) -> float | None:
         if "_sorted" in name:
             per = (4 + 14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4 + 6 * 8 + 4)
         else:
-            per = (14 * 8 + 8 + 4) + (11 * 8 + 6 * 8 + 4)
+            per = (14 * 8 + 8 + 4) + (10 * 8 + 6 * 8 + 4)
         if name.endswith("_last"):
             per += 11 * 8
         return float(n) * per + float(ext) * per_cell

@@ -204,7 +204,9 @@ __device__ __forceinline__ void store_drop_rows(double* out, int64_t ld, int64_t
 }
 
 // State that moves with a droplet in the fused step but is constant in it.
-__device__ __forceinline__ void store_dynamic(double* hot, int64_t s, const Drop& d) {
+// numact changes only when the droplet activates: with write_numact false it
+// is left as it is (the in-place step stores it only when it changed).
+__device__ __forceinline__ void store_dynamic(double* hot, int64_t s, const Drop& d, bool write_numact = true) {
   hot[hix(kHX, s)] = d.x;
   hot[hix(kHY, s)] = d.y;
   hot[hix(kHZ, s)] = d.z;
@@ -215,7 +217,7 @@ __device__ __forceinline__ void store_dynamic(double* hot, int64_t s, const Drop
   hot[hix(kHR, s)] = d.r;
   hot[hix(kHRES, s)] = d.res;
   hot[hix(kHACTRES, s)] = d.actres;
-  hot[hix(kHNUMACT, s)] = d.numact;
+  if (write_numact) hot[hix(kHNUMACT, s)] = d.numact;
 }
 
 __device__ __forceinline__ void load_rhs(const double* cold, int64_t cap, int64_t h, Diag& o) {
@@ -306,6 +308,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   if (valid) {
     const int64_t s = (a.perm && !inplace) ? (int64_t)a.perm[t] : t;
     Drop d = load_drop(a.src.hot, s);
+    const double numact0 = d.numact;
     const int64_t mult = a.src.mult[s];
     const int32_t home = inplace ? 0 : a.src.home[s];
     Diag o;
@@ -332,7 +335,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];
       }
-      store_dynamic(a.dst.hot, t, d);
+      store_dynamic(a.dst.hot, t, d, !inplace || d.numact != numact0);
       if (!inplace) {
         a.dst.hot[hix(kHMS, t)] = d.m_s;
         a.dst.hot[hix(kHKAPPA, t)] = d.kappa;

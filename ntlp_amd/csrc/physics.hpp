@@ -375,6 +375,10 @@ __device__ __forceinline__ double rad_solver2_f32(const DevConsts& C, double Tf,
   return (double)guess * 1e-6;
 }
 
+// Squared residual of the unchanged state below which the mixed solve starts
+// there without forming the Koehler guess.
+constexpr double kStartResid2 = 1e-4;
+
 // Radius/temperature block of particle_update_BE (3103-3178) on a droplet
 // whose velocity is already the new one.  Updates d.r, d.Tp, activation
 // bookkeeping and the tally.  PREC_FP64: reference residual and FD
@@ -388,19 +392,17 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
                                                 double Rep, double esa, Tally& tl) {
   const double dt = C.dt;
   if (fl.ievap == 1 && qinf > 0.0) {
-    int mflag, flag, it1 = 0, it2 = 0;
-    double rt_start[2], rt_zeroes[2];
-    double guess = (PREC == PREC_MIXED) ? rad_solver2_f32(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag)
-                                        : rad_solver2(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
+    int mflag = 0, flag, it1 = 0, it2 = 0;
+    double rt_start[2] = {1.0, 1.0}, rt_zeroes[2];
     // reciprocals of the closed-form solve (the reference-order FP64 solve divides)
     const double ir = (PREC == PREC_FP64) ? 0.0 : rcp64(d.r);
     const double iTp = (PREC == PREC_FP64) ? 0.0 : rcp64(d.Tp);
-    if (mflag == 0) {
-      rt_start[0] = (PREC == PREC_FP64) ? guess / d.r : guess * ir;
-      rt_start[1] = (PREC == PREC_FP64) ? Tf / d.Tp : Tf * iTp;
-    } else {
-      rt_start[0] = 1.0;
-      rt_start[1] = 1.0;
+    if (PREC != PREC_MIXED) {  // the reference's Koehler start (mixed: below)
+      const double guess = rad_solver2(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
+      if (mflag == 0) {
+        rt_start[0] = (PREC == PREC_FP64) ? guess / d.r : guess * ir;
+        rt_start[1] = (PREC == PREC_FP64) ? Tf / d.Tp : Tf * iTp;
+      }
     }
     if (PREC == PREC_FP64) {
       const double ri = C.radius_init;
@@ -452,13 +454,24 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
       P.pr6 = 0.6 * C.Pra13;
       const AJEval ev{C, P};
       if (PREC == PREC_MIXED) {
-        const double alt[2] = {1.0, Tf * iTp};
-        double fa[2], fb[2];
-        ev.f(rt_start[0], rt_start[1], fa);
-        ev.f(alt[0], alt[1], fb);
-        if (!(fa[0] * fa[0] + fa[1] * fa[1] <= fb[0] * fb[0] + fb[1] * fb[1])) {
-          rt_start[0] = alt[0];
-          rt_start[1] = alt[1];
+        // Start from the unchanged state (1, Tf/Tp) when its residual is
+        // already small; otherwise from the better of it and the FP32 Koehler
+        // guess (the guess, with its log, acos/cos or cbrt, costs as much as a
+        // residual evaluation and rarely wins over a step of a few 1e-4 s).
+        rt_start[0] = 1.0;
+        rt_start[1] = Tf * iTp;
+        double fb[2];
+        ev.f(rt_start[0], rt_start[1], fb);
+        const double nb = fb[0] * fb[0] + fb[1] * fb[1];
+        if (!(nb < kStartResid2)) {
+          const double guess = rad_solver2_f32(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
+          const double gs[2] = {mflag == 0 ? guess * ir : 1.0, mflag == 0 ? Tf * iTp : 1.0};
+          double fa[2];
+          ev.f(gs[0], gs[1], fa);
+          if (fa[0] * fa[0] + fa[1] * fa[1] <= nb) {
+            rt_start[0] = gs[0];
+            rt_start[1] = gs[1];
+          }
         }
       }
       flag = gauss_newton(ev, rt_start, rt_zeroes, it1);
