@@ -725,7 +725,6 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
   constexpr int kU = 2;
   static_assert(kMaxSeg <= kBlock, "one cell column per thread in the run table");
   __shared__ double s_acc[6 * kNodes];
-  __shared__ double s_zz[kMaxLevels];
   __shared__ int s_beg[kMaxSeg], s_pre[kMaxSeg + 1];
   __shared__ int s_wave[kBlock / 64];
   const int64_t b = xcd_block();
@@ -737,7 +736,6 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
   const int X0 = 1 + it * TI, X1 = min(X0 + TI - 1, nnx + 1);
   const int Y0 = 1 + jt * TJ, Y1 = min(Y0 + TJ - 1, nny + 1);
   const int K0 = kt * TK, K1 = min(K0 + TK - 1, nnz + 1);
-  for (int k = threadIdx.x; k < nnz + 2; k += blockDim.x) s_zz[k] = a.g.zz[k];
   for (int l = threadIdx.x; l < 6 * kNodes; l += blockDim.x) s_acc[l] = 0.0;
   // x cells whose nodes reach [X0, X1]: node ix takes cells ix-1 and ix-2;
   // node 2 also takes cell nnx through the periodic image nnx+2
@@ -816,7 +814,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
       for (int c = 0; c < 2; ++c) {
         wtx[c] = 1.0 - fabs(x[u] - dx * (double)(c + i0)) * rdx;
         wty[c] = 1.0 - fabs(y[u] - dy * (double)(c + j0)) * rdy;
-        wtz[c] = 1.0 - fabs(z[u] - s_zz[kpt + c]) * rdz;
+        wtz[c] = 1.0 - fabs(z[u] - a.g.zz[kpt + c]) * rdz;  // two levels, cached
         int xi = i0 + 1 + c, yj = j0 + 1 + c;
         if (xi > nnx + 1) xi -= nnx;
         if (yj > nny + 1) yj -= nny;
