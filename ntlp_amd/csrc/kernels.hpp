@@ -120,8 +120,11 @@ __device__ __forceinline__ int wave_sum_i32(int v) {
   return v;
 }
 
-// s_red: LDS scratch of kTallyWidth ints, zeroed (with the histogram part)
-// before the kernel's first __syncthreads.
+// s_red: LDS scratch of kTallyWidth + 1 ints (the sums, then an arrival
+// counter), zeroed before the kernel's first __syncthreads.  Each wave adds
+// its counts into LDS; the last wave of the workgroup to arrive adds the
+// workgroup's sums to its slot.  There is no workgroup barrier, so a wave
+// whose droplets converged quickly exits without waiting for the others.
 __device__ __forceinline__ void flush_tally(const Tally& tl, unsigned long long* slots, int* s_red) {
   const int c_n100 = __popcll(__ballot(tl.num100 != 0));
   const int c_imp = __popcll(__ballot(tl.numimpos != 0));
@@ -137,13 +140,21 @@ __device__ __forceinline__ void flush_tally(const Tally& tl, unsigned long long*
       if (v[k]) atomicAdd(&s_red[k], v[k]);
   }
   if (tl.hist >= 0) atomicAdd(&s_red[kTHist0 + (tl.hist < 63 ? tl.hist : 63)], 1);
-  __syncthreads();
-  if (threadIdx.x < kTallyWidth && s_red[threadIdx.x] != 0)
-    atomicAdd(&slots[(blockIdx.x % kTallySlots) * kTallyWidth + threadIdx.x], (unsigned long long)s_red[threadIdx.x]);
+  __threadfence_block();  // this wave's adds precede its arrival
+  int last = 0;
+  if (lane_id() == 0) last = atomicAdd(&s_red[kTallyWidth], 1) == (int)(blockDim.x >> 6) - 1;
+  last = __shfl(last, 0, 64);
+  if (last) {
+    __threadfence_block();
+    for (int c = lane_id(); c < kTallyWidth; c += 64) {
+      const int v = atomicAdd(&s_red[c], 0);
+      if (v) atomicAdd(&slots[(blockIdx.x % kTallySlots) * kTallyWidth + c], (unsigned long long)v);
+    }
+  }
 }
 
 __device__ __forceinline__ void zero_tally_lds(int* s_red) {
-  for (int i = threadIdx.x; i < kTallyWidth; i += blockDim.x) s_red[i] = 0;
+  for (int i = threadIdx.x; i <= kTallyWidth; i += blockDim.x) s_red[i] = 0;
 }
 
 // Folds the slots into the step's row (and counts the step), clearing them.
@@ -296,7 +307,7 @@ __device__ __forceinline__ void count_key(int* counts, int key, bool valid) {
 template <int MODE, int PREC, int ILIN, bool SERVE = false>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
-  __shared__ int s_red[kTallyWidth];
+  __shared__ int s_red[kTallyWidth + 1];
   stage_grid(a.g, s_z, s_zz);
   zero_tally_lds(s_red);
   __syncthreads();
@@ -364,7 +375,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
 template <int MODE, int PREC, int ILIN>
 __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
-  __shared__ int s_red[kTallyWidth];
+  __shared__ int s_red[kTallyWidth + 1];
   stage_grid(a.g, s_z, s_zz);
   zero_tally_lds(s_red);
   __syncthreads();
@@ -672,8 +683,9 @@ __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
 // so the box reads (TI+2)(TJ+2) runs at most.  The runs are laid end to end
 // and dealt to the threads kU droplets at a time: a thread first loads the
 // kU slots from perm, then the kU droplets' positions, keys and amounts, and
-// only then adds their in-box corners into LDS, so that its loads overlap
-// instead of forming a chain of dependent misses per droplet.  The box is
+// only then adds their in-box corners into LDS.  The kernel is bound by the
+// latency of those loads and by the FP64 LDS atomics, so it is kept small (67
+// VGPRs, 25 KB of LDS) to keep six workgroups resident per CU.  The box is
 // finally stored with plain coalesced writes.  Every node belongs to exactly
 // one box, so the fields need no clearing.  A droplet near a box face is read
 // by the neighbouring boxes too: the read amplification is
@@ -716,13 +728,13 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int* tot
 
 template <int TI, int TJ, int TK>
 __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
-  // node (li, lj, lk) of the box at (li*TJ + lj)*TKP + lk: the odd column
-  // pitch puts the 4 node columns a droplet touches at one level in 4
-  // different LDS banks
-  constexpr int TKP = TK + 1 - (TK & 1);
+  // node (li, lj, lk) of the box at (li*TJ + lj)*TKP + lk (25 KB of LDS for
+  // 8x8x8: six blocks fit a CU; an odd pitch, against bank conflicts, made
+  // it 28 KB and five blocks, which measured slower)
+  constexpr int TKP = TK;
   constexpr int kNodes = TI * TJ * TKP;
   constexpr int kMaxSeg = (TI + 2) * (TJ + 2);
-  constexpr int kU = 2;
+  constexpr int kU = 1;  // droplets per thread per pass (1 to 8 measured; 1 fastest)
   static_assert(kMaxSeg <= kBlock, "one cell column per thread in the run table");
   __shared__ double s_acc[6 * kNodes];
   __shared__ int s_beg[kMaxSeg], s_pre[kMaxSeg + 1];
