@@ -66,6 +66,11 @@ struct ServeCtl {
 struct StepArgs {
   DevGrid g;
   DevConsts C;
+  // the same constants in device memory: the droplet kernels read them
+  // through this pointer where they are used, so that the ~100 dwords of
+  // DevConsts are not held in scalar registers across the Newton solve (they
+  // spilled to vector-register lanes and were reloaded every iteration)
+  const DevConsts* Cp;
   DevFlags fl;
   DevFields F;
   int64_t n, cap;
@@ -306,6 +311,7 @@ __device__ __forceinline__ void count_key(int* counts, int key, bool valid) {
 // no coupling amounts are formed.
 template <int MODE, int PREC, int ILIN, bool SERVE = false>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+  const DevConsts& C = *a.Cp;
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
   __shared__ int s_red[kTallyWidth + 1];
   stage_grid(a.g, s_z, s_zz);
@@ -330,9 +336,9 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
       o.Tprhs_s = o.Tprhs_L = o.radrhs = 0.0;
       o.kpt = kg;
       for (int st = 1; st <= 3; ++st)
-        rk3_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, st, a.it, o, o.kpt);
+        rk3_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, st, a.it, o, o.kpt);
     } else {
-      be_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg);
+      be_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg);
     }
     if (SERVE && a.serve->last[home >> kServeIdxBits]) {
       const int64_t row = a.serve->out_off[home >> kServeIdxBits] + (home & kServeIdxMask);
@@ -342,7 +348,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     } else {
       if (!SERVE) {
         double am[6];
-        coupling_amounts<PREC == PREC_MIXED>(a.C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
+        coupling_amounts<PREC == PREC_MIXED>(C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
 #pragma unroll
         for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];
       }
@@ -374,6 +380,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
 // buffers (no reordering), reading and writing the record columns.
 template <int MODE, int PREC, int ILIN>
 __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
+  const DevConsts& C = *a.Cp;
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
   __shared__ int s_red[kTallyWidth + 1];
   stage_grid(a.g, s_z, s_zz);
@@ -387,11 +394,11 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
     Diag o;
     if (MODE == 1) {
       load_rhs(a.cold, a.cap, home, o);
-      rk3_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.istage, a.it, o);
+      rk3_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.istage, a.it, o);
       store_rhs(a.cold, a.cap, home, o);
       if (a.istage == 1) store_old(a.cold, a.cap, home, o);
     } else {
-      be_droplet<PREC, ILIN>(a.g, a.C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl);
+      be_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl);
       store_old(a.cold, a.cap, home, o);
     }
     store_gathered(a.cold, a.cap, home, o);

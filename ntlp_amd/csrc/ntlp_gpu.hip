@@ -53,6 +53,7 @@ struct Ctx {
   DevConsts C{};
   DevFlags fl{};
   double* d_grid = nullptr;  // z | zz | dzw | dzu
+  DevConsts* d_consts = nullptr;  // device copy of C
   // fields
   double* ext[5] = {};
   float* f32[3] = {};
@@ -264,6 +265,7 @@ void release_all() {
   dfree(ctx.tally);
   dfree(ctx.tally_slots);
   dfree(ctx.d_grid);
+  dfree(ctx.d_consts);
   for (auto& t : ctx.pending) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -294,6 +296,7 @@ StepArgs base_args() {
   memset(&a, 0, sizeof(a));
   a.g = ctx.g;
   a.C = ctx.C;
+  a.Cp = ctx.d_consts;
   a.fl = ctx.fl;
   a.F = fields_view();
   a.n = ctx.n;
@@ -309,6 +312,40 @@ StepArgs base_args() {
   a.counts = ctx.counts;
   a.tally = ctx.tally_slots;
   return a;
+}
+
+// Constant subexpressions of the step, formed once from the namelist values
+// (in the reference's order where the FP64 path uses them).
+void derive_consts(DevConsts& C) {
+  C.Sc13 = std::pow(C.Sc, 1. / 3.);
+  C.Pra13 = std::pow(C.Pra, 1. / 3.);
+  C.LvMw_Ru = C.Lv * C.Mw / C.Ru;
+  C.Mw2Gam = 2. * C.Mw * C.Gam;
+  C.Ru_rhow = C.Ru * C.rhow;
+  C.Lv3 = 3. * C.Lv;
+  C.Mw_Ru = C.Mw / C.Ru;
+  C.Mw2Gam_Ru_rhow = 2. * C.Mw * C.Gam / (C.Ru * C.rhow);
+  C.isol = 1.0 / ((2.0 / 3.0) * C.pi2 * C.rhos);
+  C.pi2_23 = C.pi2 * 2.0 / 3.0;
+  C.inuf = 1.0 / C.nuf;
+  C.idt = 1.0 / C.dt;
+  C.Lv3_Cpp = 3. * C.Lv / C.Cpp;
+  C.Cpv_Cpa = C.Cpv / C.Cpa;
+  C.hA0 = 0.5 * C.dt * C.nuf / (C.Sc * C.rhow);
+  C.hB0 = 1.5 * C.dt * C.CpaCpp * C.nuf / (C.Pra * C.rhow);
+  C.Ru_Mw = C.Ru / C.Mw;
+  C.irhos = 1.0 / C.rhos;
+  C.nuf18 = 18.0 * C.nuf;
+  C.isurf_rho = 1.0 / C.surf_rho;
+  C.Mw2Gam_Ru_rhow_um = C.Mw2Gam_Ru_rhow * 1e6;
+  C.isol_um3 = C.isol * 1e18;
+}
+
+// the device copy of ctx.C, in stream order with the kernels that read it
+int upload_consts() {
+  HIPCHK(hipMemcpyAsync(ctx.d_consts, &ctx.C, sizeof(DevConsts), hipMemcpyHostToDevice, ctx.stream));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return 0;
 }
 
 // exclusive scan counts -> offsets (ncell + 1 entries)
@@ -644,28 +681,8 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
     C.zetas[d] = c->zetas[d];
     C.gama[d] = c->gama[d];
   }
-  C.Sc13 = std::pow(c->Sc, 1. / 3.);
-  C.Pra13 = std::pow(c->Pra, 1. / 3.);
-  C.LvMw_Ru = c->Lv * c->Mw / c->Ru;
-  C.Mw2Gam = 2. * c->Mw * c->Gam;
-  C.Ru_rhow = c->Ru * c->rhow;
-  C.Lv3 = 3. * c->Lv;
-  C.Mw_Ru = c->Mw / c->Ru;
-  C.Mw2Gam_Ru_rhow = 2. * c->Mw * c->Gam / (c->Ru * c->rhow);
-  C.isol = 1.0 / ((2.0 / 3.0) * c->pi2 * c->rhos);
-  C.pi2_23 = c->pi2 * 2.0 / 3.0;
-  C.inuf = 1.0 / c->nuf;
-  C.idt = 1.0 / c->dt;
-  C.Lv3_Cpp = 3. * c->Lv / c->Cpp;
-  C.Cpv_Cpa = c->Cpv / c->Cpa;
-  C.hA0 = 0.5 * c->dt * c->nuf / (c->Sc * c->rhow);
-  C.hB0 = 1.5 * c->dt * c->CpaCpp * c->nuf / (c->Pra * c->rhow);
-  C.Ru_Mw = c->Ru / c->Mw;
-  C.irhos = 1.0 / c->rhos;
-  C.nuf18 = 18.0 * c->nuf;
-  C.isurf_rho = 1.0 / c->surf_rho;
-  C.Mw2Gam_Ru_rhow_um = C.Mw2Gam_Ru_rhow * 1e6;
-  C.isol_um3 = C.isol * 1e18;
+  derive_consts(C);
+  CHK(dalloc(ctx.d_consts, 1, "constants"));
   DevFlags& fl = ctx.fl;
   fl.ilin = f->ilin;
   fl.icouple = f->icouple;
@@ -685,6 +702,7 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   CHK(dalloc(ctx.tally_slots, kTallySlots * kTallyWidth, "counter slots"));
   ctx.ext_elems = (int64_t)(nz + 2) * (grid->nny + 5) * (grid->nnx + 5);
   ctx.init = true;
+  CHK(upload_consts());
   CHK(fill_i32(ctx.counts, ctx.ncell, 0));
   CHK(fill_f64(ctx.src, 6 * ctx.nsrc, 0.0));
   CHK(launch("fill", [&] { k_fill_u64<<<64, kBlock, 0, ctx.stream>>>(ctx.tally, kTallyRows * kTallyWidth); }));
@@ -702,9 +720,11 @@ int ntlp_gpu_finalize(void) {
 
 int ntlp_gpu_set_dt(double dt) {
   CHK(need_init());
+  if (ctx.serving) return fail(NTLP_E_STATE, "the serving loop steps with the current dt: close it first");
   if (!(dt > 0)) return fail(NTLP_E_INVALID, "dt must be positive");
   ctx.C.dt = dt;
-  return NTLP_OK;
+  derive_consts(ctx.C);
+  return upload_consts();
 }
 
 int ntlp_gpu_set_fields(const double* u, const double* v, const double* w, const double* T, const double* q,

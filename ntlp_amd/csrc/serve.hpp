@@ -235,6 +235,7 @@ StepArgs serve_step_args(int cur) {
   memset(&a, 0, sizeof(a));
   a.g = ctx.g;
   a.C = ctx.C;
+  a.Cp = ctx.d_consts;
   a.fl = ctx.fl;
   a.F = fields_view();
   a.n = srv.cap;

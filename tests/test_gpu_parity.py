@@ -318,6 +318,31 @@ def test_fused_step_sources_match_oracle_deposit(prec):
     s.close()
 
 
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_MIXED])
+def test_set_dt_matches_a_context_made_with_that_dt(prec):
+    """ntlp_gpu_set_dt (les.F's get_dt changes dt between steps) refreshes every
+    constant derived from dt: two steps after set_dt equal, bit for bit, two
+    steps of a context initialised with that dt (the sources to rounding: the
+    scatter's LDS atomics add in whatever order the hardware runs them)."""
+    import dataclasses
+
+    grid, consts, flags, fields, p, ora = small_case(seed=23, n=3000)
+    a = make_step(grid, consts, flags, fields, p)
+    a.set_dt(2.5e-4)
+    a.step(mode=MODE_BE, precision=prec, nsteps=2, it0=2)
+    got = a.get()
+    src = a.sources()
+    a.close()
+    b = make_step(grid, dataclasses.replace(consts, dt=2.5e-4), flags, fields, p)
+    b.step(mode=MODE_BE, precision=prec, nsteps=2, it0=2)
+    want = b.get()
+    src_b = b.sources()
+    b.close()
+    assert np.array_equal(got.soa, want.soa)
+    for f in range(6):
+        assert np.max(np.abs(src[f] - src_b[f])) <= 1e-12 * (np.max(np.abs(src_b[f])) + 1e-300), f
+
+
 def test_fused_steps_across_reorderings_match_oracle():
     """20 fused FP64 BE steps: in-place steps between the periodic
     reorderings of the slots into cell order, and the reorderings themselves,
