@@ -113,38 +113,55 @@ def run_steps(s, init_state, mode, prec, steps, warmup, it0=2):
     return dt
 
 
-def cpu_baseline(wl_id: int, seed: int, target_s: float = 12.0):
-    """The oracle (single-threaded FP64 C restatement of the reference loop)
-    on a bounded sample of the same workload: same grid and fields, the first
-    droplets of the same ensemble, one BE step + coupling deposit."""
+def cpu_baseline(wl_id: int, seed: int, n_full: int, target_s: float = 8.0):
+    """The oracle (FP64 C restatement of the reference loop: BE step + coupling
+    deposit) on a bounded sample of the same workload -- same grid and fields,
+    the first droplets of the same ensemble -- single-threaded and over all
+    host cores (OpenMP).  Each step also zeroes the full source fields
+    (particles.f90:1167-1170), a per-step cost that a small sample would
+    otherwise spread over few droplets: it is timed apart and the full
+    workload's CPU step is estimated as t_droplets * n_full / n_sample +
+    t_zero."""
     from ntlp_amd import COL, synth
     from oracle import Oracle
 
-    n_probe = 20000
-    wl = synth.workload(wl_id, n=n_probe, seed=seed)
-    ora = Oracle(wl.grid, wl.consts, wl.flags)
-    ora.set_fields(*wl.fields)
-    p = wl.droplets
-    ora.interp(p.soa)
-    for d in "xyz":
-        p["v" + d] = p["uf" + d]
-    p["Tp"] = p["Tf"]
-    p["rc"] = ora.crit_radius(p.soa, COL)
-    t0 = time.perf_counter()
-    ora.update_be(p.soa, 2)
-    ora.coupling(p.soa, p.mult)
-    t_probe = time.perf_counter() - t0
-    reps = max(1, int(target_s / max(t_probe, 1e-3)))
-    reps = min(reps, 200)
-    t0 = time.perf_counter()
-    for r in range(reps):
-        ora.update_be(p.soa, 3 + r)
-        ora.coupling(p.soa, p.mult)
-    t = time.perf_counter() - t0
-    rate = n_probe * reps / t
-    return {"value": rate, "unit": "droplet-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (C, FP64, -O2) BE step + coupling deposit on {n_probe} droplets of the same "
-                      f"ensemble and fields, {reps} consecutive steps, {t:.1f}s"}
+    def leg(n_probe: int, par: bool):
+        wl = synth.workload(wl_id, n=n_probe, seed=seed)
+        ora = Oracle(wl.grid, wl.consts, wl.flags)
+        ora.set_fields(*wl.fields)
+        p = wl.droplets
+        ora.interp(p.soa)
+        for d in "xyz":
+            p["v" + d] = p["uf" + d]
+        p["Tp"] = p["Tf"]
+        p["rc"] = ora.crit_radius(p.soa, COL)
+        upd = ora.update_be_par if par else ora.update_be
+        cpl = ora.coupling_par if par else ora.coupling
+        t0 = time.perf_counter()
+        cpl(p.soa[:, :0].copy(), p.mult[:0])  # zeroing alone
+        t_zero = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        upd(p.soa, 2)
+        cpl(p.soa, p.mult)
+        t_probe = time.perf_counter() - t0
+        reps = int(min(max(1, target_s / max(t_probe, 1e-3)), 200))
+        t0 = time.perf_counter()
+        for r in range(reps):
+            upd(p.soa, 3 + r)
+            cpl(p.soa, p.mult)
+        t = time.perf_counter() - t0
+        t_drop = max(t / reps - t_zero, 1e-12)
+        rate = n_full / (t_drop * n_full / n_probe + t_zero)
+        return rate, reps, t, t_zero, ora.max_threads() if par else 1
+
+    r1, reps1, t1, z1, _ = leg(20000, False)
+    rp, repsp, tp, zp, cores = leg(200000, True)
+    return {"value": rp, "unit": "droplet-updates/s", "cores": cores, "kind": "port",
+            "single_thread_value": r1,
+            "sample": f"oracle (C, FP64, -O2): BE step + coupling deposit of the same ensemble and fields; "
+                      f"{cores} OpenMP threads on 200000 droplets x {repsp} steps ({tp:.1f} s) and 1 thread on "
+                      f"20000 droplets x {reps1} steps ({t1:.1f} s); zeroing the sources costs {zp * 1e3:.0f} ms "
+                      f"({z1 * 1e3:.0f} ms on 1 thread) per step and is added once per step of {n_full} droplets"}
 
 
 def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budget_bytes: int = 32 << 30,
@@ -464,7 +481,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(args.workload, args.seed)
+        cpu = cpu_baseline(args.workload, args.seed, n)
         log(f"[bench] cpu baseline {cpu['value']:.3g} droplet-updates/s")
 
     serve = None

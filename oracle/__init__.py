@@ -74,6 +74,9 @@ def lib() -> ctypes.CDLL:
             ("or_update_rk3", None, [vp, vp, vp, vp, vp, i64, i32, i32]),
             ("or_update_be", None, [vp, vp, vp, vp, vp, i64, i32, vp]),
             ("or_coupling", None, [vp, vp, vp, vp, vp, i64, vp]),
+            ("or_update_be_par", None, [vp, vp, vp, vp, vp, i64, i32, vp]),
+            ("or_coupling_par", None, [vp, vp, vp, vp, vp, i64, vp]),
+            ("or_max_threads", i32, []),
             ("or_ie_vrt_nd", None, [vp, vp, dp, vp, dp, dp, dp, vp]),
             ("or_rad_solver2", dp, [vp, vp, dp, vp]),
             ("or_gauss_newton_2d", None, [vp, vp, vp, dp, dp, vp, vp, vp, vp]),
@@ -158,6 +161,25 @@ class Oracle:
         self.L.or_update_be(ctypes.byref(self.g), ctypes.byref(self.c), ctypes.byref(self.f), ctypes.byref(self.fl),
                             _p(soa), soa.shape[1], it, ctypes.byref(cnt))
         return {n: int(getattr(cnt, n)) for n, _ in _Cnt._fields_}
+
+    def update_be_par(self, soa: np.ndarray, it: int) -> dict:
+        """update_be over the host's cores (OpenMP): CPU timing baseline."""
+        cnt = _Cnt()
+        self.L.or_update_be_par(ctypes.byref(self.g), ctypes.byref(self.c), ctypes.byref(self.f),
+                                ctypes.byref(self.fl), _p(soa), soa.shape[1], it, ctypes.byref(cnt))
+        return {n: int(getattr(cnt, n)) for n, _ in _Cnt._fields_}
+
+    def coupling_par(self, soa: np.ndarray, mult: np.ndarray) -> np.ndarray:
+        """coupling over the host's cores (OpenMP, atomic deposits): CPU timing baseline."""
+        g = self.grid
+        out = np.empty((6, g.nnx + 1, g.nny + 1, g.nnz + 2))
+        mult = np.ascontiguousarray(mult, dtype=np.int64)
+        self.L.or_coupling_par(ctypes.byref(self.g), ctypes.byref(self.c), ctypes.byref(self.f), _p(soa), _p(mult),
+                               soa.shape[1], _p(out))
+        return out
+
+    def max_threads(self) -> int:
+        return int(self.L.or_max_threads())
 
     def coupling(self, soa: np.ndarray, mult: np.ndarray) -> np.ndarray:
         g = self.grid

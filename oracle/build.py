@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 def build_oracle() -> str:
     tmp = LIB + ".tmp"
-    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-std=c99", "-fPIC", "-shared",
+    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-std=c99", "-fPIC", "-shared", "-fopenmp",
            "-Wall", "-o", tmp, os.path.join(HERE, "ntlp_oracle.c"), "-lm"]
     proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if proc.returncode != 0:

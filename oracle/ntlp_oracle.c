@@ -11,6 +11,7 @@
  */
 #include "ntlp_oracle.h"
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -688,12 +689,13 @@ static void be_microphysics(const or_consts *c, const or_flags *f, double *p, do
   p[OR_ACTRES] = p[OR_ACTRES] + dt;
 }
 
-/* particle_update_BE (3000-3280) without the flux statistics (3209-3238) */
-void or_update_be(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
-                  double *soa, int64_t n, int it, or_counters *cnt) {
+/* particle_update_BE (3000-3280) without the flux statistics (3209-3238), on
+ * droplets i0 <= i < i1 of a SoA array with n droplets per column */
+static void update_be_range(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
+                            double *soa, int64_t n, int64_t i0, int64_t i1, int it, or_counters *cnt) {
   const double dt = c->dt;
   double p[NR];
-  for (int64_t i = 0; i < n; ++i) {
+  for (int64_t i = i0; i < i1; ++i) {
     load_rec(soa, n, i, p);
     p[OR_VXOLD] = p[OR_VX]; p[OR_VYOLD] = p[OR_VY]; p[OR_VZOLD] = p[OR_VZ];
     p[OR_TPOLD] = p[OR_TP];
@@ -719,6 +721,31 @@ void or_update_be(const or_grid *g, const or_consts *c, const or_flags *f, const
   }
 }
 
+void or_update_be(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
+                  double *soa, int64_t n, int it, or_counters *cnt) {
+  update_be_range(g, c, f, fl, soa, n, 0, n, it, cnt);
+}
+
+/* The same over the host's cores (OpenMP; a CPU timing baseline, not a
+ * parity oracle): droplets are independent, the counters are summed per
+ * thread. */
+void or_update_be_par(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
+                      double *soa, int64_t n, int it, or_counters *cnt) {
+#pragma omp parallel
+  {
+    or_counters local;
+    memset(&local, 0, sizeof(local));
+    const int64_t nt = omp_get_num_threads(), t = omp_get_thread_num();
+    update_be_range(g, c, f, fl, soa, n, n * t / nt, n * (t + 1) / nt, it, &local);
+#pragma omp critical
+    {
+      int64_t *a = (int64_t *)cnt;
+      const int64_t *b = (const int64_t *)&local;
+      for (size_t k = 0; k < sizeof(or_counters) / sizeof(int64_t); ++k) a[k] += b[k];
+    }
+  }
+}
+
 /* ------------------------------------------------------------ coupling */
 
 static inline int64_t src_off(const or_grid *g, int k, int j, int i) {
@@ -729,13 +756,25 @@ static inline int64_t src_off(const or_grid *g, int k, int j, int i) {
 /* particle_coupling_update (1151-1276).  Node indices are wrapped/clamped
  * into the array where the reference would write outside it (its ERROR1-4
  * branches, 1228-1236). */
-void or_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const double *soa,
-                 const int64_t *mult, int64_t n, double *src) {
+/* src[o] = src[o] + v, as one atomic update when several threads deposit */
+#define OR_ADD(dst, v)        \
+  do {                        \
+    double *a_ = &(dst);      \
+    const double v_ = (v);    \
+    if (atomic) {             \
+      _Pragma("omp atomic")   \
+      *a_ += v_;              \
+    } else {                  \
+      *a_ = *a_ + v_;         \
+    }                         \
+  } while (0)
+
+static void coupling_range(const or_grid *g, const or_consts *c, const or_flags *f, const double *soa,
+                           const int64_t *mult, int64_t n, int64_t i0, int64_t i1, double *src, int atomic) {
   const double dt = c->dt, dx = g->dx, dy = g->dy;
   int64_t nf = (int64_t)(g->nnz + 2) * (g->nny + 1) * (g->nnx + 1);
-  memset(src, 0, sizeof(double) * 6 * nf);
   double p[NR];
-  for (int64_t ip = 0; ip < n; ++ip) {
+  for (int64_t ip = i0; ip < i1; ++ip) {
     load_rec(soa, n, ip, p);
     int32_t ci[5];
     or_cell_index(g, p[OR_X], p[OR_Y], p[OR_Z], ci);
@@ -775,28 +814,50 @@ void or_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const 
           if (ix > g->nnx + 1) ix -= g->nnx;
           if (iy > g->nny + 1) iy -= g->nny;
           int64_t o = src_off(g, iz, iy, ix);
+          /* the reference's src = src - a - b as two additions of -a, -b (a + (-b) == a - b) */
           if (f->icouple == 1) {
             for (int d = 0; d < 3; ++d) {
-              src[d * nf + o] = src[d * nf + o] -
-                                partmass / rhoa * (vrhs[d] - c->part_grav[d]) * wtt / dV * rm;
-              src[d * nf + o] = src[d * nf + o] - c->rhow / rhoa * c->pi2 * 2 * (r * r) * radrhs *
-                                                      p[OR_VX + d] * wtt / dV * rm;
+              OR_ADD(src[d * nf + o], -(partmass / rhoa * (vrhs[d] - c->part_grav[d]) * wtt / dV * rm));
+              OR_ADD(src[d * nf + o], -(c->rhow / rhoa * c->pi2 * 2 * (r * r) * radrhs *
+                                         p[OR_VX + d] * wtt / dV * rm));
             }
           }
           if (f->iTcouple == 1) {
-            src[3 * nf + o] = src[3 * nf + o] -
-                              (Tprhs_s * 6.0 * c->rhow / rhop / c->CpaCpp / taup_i * (c->pi2 / 2.0) * r *
-                               c->nuf) * wtt / dV * rm;
+            OR_ADD(src[3 * nf + o], -((Tprhs_s * 6.0 * c->rhow / rhop / c->CpaCpp / taup_i * (c->pi2 / 2.0) * r *
+                                       c->nuf) * wtt / dV * rm));
           }
           if (f->iHcouple == 1) {
-            src[4 * nf + o] = src[4 * nf + o] - c->rhow / rhoa * c->pi2 * 2 * (r * r) * radrhs * wtt / dV * rm;
-            src[5 * nf + o] = src[5 * nf + o] -
-                              c->rhow / rhoa * c->pi2 * 2 * (r * r) * radrhs * c->Cpv / c->Cpa * p[OR_TP] * wtt / dV * rm +
-                              c->rhow / rhoa * c->pi2 * 2 * (r * r) * radrhs * c->Cpv / c->Cpa * p[OR_TF] * wtt / dV * rm;
+            OR_ADD(src[4 * nf + o], -(c->rhow / rhoa * c->pi2 * 2 * (r * r) * radrhs * wtt / dV * rm));
+            OR_ADD(src[5 * nf + o], -(c->rhow / rhoa * c->pi2 * 2 * (r * r) * radrhs * c->Cpv / c->Cpa * p[OR_TP] *
+                                      wtt / dV * rm));
+            OR_ADD(src[5 * nf + o], c->rhow / rhoa * c->pi2 * 2 * (r * r) * radrhs * c->Cpv / c->Cpa * p[OR_TF] *
+                                        wtt / dV * rm);
           }
         }
   }
 }
+
+void or_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const double *soa,
+                 const int64_t *mult, int64_t n, double *src) {
+  int64_t nf = (int64_t)(g->nnz + 2) * (g->nny + 1) * (g->nnx + 1);
+  memset(src, 0, sizeof(double) * 6 * nf);
+  coupling_range(g, c, f, soa, mult, n, 0, n, src, 0);
+}
+
+/* The same over the host's cores (OpenMP, atomic deposits; a CPU timing
+ * baseline, not a parity oracle: the order of the additions varies). */
+void or_coupling_par(const or_grid *g, const or_consts *c, const or_flags *f, const double *soa,
+                     const int64_t *mult, int64_t n, double *src) {
+  int64_t nf = (int64_t)(g->nnz + 2) * (g->nny + 1) * (g->nnx + 1);
+  memset(src, 0, sizeof(double) * 6 * nf);
+#pragma omp parallel
+  {
+    const int64_t nt = omp_get_num_threads(), t = omp_get_thread_num();
+    coupling_range(g, c, f, soa, mult, n, n * t / nt, n * (t + 1) / nt, src, 1);
+  }
+}
+
+int or_max_threads(void) { return omp_get_max_threads(); }
 
 /* ------------------------------------------------ coupling exchange/apply */
 

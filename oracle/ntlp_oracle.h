@@ -111,6 +111,13 @@ void or_update_be(const or_grid *g, const or_consts *c, const or_flags *f, const
 /* particle_coupling_update (1151-1276): src[6][(nnz+2)*(nny+1)*(nnx+1)] */
 void or_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const double *soa,
                  const int64_t *mult, int64_t n, double *src);
+/* OpenMP versions over the host's cores, for the CPU timing baseline only
+ * (the deposit's additions are atomic, so their order varies run to run) */
+void or_update_be_par(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
+                      double *soa, int64_t n, int it, or_counters *cnt);
+void or_coupling_par(const or_grid *g, const or_consts *c, const or_flags *f, const double *soa,
+                     const int64_t *mult, int64_t n, double *src);
+int or_max_threads(void);
 
 /* particle_coupling_exchange (1066-1149) on one periodic domain: the +1
  * ghost plane x = nnx+1 is added onto x = 1 (all y incl. nny+1), then the

@@ -15,7 +15,7 @@ from ntlp_amd import COL, NREAL, Consts, Flags, Particles
 from ntlp_amd.grid import Grid
 from oracle import Oracle
 
-from .helpers import edge_positions, small_grid
+from .helpers import edge_positions, small_case, small_grid
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -517,3 +517,16 @@ def test_boundary_conditions_hand_cases():
     assert 0 <= p["x"][1] < g.xl / 2 and g.xl / 2 < p["x"][2] <= g.xl
     # x == xl is not wrapped (strict >, particles.f90:2826) when it stays put
     assert p["x"][3] == g.xl
+
+
+def test_parallel_oracle_matches_sequential():
+    """The OpenMP oracle the bench times as its all-core CPU baseline computes
+    what the sequential oracle does: the same state bit for bit, the same
+    counters, and the deposit to rounding (its atomic additions run in any
+    order)."""
+    grid, consts, flags, fields, p, ora = small_case(seed=3, n=5000)
+    a, b = p.copy(), p.copy()
+    assert ora.update_be(a.soa, 2) == ora.update_be_par(b.soa, 2)
+    assert np.array_equal(a.soa, b.soa)
+    sa, sb = ora.coupling(a.soa, a.mult), ora.coupling_par(b.soa, b.mult)
+    assert np.max(np.abs(sa - sb)) <= 1e-14 * np.max(np.abs(sa))
