@@ -29,8 +29,8 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # multiply-adds fused only within one source expression, before any loop
     # hoisting: every kernel that inlines the same droplet arithmetic (the
-    # one-droplet step, the per-box fused step, the serving step) rounds it
-    # identically
+    # multi-step kernel, the single-call update kernels, the serving step)
+    # rounds it identically
     "-ffp-contract=on",
     # no machine-level hoisting of loop invariants: the droplet kernels' Newton
     # loops otherwise keep hoisted copies of constants live in vector

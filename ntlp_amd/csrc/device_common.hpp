@@ -51,6 +51,9 @@ struct DevConsts {
   double isurf_rho;       // 1 / surf_rho
   double Mw2Gam_Ru_rhow_um;  // Mw2Gam_Ru_rhow in micrometre units (x 1e6)
   double isol_um3;        // isol in micrometre units (x 1e18)
+  // 0.6 Sc^(1/3), 0.6 Pra^(1/3) of Sherwood/Nusselt: read from the constants
+  // (scalar registers) rather than kept per droplet in vector registers
+  double sc6, pr6;
 };
 
 struct DevFlags {

@@ -339,6 +339,8 @@ void derive_consts(DevConsts& C) {
   C.isurf_rho = 1.0 / C.surf_rho;
   C.Mw2Gam_Ru_rhow_um = C.Mw2Gam_Ru_rhow * 1e6;
   C.isol_um3 = C.isol * 1e18;
+  C.sc6 = 0.6 * C.Sc13;
+  C.pr6 = 0.6 * C.Pra13;
 }
 
 // the device copy of ctx.C, in stream order with the kernels that read it

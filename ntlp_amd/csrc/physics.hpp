@@ -158,7 +158,7 @@ struct RefEval {
 //   E  = Lv Mw/Ru (Tn - Tf)/(Tf Tn) + 2 Mw Γ/(Ru ρw rn Tn) - κ m_s/(ρs V(rn))
 // (using ρp V - m_s = ρw V; taup0 cancels), F = (tr - 1 - X, tt - 1 - Y).
 struct AJIn {
-  double r, Tp, Tf, qinf, K, csol, LMRiTf, sq, hA, hB, Cc, sc6, pr6;
+  double r, Tp, Tf, qinf, K, csol, LMRiTf, sq, hA, hB, Cc;
 };
 
 struct AJEval {
@@ -170,7 +170,7 @@ struct AJEval {
     const double irn = rcp64(rn), iTn = rcp64(Tn);
     const double irn2 = irn * irn;
     const double s = P.sq * sqrt(rn);  // sqrt(Re)
-    const double Sh = 2.0 + P.sc6 * s, Nu = 2.0 + P.pr6 * s;
+    const double Sh = 2.0 + C.sc6 * s, Nu = 2.0 + C.pr6 * s;
     const double curv = C.Mw2Gam_Ru_rhow * irn * iTn;
     const double solu = P.csol * irn2 * irn;
     const double dT = Tn - P.Tf;
@@ -450,8 +450,6 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
       P.hA = C.hA0 * rhoa * ir;
       P.hB = C.hB0 * rhoa * iTp;
       P.Cc = C.Lv3_Cpp * d.r * iTp;
-      P.sc6 = 0.6 * C.Sc13;
-      P.pr6 = 0.6 * C.Pra13;
       const AJEval ev{C, P};
       if (PREC == PREC_MIXED) {
         // Start from the unchanged state (1, Tf/Tp) when its residual is
@@ -534,15 +532,6 @@ __device__ __forceinline__ double qstar_f32(const DevConsts& C, const Drop& d, d
 }
 
 // ---------------------------------------------------------- BE droplet
-
-// particle_bcs_periodic alone (2814-2843); it does not depend on the radius,
-// so it may run before the wall reflection of the same step.
-__device__ __forceinline__ void apply_periodic(const DevGrid& g, Drop& d) {
-  if (d.x > g.xl) d.x = d.x - g.xl;
-  else if (d.x < 0) d.x = g.xl + d.x;
-  if (d.y > g.yl) d.y = d.y - g.yl;
-  else if (d.y < 0) d.y = g.yl + d.y;
-}
 
 // What the radius/temperature solve of one droplet needs from the advance.
 struct BeHand {
