@@ -171,8 +171,11 @@ int ntlp_gpu_particle_update_rk3_prec(int32_t istage, int32_t it, int32_t precis
 int ntlp_gpu_particle_update_be_prec(int32_t it, int32_t precision);
 /* particle_coupling_update (particles.f90:1151-1276).  Sources stay resident;
  * when src_out is not NULL it receives partsrc_t(0:nnz+1,1:nny+1,1:nnx+1,1:3),
- * partTsrc_t, partHsrc_t, partTEsrc_t as 6 consecutive arrays.  deterministic=1
- * uses the fixed-order node-centric sum instead of segmented atomics. */
+ * partTsrc_t, partHsrc_t, partTEsrc_t as 6 consecutive arrays.  The method:
+ * deterministic=0 sums each 8x8x8 node box in LDS and stores it (the default,
+ * used by ntlp_gpu_step); 1 sums each node's droplets in a fixed order (same
+ * bits on every run); 2 sums runs of equal cells inside each wavefront, then
+ * adds the run sums with FP64 global atomics.  Other values: NTLP_E_INVALID. */
 int ntlp_gpu_particle_coupling_update(double *src_out, int32_t deterministic);
 
 /* Copies the resident coupling sources (same layout as src_out above) as the

@@ -340,9 +340,11 @@ class DropletStep:
     def particle_update_BE(self, it: int = 2, precision: int = PREC_FP64) -> None:
         self._check(self.lib.ntlp_gpu_particle_update_be_prec(it, precision))
 
-    def particle_coupling_update(self, deterministic: bool = False) -> np.ndarray:
+    def particle_coupling_update(self, deterministic: int = 0) -> np.ndarray:
         """Returns partsrc_t(:,:,:,1:3), partTsrc_t, partHsrc_t, partTEsrc_t as
-        an array (6, nnx+1, nny+1, nnz+2) (C order; z fastest)."""
+        an array (6, nnx+1, nny+1, nnz+2) (C order; z fastest).  deterministic
+        picks the summation: 0 (or False) LDS node boxes, 1 (or True) a fixed
+        order per node, 2 wavefront-segmented run sums with global atomics."""
         g = self.grid
         out = np.empty((6, g.nnx + 1, g.nny + 1, g.nnz + 2))
         self._check(self.lib.ntlp_gpu_particle_coupling_update(_ptr(out), int(deterministic)))

@@ -416,7 +416,11 @@ unsigned src_mask() {
   return m;
 }
 
+// wave-segmented sums of equal-key runs, then FP64 global atomics (clears the
+// fields first); works in slot order, sorted or not
 int scatter_atomic() {
+  ctx.src_valid = true;
+  ctx.xsrc_valid = false;
   CHK(fill_f64(ctx.src, 6 * ctx.nsrc, 0.0));
   ScatterArgs s;
   s.g = ctx.g;
@@ -428,7 +432,7 @@ int scatter_atomic() {
   s.src = ctx.src;
   s.nsrc = ctx.nsrc;
   s.fmask = src_mask();
-  return launch("scatter", [&] { k_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(s); });
+  return launch("scatter_seg", [&] { k_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(s); });
 }
 
 // atomic-free deterministic scatter over the current cell sort (perm and
@@ -1049,13 +1053,17 @@ int ntlp_gpu_particle_update_be(int32_t it) { return ntlp_gpu_particle_update_be
 
 int ntlp_gpu_particle_coupling_update(double* src_out, int32_t deterministic) {
   CHK(need_drops());
+  if (deterministic < 0 || deterministic > 2)
+    return fail(NTLP_E_INVALID, "coupling method " + std::to_string(deterministic) +
+                                    ": 0 (box), 1 (fixed order) or 2 (segmented atomics)");
   if (!ctx.amounts_valid) {  // amounts from the state and its old values in cold
     CHK(flush_diag());
     StepArgs a = base_args();
     CHK(launch("amounts", [&] { k_amounts<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(a); }));
     ctx.amounts_valid = true;
   }
-  if (deterministic) CHK(scatter_pull());
+  if (deterministic == 1) CHK(scatter_pull());
+  else if (deterministic == 2) CHK(scatter_atomic());
   else CHK(scatter_tile());
   if (src_out)
     HIPCHK(hipMemcpyAsync(src_out, ctx.src, sizeof(double) * 6 * ctx.nsrc, hipMemcpyDeviceToHost, ctx.stream));

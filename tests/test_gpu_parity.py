@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from ntlp_amd import (COL, LAYOUT_CORE, LAYOUT_EXT, LAYOUT_XPENCIL, MODE_BE, MODE_RK3, PREC_FP64,
-                      PREC_FP64_AJ, PREC_MIXED, DropletStep, Flags)
+                      PREC_FP64_AJ, PREC_MIXED, DropletStep, Flags, NtlpError)
 
 from .helpers import edge_positions, rel_err, small_case
 
@@ -207,11 +207,12 @@ def test_particle_update_rk3_fp64():
     s.close()
 
 
-@pytest.mark.parametrize("deterministic", [False, True])
+@pytest.mark.parametrize("deterministic", [0, 1, 2])
 @pytest.mark.parametrize("kind", ["uniform", "stretched"])
 def test_coupling_update(deterministic, kind):
     """particle_coupling_update on the oracle's own post-step state: only the
-    deposit arithmetic and the summation order differ."""
+    deposit arithmetic and the summation order differ.  Methods: 0 LDS node
+    boxes, 1 fixed order per node, 2 wavefront-segmented sums + atomics."""
     grid, consts, flags, fields, p, ora = small_case(seed=8, kind=kind)
     want = p.copy()
     ora.update_be(want.soa, 2)
@@ -224,9 +225,17 @@ def test_coupling_update(deterministic, kind):
         # terms ~Tp/(Tp-Tf) ~ 3000 times larger than their sum cancel
         tol = 1e-12 if f < 5 else 1e-9
         assert np.max(np.abs(src[f] - src_o[f])) <= tol * scale, f
-    if deterministic:
-        again = s.particle_coupling_update(deterministic=True)
+    if deterministic == 1:
+        again = s.particle_coupling_update(deterministic=1)
         assert np.array_equal(again, src)
+    s.close()
+
+
+def test_coupling_method_out_of_range_is_an_error():
+    grid, consts, flags, fields, p, ora = small_case(seed=8, n=256)
+    s = make_step(grid, consts, flags, fields, p)
+    with pytest.raises(NtlpError):
+        s.particle_coupling_update(deterministic=3)
     s.close()
 
 
