@@ -1227,18 +1227,36 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const int32_t* in, i
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
   // thread-contiguous items for a cheap serial prefix
   const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
+  static_assert(kScanItems == 8, "two 16-byte vectors per thread");
+  // 16-byte loads and stores away from the end (in and out come from
+  // hipMalloc, so in + i0 and out + i0 are 32-byte aligned)
+  const bool whole = i0 + kScanItems <= n;
   int v[kScanItems], sum = 0;
+  if (whole) {
+    const int4 a = reinterpret_cast<const int4*>(in + i0)[0], b = reinterpret_cast<const int4*>(in + i0)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    v[k] = (i0 + k < n) ? in[i0 + k] : 0;
-    sum += v[k];
+    for (int k = 0; k < kScanItems; ++k) v[k] = (i0 + k < n) ? in[i0 + k] : 0;
   }
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) sum += v[k];
   int total;
   int run = block_exclusive_scan(sum, s_wave, total) + bsum[blockIdx.x];
+  int o[kScanItems];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    if (i0 + k < n) out[i0 + k] = run;
+    o[k] = run;
     run += v[k];
+  }
+  if (whole) {
+    reinterpret_cast<int4*>(out + i0)[0] = make_int4(o[0], o[1], o[2], o[3]);
+    reinterpret_cast<int4*>(out + i0)[1] = make_int4(o[4], o[5], o[6], o[7]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+      if (i0 + k < n) out[i0 + k] = o[k];
   }
   // grand total at out[n]
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = bsum[blockIdx.x] + total;
