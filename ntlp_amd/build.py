@@ -7,7 +7,9 @@ present.  Both always recompile.
 """
 from __future__ import annotations
 
+import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -39,13 +41,40 @@ HIPCC_FLAGS = [
 ]
 
 
-def _run(cmd: list[str], cwd: str | None = None) -> None:
+def _run(cmd: list[str], cwd: str | None = None) -> str:
+    """Runs cmd; raises with its output on failure, echoes anything it printed
+    except the host pass's packed-FP32 notice and the resource remarks, and
+    returns the full output."""
     proc = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    out = "\n".join(l for l in proc.stdout.splitlines() if "packed-fp32-ops' is not a recognized" not in l)
+    out = "\n".join(l for l in proc.stdout.splitlines()
+                    if "packed-fp32-ops' is not a recognized" not in l and "kernel-resource-usage" not in l
+                    and not l.startswith("In file included from"))
     if proc.returncode != 0:
         raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
     if out.strip():
         sys.stderr.write(out + "\n")
+    return proc.stdout
+
+
+# per-kernel register, scratch and occupancy figures of the last library build
+RESOURCES = os.path.join(HERE, "kernel_resources.json")
+_REMARK = re.compile(r"remark:\s+(Function Name|VGPRs|AGPRs|ScratchSize \[bytes/lane\]|"
+                     r"Occupancy \[waves/SIMD\]|VGPRs Spill|SGPRs Spill|LDS Size \[bytes/block\]):\s+(\S+)")
+
+
+def parse_resources(text: str) -> dict:
+    """{mangled kernel name: {field: int}} from -Rpass-analysis=kernel-resource-usage."""
+    res, cur = {}, None
+    for line in text.splitlines():
+        m = _REMARK.search(line)
+        if not m:
+            continue
+        key, val = m.group(1), m.group(2)
+        if key == "Function Name":
+            cur = res.setdefault(val, {})
+        elif cur is not None:
+            cur[key.split(" [")[0]] = int(val)
+    return res
 
 
 def hipcc() -> str:
@@ -58,8 +87,11 @@ def hipcc() -> str:
 def build_library() -> str:
     """Compile csrc/ntlp_gpu.hip into ntlp_amd/libntlp_gpu.so (gfx950)."""
     tmp = LIB + ".tmp"
-    _run([hipcc(), *HIPCC_FLAGS, "-o", tmp, os.path.join(CSRC, "ntlp_gpu.hip")])
+    out = _run([hipcc(), *HIPCC_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-fno-caret-diagnostics", "-o", tmp,
+                os.path.join(CSRC, "ntlp_gpu.hip")])
     os.replace(tmp, LIB)
+    with open(RESOURCES, "w") as f:
+        json.dump(parse_resources(out), f, indent=1, sort_keys=True)
     return LIB
 
 
