@@ -91,6 +91,7 @@ struct StepArgs {
   int write_cold;        // write the record diagnostics (final step of a fused call)
   int inplace;           // update slot t in place (src == dst, no perm, keys == keys_in)
   double* diag;          // [kNCold][cap] diagnostics of the final step in dst slot order
+  int lazy_j;            // BE library steps: index of this step within its call
 };
 
 // Workgroup -> droplet chunk mapping that keeps consecutive chunks on one XCD:
@@ -176,8 +177,25 @@ __global__ void k_tally_close(unsigned long long* slots, unsigned long long* row
 
 // ------------------------------------------------------------ load/store
 
-__device__ __forceinline__ Drop load_drop(const double* hot, int64_t s) {
+// with_res false: the residence times stay in HBM (d.res = d.actres = 0)
+__device__ __forceinline__ Drop load_drop(const double* hot, int64_t s, bool with_res = true) {
   Drop d;
+  if (!with_res) {
+    d.x = hot[hix(kHX, s)];
+    d.y = hot[hix(kHY, s)];
+    d.z = hot[hix(kHZ, s)];
+    d.vx = hot[hix(kHVX, s)];
+    d.vy = hot[hix(kHVY, s)];
+    d.vz = hot[hix(kHVZ, s)];
+    d.Tp = hot[hix(kHTP, s)];
+    d.r = hot[hix(kHR, s)];
+    d.m_s = hot[hix(kHMS, s)];
+    d.kappa = hot[hix(kHKAPPA, s)];
+    d.rc = hot[hix(kHRC, s)];
+    d.res = d.actres = 0.0;
+    d.numact = hot[hix(kHNUMACT, s)];
+    return d;
+  }
   d.x = hot[hix(kHX, s)];
   d.y = hot[hix(kHY, s)];
   d.z = hot[hix(kHZ, s)];
@@ -222,7 +240,8 @@ __device__ __forceinline__ void store_drop_rows(double* out, int64_t ld, int64_t
 // State that moves with a droplet in the fused step but is constant in it.
 // numact changes only when the droplet activates: with write_numact false it
 // is left as it is (the in-place step stores it only when it changed).
-__device__ __forceinline__ void store_dynamic(double* hot, int64_t s, const Drop& d, bool write_numact = true) {
+__device__ __forceinline__ void store_dynamic(double* hot, int64_t s, const Drop& d, bool write_numact = true,
+                                              bool write_res = true) {
   hot[hix(kHX, s)] = d.x;
   hot[hix(kHY, s)] = d.y;
   hot[hix(kHZ, s)] = d.z;
@@ -231,9 +250,38 @@ __device__ __forceinline__ void store_dynamic(double* hot, int64_t s, const Drop
   hot[hix(kHVZ, s)] = d.vz;
   hot[hix(kHTP, s)] = d.Tp;
   hot[hix(kHR, s)] = d.r;
-  hot[hix(kHRES, s)] = d.res;
-  hot[hix(kHACTRES, s)] = d.actres;
+  if (write_res) {
+    hot[hix(kHRES, s)] = d.res;
+    hot[hix(kHACTRES, s)] = d.actres;
+  }
   if (write_numact) hot[hix(kHNUMACT, s)] = d.numact;
+}
+
+// Residence times over the steps of one multi-step BE call.  res and actres
+// grow by dt every step (particles.f90:3204-3205) and actres restarts from 0
+// when the droplet activates (3172).  The call's step kernels leave both in
+// HBM; an activation in step j of the call leaves j NaN-boxed in actres, and
+// k_res_settle then makes the call's additions, one dt at a time in step
+// order, so the bits are those of per-step updates.  This keeps 32 bytes per
+// droplet and step out of the step kernel's traffic.
+constexpr int kActMarkHi = 0x7FFDEAD0;
+__device__ __forceinline__ double act_mark(int j) { return __hiloint2double(kActMarkHi, j); }
+
+__global__ void __launch_bounds__(kBlock) k_res_settle(double* hot, int64_t n, int nsteps, double dt) {
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
+    double res = hot[hix(kHRES, t)], act = hot[hix(kHACTRES, t)];
+    int j0 = 0;
+    if (__double2hiint(act) == kActMarkHi) {  // activated in step j0 of the call
+      j0 = __double2loint(act);
+      act = 0.0;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+      res = res + dt;
+      if (k >= j0) act = act + dt;
+    }
+    hot[hix(kHRES, t)] = res;
+    hot[hix(kHACTRES, t)] = act;
+  }
 }
 
 __device__ __forceinline__ void load_rhs(const double* cold, int64_t cap, int64_t h, Diag& o) {
@@ -322,9 +370,11 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
   int key = -1;
   const bool inplace = !SERVE && a.inplace;
+  // BE steps of the library keep the residence times in HBM (k_res_settle)
+  constexpr bool LAZY = (MODE == 2) && !SERVE;
   if (valid) {
     const int64_t s = (a.perm && !inplace) ? (int64_t)a.perm[t] : t;
-    Drop d = load_drop(a.src.hot, s);
+    Drop d = load_drop(a.src.hot, s, !LAZY);
     const double numact0 = d.numact;
     const int64_t mult = a.src.mult[s];
     const int32_t home = inplace ? 0 : a.src.home[s];
@@ -338,7 +388,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
       for (int st = 1; st <= 3; ++st)
         rk3_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, st, a.it, o, o.kpt);
     } else {
-      be_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg);
+      be_droplet<PREC, ILIN, LAZY>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg);
     }
     if (SERVE && a.serve->last[home >> kServeIdxBits]) {
       const int64_t row = a.serve->out_off[home >> kServeIdxBits] + (home & kServeIdxMask);
@@ -352,7 +402,14 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];
       }
-      store_dynamic(a.dst.hot, t, d, !inplace || d.numact != numact0);
+      store_dynamic(a.dst.hot, t, d, !inplace || d.numact != numact0, !LAZY);
+      if (LAZY) {
+        if (!inplace) {  // carried to the new slot as they are
+          a.dst.hot[hix(kHRES, t)] = a.src.hot[hix(kHRES, s)];
+          a.dst.hot[hix(kHACTRES, t)] = a.src.hot[hix(kHACTRES, s)];
+        }
+        if (tl.actnum != 0) a.dst.hot[hix(kHACTRES, t)] = act_mark(a.lazy_j);
+      }
       if (!inplace) {
         a.dst.hot[hix(kHMS, t)] = d.m_s;
         a.dst.hot[hix(kHKAPPA, t)] = d.kappa;

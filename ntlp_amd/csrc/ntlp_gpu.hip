@@ -577,6 +577,7 @@ int run_fused(int mode, int prec, int nsteps, int it0, bool scatter) {
     StepArgs a = base_args();
     a.it = it0 + k;
     a.write_cold = (k == nsteps - 1) ? 1 : 0;
+    a.lazy_j = k;
     if (reorder) {  // read through the sort, write slot t of the other buffer
       a.perm = ctx.perm;
       a.keys_in = ctx.keys[ctx.cur];
@@ -601,6 +602,12 @@ int run_fused(int mode, int prec, int nsteps, int it0, bool scatter) {
     CHK(close_step());
     CHK(finish_bin(nullptr, false));
     if (scatter) CHK(tile_sorted());
+  }
+  if (mode == NTLP_MODE_BE) {  // the call's residence-time additions
+    double* hot = ctx.hot[ctx.cur];
+    const int64_t n = ctx.n;
+    const double dt = ctx.C.dt;
+    CHK(launch("res_settle", [&] { k_res_settle<<<grid_flat(n), kBlock, 0, ctx.stream>>>(hot, n, nsteps, dt); }));
   }
   ctx.diag_mask = mask;
   ctx.amounts_valid = true;

@@ -617,7 +617,7 @@ __device__ __forceinline__ BeHand be_advance(const DevGrid& g, const DevConsts& 
 
 // Second half (3103-3206 and the wall reflection of particle_bcs_nonperiodic):
 // radius/temperature solve, q*, residence times, then walls at the new radius.
-template <int PREC>
+template <int PREC, bool LAZY = false>
 __device__ __forceinline__ void be_finish(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
                                           const double* s_z, Drop& d, const BeHand& h, double& qstar,
                                           Tally& tl) {
@@ -625,18 +625,20 @@ __device__ __forceinline__ void be_finish(const DevGrid& g, const DevConsts& C, 
   const double esa = (PREC == PREC_FP64) ? mod_magnus(h.Tf) : mod_magnus_fast(h.Tf);
   be_microphysics<PREC>(C, fl, d, h.Tf, h.qinf, h.rhoa, h.diffnorm, h.Rep, esa, tl);
   qstar = (PREC == PREC_MIXED) ? qstar_f32(C, d, h.Tf, h.rhoa, esa) : qstar_f64(C, d, h.Tf, h.rhoa, esa);
-  d.res = d.res + dt;
-  d.actres = d.actres + dt;
+  if (!LAZY) {  // LAZY: left to the end of the call (k_res_settle)
+    d.res = d.res + dt;
+    d.actres = d.actres + dt;
+  }
   apply_bcs(g, s_z, d);
 }
 
 // One droplet of particle_update_BE (3043-3206) plus its boundary conditions.
-template <int PREC, int ILIN>
+template <int PREC, int ILIN, bool LAZY = false>
 __device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
                                            const DevFields& F, const double* s_z, const double* s_zz,
                                            Drop& d, int it, Diag& o, Tally& tl, int kguess = -1) {
   const BeHand h = be_advance<PREC, ILIN>(g, C, fl, F, s_z, s_zz, d, it, o, kguess);
-  be_finish<PREC>(g, C, fl, s_z, d, h, o.qstar, tl);
+  be_finish<PREC, LAZY>(g, C, fl, s_z, d, h, o.qstar, tl);
 }
 
 // ---------------------------------------------------------- RK3 droplet

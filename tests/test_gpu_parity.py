@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from ntlp_amd import (COL, LAYOUT_CORE, LAYOUT_EXT, LAYOUT_XPENCIL, MODE_BE, MODE_RK3, PREC_FP64,
-                      PREC_FP64_AJ, PREC_MIXED, DropletStep, Flags, NtlpError)
+                      PREC_FP64_AJ, PREC_MIXED, DropletStep, Flags, NtlpError, synth)
 
 from .helpers import edge_positions, rel_err, small_case
 
@@ -290,6 +290,33 @@ def test_fused_steps_match_single_steps(mode):
         scale = np.max(np.abs(src_o[f])) + 1e-300
         assert np.max(np.abs(src[f] - src_o[f])) <= 1e-5 * scale, f
     s.close()
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_MIXED])
+def test_residence_times_of_a_multi_step_call_equal_per_step_calls(prec):
+    """A BE call of many steps adds the residence times at its end
+    (particles.f90:3204-3205, actres restarting at activations, 3172); the
+    bits equal those of one-step calls, across reorderings and activations."""
+    wl = synth.workload(3, n=400_000, seed=3)
+    s = DropletStep(wl.grid, wl.consts, wl.flags)
+    s.set_fields(*wl.fields)
+    s.put(wl.droplets)
+    s.interp(set_vp=True, set_tp=True)
+    s.crit_radius()
+    state = s.get()
+    s.counters(reset=True)
+    s.step(MODE_BE, prec, nsteps=20, it0=2)
+    many = s.get()
+    activations = s.counters(reset=True)["actnum"]
+    s.put(state)
+    for k in range(20):
+        s.step(MODE_BE, prec, nsteps=1, it0=2 + k)
+    ones = s.get()
+    s.close()
+    assert activations > 0
+    for c in ("res", "actres", "numact", "radius", "Tp"):
+        assert np.array_equal(many[c], ones[c]), c
+    assert np.all(many["actres"] <= many["res"])
 
 
 @pytest.mark.parametrize("prec", [PREC_FP64, PREC_MIXED])
