@@ -36,14 +36,17 @@ def kernel_bytes(name: str, n: int, grid, precision: int) -> float | None:
     """Algorithmic HBM bytes of one launch of a fused-step kernel over n
     droplets (each byte the kernel must move once).
 
-    step_be (in place): reads the 14 FP64 hot columns, mult (8) and the u-cell
-      key (4); writes the 10 columns that change every step (numact only when
-      a droplet activates), the 6 FP64 coupling amounts and the new key; reads
-      once the gathered fields over the ext array (u, v, w in FP32 for mixed
-      precision, else FP64; T, q FP64).
+    step_be (in place): reads 12 of the 14 FP64 hot columns (the residence
+      times stay in HBM until the end of the call), mult (8) and the u-cell
+      key (4); writes the 8 columns that change every step (numact and actres
+      only when a droplet activates), the 6 FP64 coupling amounts and the new
+      key; reads once the gathered fields over the ext array (u, v, w in FP32
+      for mixed precision, else FP64; T, q FP64).
     step_be_sorted (the step that reorders into cell order): reads also the
       sort permutation (4) and home (4), writes all 14 hot columns, mult and
       home into the other buffer.
+    res_settle (once per call): reads and writes the two residence-time
+      columns.
     *_last: the last step of a call also writes the 11 BE diagnostics (the
       gathered fluid state and the pre-step values), 88 bytes per droplet.
     scatter: reads the sort permutation, the key, x, y, z and the 6 amounts of
@@ -56,10 +59,12 @@ def kernel_bytes(name: str, n: int, grid, precision: int) -> float | None:
         if "_sorted" in name:
             per = (4 + 14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4 + 6 * 8 + 4)
         else:
-            per = (14 * 8 + 8 + 4) + (10 * 8 + 6 * 8 + 4)
+            per = (12 * 8 + 8 + 4) + (8 * 8 + 6 * 8 + 4)
         if name.endswith("_last"):
             per += 11 * 8
         return float(n) * per + float(ext) * per_cell
+    if name == "res_settle":
+        return float(n) * 4 * 8
     if name == "scatter":
         return float(n) * (4 + 4 + 3 * 8 + 6 * 8) + 6 * 8 * float(nodes)
     return None
