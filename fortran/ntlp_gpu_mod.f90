@@ -65,7 +65,8 @@ module ntlp_gpu
        ntlp_gpu_particle_coupling_exchange, ntlp_gpu_apply_particle_coupling, &
        ntlp_gpu_bin, ntlp_gpu_wcell_counts, ntlp_gpu_num_ucells, ntlp_gpu_step, ntlp_gpu_counters, &
        ntlp_gpu_calc_tauc_min, ntlp_gpu_particle_xy_stats, &
-       ntlp_gpu_step_counters, ntlp_gpu_sync
+       ntlp_gpu_step_counters, ntlp_gpu_sync, ntlp_gpu_be_flux
+  public :: gpu_be_flux
   public :: gpu_check, gpu_setup, gpu_set_fields, gpu_put, gpu_get, gpu_interp, gpu_crit_radius, &
        gpu_particle_update_rk3, gpu_particle_update_BE, gpu_particle_coupling_update, gpu_step, gpu_counters, &
        gpu_particle_coupling_exchange, gpu_apply_particle_coupling, gpu_calc_tauc_min, gpu_particle_xy_stats
@@ -199,6 +200,10 @@ module ntlp_gpu
        integer(c_int64_t), intent(out) :: rows(*)
        integer(c_int32_t), value :: max_rows
      end function ntlp_gpu_step_counters
+     integer(c_int32_t) function ntlp_gpu_be_flux(pflux, pmassflux, penegflux) bind(c, name="ntlp_gpu_be_flux")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: pflux, pmassflux, penegflux
+     end function ntlp_gpu_be_flux
      integer(c_int32_t) function ntlp_gpu_sync() bind(c, name="ntlp_gpu_sync")
        import :: c_int32_t
      end function ntlp_gpu_sync
@@ -344,6 +349,13 @@ contains
     call gpu_check(ntlp_gpu_step(int(mode, c_int32_t), int(prec, c_int32_t), int(nsteps, c_int32_t), &
          int(it0, c_int32_t)), 'ntlp_gpu_step')
   end subroutine gpu_step
+
+  ! pflux, pmassflux, penegflux(0:nnz) of the last particle_update_BE step
+  ! (particles.f90:3208-3238), before les.F divides them by dt*xl*yl.
+  subroutine gpu_be_flux(pflux, pmassflux, penegflux)
+    real(c_double), intent(out), target :: pflux(0:), pmassflux(0:), penegflux(0:)
+    call gpu_check(ntlp_gpu_be_flux(c_loc(pflux), c_loc(pmassflux), c_loc(penegflux)), 'ntlp_gpu_be_flux')
+  end subroutine gpu_be_flux
 
   subroutine gpu_counters(c, reset)
     type(ntlp_counters_t), intent(out) :: c

@@ -227,6 +227,14 @@ int ntlp_gpu_calc_tauc_min(double *tauc_min);
  * ntlp_gpu_counters. */
 int ntlp_gpu_particle_xy_stats(double *s0, double *s1, double *lwp);
 
+/* The flux statistics of the last particle_update_BE step (particles.f90:
+ * 3030-3032, 3208-3238): pflux(0:nnz), pmassflux(0:nnz), penegflux(0:nnz),
+ * the droplets (times mult), droplet mass and droplet enthalpy that crossed
+ * each w level z(iz) upward minus those that crossed it downward, counted
+ * between the advance and the walls, with zl = z(nnz).  Each output has
+ * nnz+1 values and may be NULL.  NTLP_E_STATE before the first BE step. */
+int ntlp_gpu_be_flux(double *pflux, double *pmassflux, double *penegflux);
+
 /* -------------------------------------------------------- fused stepping */
 
 /* nsteps whole droplet steps against the resident fields (which stay fixed),
@@ -237,7 +245,7 @@ int ntlp_gpu_step(int32_t mode, int32_t precision, int32_t nsteps, int32_t it0);
 
 /* -------------------------------------------------- counters and timing */
 
-/* Sums of the Newton bookkeeping since the last reset. */
+/* Sums of the Newton bookkeeping over every step since the last reset. */
 int ntlp_gpu_counters(ntlp_counters *out, int32_t reset);
 /* Per-step rows of the same counters, oldest first, for the last steps (up to
  * max_rows, at most 4095): row r holds NTLP_STEP_ROW int64 values num100,

@@ -228,6 +228,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_calc_tauc_min": (ctypes.c_int, [ctypes.POINTER(dp)]),
         "ntlp_gpu_particle_xy_stats": (ctypes.c_int, [vp, vp, vp]),
         "ntlp_gpu_step": (ctypes.c_int, [i32, i32, i32, i32]),
+        "ntlp_gpu_be_flux": (ctypes.c_int, [vp, vp, vp]),
         "ntlp_gpu_counters": (ctypes.c_int, [ctypes.POINTER(_CCounters), i32]),
         "ntlp_gpu_step_counters": (ctypes.c_int, [vp, i32]),
         "ntlp_gpu_timing": (ctypes.c_int, [i32]),
@@ -413,6 +414,14 @@ class DropletStep:
     # ------------------------------------------------------------ stepping
     def step(self, mode: int = MODE_BE, precision: int = PREC_FP64, nsteps: int = 1, it0: int = 2) -> None:
         self._check(self.lib.ntlp_gpu_step(mode, precision, nsteps, it0))
+
+    def be_flux(self) -> np.ndarray:
+        """Flux statistics of the last particle_update_BE step
+        (particles.f90:3208-3238): (3, nnz+1) = pflux, pmassflux, penegflux
+        over w levels 0..nnz."""
+        out = np.empty((3, self.grid.nnz + 1))
+        self._check(self.lib.ntlp_gpu_be_flux(_ptr(out[0]), _ptr(out[1]), _ptr(out[2])))
+        return out
 
     def counters(self, reset: bool = False) -> dict:
         c = _CCounters()

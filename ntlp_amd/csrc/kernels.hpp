@@ -92,6 +92,7 @@ struct StepArgs {
   int inplace;           // update slot t in place (src == dst, no perm, keys == keys_in)
   double* diag;          // [kNCold][cap] diagnostics of the final step in dst slot order
   int lazy_j;            // BE library steps: index of this step within its call
+  double* flux;          // BE flux statistics [3][nnz+1] (zero on entry), or null
 };
 
 // Workgroup -> droplet chunk mapping that keeps consecutive chunks on one XCD:
@@ -163,16 +164,23 @@ __device__ __forceinline__ void zero_tally_lds(int* s_red) {
   for (int i = threadIdx.x; i <= kTallyWidth; i += blockDim.x) s_red[i] = 0;
 }
 
-// Folds the slots into the step's row (and counts the step), clearing them.
-__global__ void k_tally_close(unsigned long long* slots, unsigned long long* row, int steps) {
+// Folds the slots into the step's row and into the running totals (and
+// counts the step), clearing the slots; clears `flux_next` (nflux values),
+// the flux statistics buffer of the next BE step, when it is not null.
+__global__ void k_tally_close(unsigned long long* slots, unsigned long long* row, unsigned long long* totals,
+                              int steps, double* flux_next, int nflux) {
   for (int c = threadIdx.x; c < kTallyWidth; c += blockDim.x) {
     unsigned long long s = 0;
     for (int k = 0; k < kTallySlots; ++k) {
       s += slots[k * kTallyWidth + c];
       slots[k * kTallyWidth + c] = 0ull;
     }
-    row[c] = s + (c == kTSteps ? (unsigned long long)steps : 0ull);
+    s += (c == kTSteps ? (unsigned long long)steps : 0ull);
+    row[c] = s;
+    totals[c] += s;
   }
+  if (flux_next)
+    for (int i = threadIdx.x; i < nflux; i += blockDim.x) flux_next[i] = 0.0;
 }
 
 // ------------------------------------------------------------ load/store
@@ -357,7 +365,9 @@ __device__ __forceinline__ void count_key(int* counts, int key, bool valid) {
 // block; a droplet whose request takes its last step here is written to the
 // request's output rows and leaves the pool (key -1: the next sort drops it);
 // no coupling amounts are formed.
-template <int MODE, int PREC, int ILIN, bool SERVE = false>
+// LAZY (BE steps of the library): the residence times stay in HBM until
+// k_res_settle at the end of the call.
+template <int MODE, int PREC, int ILIN, bool SERVE = false, bool LAZY = false>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   const DevConsts& C = *a.Cp;
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
@@ -370,8 +380,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
   int key = -1;
   const bool inplace = !SERVE && a.inplace;
-  // BE steps of the library keep the residence times in HBM (k_res_settle)
-  constexpr bool LAZY = (MODE == 2) && !SERVE;
+  static_assert(!LAZY || (MODE == 2 && !SERVE), "lazy residence times: library BE steps only");
   if (valid) {
     const int64_t s = (a.perm && !inplace) ? (int64_t)a.perm[t] : t;
     Drop d = load_drop(a.src.hot, s, !LAZY);
@@ -388,7 +397,8 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
       for (int st = 1; st <= 3; ++st)
         rk3_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, st, a.it, o, o.kpt);
     } else {
-      be_droplet<PREC, ILIN, LAZY>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg);
+      be_droplet<PREC, ILIN, LAZY>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg, SERVE ? nullptr : a.flux,
+                                   mult);
     }
     if (SERVE && a.serve->last[home >> kServeIdxBits]) {
       const int64_t row = a.serve->out_off[home >> kServeIdxBits] + (home & kServeIdxMask);
@@ -455,7 +465,7 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
       store_rhs(a.cold, a.cap, home, o);
       if (a.istage == 1) store_old(a.cold, a.cap, home, o);
     } else {
-      be_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl);
+      be_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, -1, a.flux, a.src.mult[t]);
       store_old(a.cold, a.cap, home, o);
     }
     store_gathered(a.cold, a.cap, home, o);

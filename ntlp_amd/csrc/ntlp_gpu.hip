@@ -96,7 +96,15 @@ struct Ctx {
   // counters
   unsigned long long* tally = nullptr;        // kTallyRows per-step rows
   unsigned long long* tally_slots = nullptr;  // kTallySlots x kTallyWidth
+  unsigned long long* tally_totals = nullptr; // kTallyWidth running totals since the last reset
   int64_t step_row = 0;
+  // flux statistics of BE steps: two buffers of 3*(nnz+1) doubles, used by
+  // BE steps in turn (step j adds into flux[j%2]; its tally close clears the
+  // other one for step j+1)
+  double* flux = nullptr;
+  int64_t be_steps = 0;
+  // a failed multi-step call left the droplet state inconsistent
+  bool state_bad = false;
   // a serving loop is open on these fields
   bool serving = false;
   // timing
@@ -229,6 +237,8 @@ int need_fields() {
 int need_drops() {
   CHK(need_fields());
   if (ctx.n <= 0) return fail(NTLP_E_STATE, "no droplets: call ntlp_gpu_particles_put first");
+  if (ctx.state_bad)
+    return fail(NTLP_E_STATE, "a failed step call left the droplets inconsistent: put them again");
   return 0;
 }
 
@@ -264,6 +274,8 @@ void release_all() {
   dfree(ctx.xsrc);
   dfree(ctx.tally);
   dfree(ctx.tally_slots);
+  dfree(ctx.tally_totals);
+  dfree(ctx.flux);
   dfree(ctx.d_grid);
   dfree(ctx.d_consts);
   for (auto& t : ctx.pending) {
@@ -518,7 +530,13 @@ void launch_update_t(const StepArgs& a, unsigned grid) {
   k_update<MODE, PREC, ILIN><<<grid, kBlock, 0, ctx.stream>>>(a);
 }
 template <int MODE, int PREC, int ILIN>
-void launch_step_t(const StepArgs& a, unsigned grid) {
+void launch_step_t(const StepArgs& a, unsigned grid, bool lazy) {
+  if constexpr (MODE == 2) {
+    if (lazy) {
+      k_step<MODE, PREC, ILIN, false, true><<<grid, kBlock, 0, ctx.stream>>>(a);
+      return;
+    }
+  }
   k_step<MODE, PREC, ILIN><<<grid, kBlock, 0, ctx.stream>>>(a);
 }
 
@@ -538,7 +556,7 @@ int dispatch_update(const StepArgs& a, int prec) {
 // place, and the last step of a call that also writes the diagnostics
 // ("_last"), since their memory traffic differs.
 template <int MODE>
-int dispatch_step(const StepArgs& a, int prec) {
+int dispatch_step(const StepArgs& a, int prec, bool lazy) {
   const unsigned gr = grid8(a.n);
   const bool lin = ctx.fl.ilin == 1;
   static const char* const names[2][4] = {
@@ -546,18 +564,28 @@ int dispatch_step(const StepArgs& a, int prec) {
       {"step_be", "step_be_last", "step_be_sorted", "step_be_sorted_last"}};
   const char* name = names[MODE == 1 ? 0 : 1][(a.inplace ? 0 : 2) + (a.write_cold ? 1 : 0)];
   return launch(name, [&] {
-    if (prec == PREC_FP64) lin ? launch_step_t<MODE, 0, 1>(a, gr) : launch_step_t<MODE, 0, 0>(a, gr);
-    else if (prec == PREC_MIXED) lin ? launch_step_t<MODE, 1, 1>(a, gr) : launch_step_t<MODE, 1, 0>(a, gr);
-    else lin ? launch_step_t<MODE, 2, 1>(a, gr) : launch_step_t<MODE, 2, 0>(a, gr);
+    if (prec == PREC_FP64) lin ? launch_step_t<MODE, 0, 1>(a, gr, lazy) : launch_step_t<MODE, 0, 0>(a, gr, lazy);
+    else if (prec == PREC_MIXED) lin ? launch_step_t<MODE, 1, 1>(a, gr, lazy) : launch_step_t<MODE, 1, 0>(a, gr, lazy);
+    else lin ? launch_step_t<MODE, 2, 1>(a, gr, lazy) : launch_step_t<MODE, 2, 0>(a, gr, lazy);
   });
 }
 
-// closes the current step: slots -> a fresh per-step row
-int close_step() {
+int flux_len() { return 3 * (ctx.g.nnz + 1); }
+// flux statistics buffer of BE step j
+double* flux_buf(int64_t j) { return ctx.flux + (j % 2) * flux_len(); }
+
+// closes the current step: slots -> a fresh per-step row and the totals; a
+// BE step also clears the flux buffer of the next BE step
+int close_step(bool be) {
   ctx.step_row++;
   unsigned long long* row = ctx.tally + (ctx.step_row % kTallyRows) * kTallyWidth;
   unsigned long long* slots = ctx.tally_slots;
-  return launch("tally", [&] { k_tally_close<<<1, 128, 0, ctx.stream>>>(slots, row, 1); });
+  unsigned long long* totals = ctx.tally_totals;
+  double* next = be ? flux_buf(ctx.be_steps + 1) : nullptr;
+  const int nf = flux_len();
+  CHK(launch("tally", [&] { k_tally_close<<<1, 128, 0, ctx.stream>>>(slots, row, totals, 1, next, nf); }));
+  if (be) ctx.be_steps++;
+  return 0;
 }
 
 // nsteps fused steps: each reads the droplets through the cell sort, writes
@@ -567,8 +595,20 @@ int close_step() {
 // caller needs the cold columns).
 // Steps between two reorderings of the slots into cell order.
 constexpr int kReorderEvery = 16;
+// Calls of fewer BE steps update the residence times in the step kernel: the
+// settle pass reads and writes the 32 bytes per droplet that a lazy step
+// saves, so it pays from about two steps on.
+constexpr int kLazyMinSteps = 3;
 
-int run_fused(int mode, int prec, int nsteps, int it0, bool scatter) {
+// residence-time additions of the first `steps` steps of a lazy BE call
+int settle(int steps) {
+  double* hot = ctx.hot[ctx.cur];
+  const int64_t n = ctx.n;
+  const double dt = ctx.C.dt;
+  return launch("res_settle", [&] { k_res_settle<<<grid_flat(n), kBlock, 0, ctx.stream>>>(hot, n, steps, dt); });
+}
+
+int run_fused_steps(int mode, int prec, int nsteps, int it0, bool scatter, bool lazy, int& done) {
   const unsigned mask = mode == NTLP_MODE_RK3 ? kDiagRK3 : kDiagBE;
   if (ctx.diag_mask & ~mask) CHK(flush_diag());
   if (!ctx.perm_valid || !ctx.keys_valid) CHK(bin_current(nullptr, nullptr));
@@ -578,6 +618,7 @@ int run_fused(int mode, int prec, int nsteps, int it0, bool scatter) {
     a.it = it0 + k;
     a.write_cold = (k == nsteps - 1) ? 1 : 0;
     a.lazy_j = k;
+    if (mode == NTLP_MODE_BE) a.flux = flux_buf(ctx.be_steps);
     if (reorder) {  // read through the sort, write slot t of the other buffer
       a.perm = ctx.perm;
       a.keys_in = ctx.keys[ctx.cur];
@@ -588,8 +629,8 @@ int run_fused(int mode, int prec, int nsteps, int it0, bool scatter) {
       a.keys_in = ctx.keys[ctx.cur];
       a.keys = ctx.keys[ctx.cur];
     }
-    if (mode == NTLP_MODE_RK3) CHK(dispatch_step<1>(a, prec));
-    else CHK(dispatch_step<2>(a, prec));
+    if (mode == NTLP_MODE_RK3) CHK(dispatch_step<1>(a, prec, false));
+    else CHK(dispatch_step<2>(a, prec, lazy));
     if (reorder) {
       ctx.cur = 1 - ctx.cur;
       ctx.home_identity = false;
@@ -599,20 +640,35 @@ int run_fused(int mode, int prec, int nsteps, int it0, bool scatter) {
       ctx.since_reorder++;
     }
     ctx.keys_valid = true;
-    CHK(close_step());
+    done = k + 1;
+    CHK(close_step(mode == NTLP_MODE_BE));
     CHK(finish_bin(nullptr, false));
     if (scatter) CHK(tile_sorted());
   }
-  if (mode == NTLP_MODE_BE) {  // the call's residence-time additions
-    double* hot = ctx.hot[ctx.cur];
-    const int64_t n = ctx.n;
-    const double dt = ctx.C.dt;
-    CHK(launch("res_settle", [&] { k_res_settle<<<grid_flat(n), kBlock, 0, ctx.stream>>>(hot, n, nsteps, dt); }));
-  }
+  if (lazy) CHK(settle(nsteps));  // the call's residence-time additions
   ctx.diag_mask = mask;
   ctx.amounts_valid = true;
   HIPCHK(hipStreamSynchronize(ctx.stream));
   return 0;
+}
+
+// nsteps fused steps.  If a launch fails part way through a lazy BE call, the
+// residence times of the steps already made are settled (or, if that fails
+// too, the droplets are marked inconsistent until the next put).
+int run_fused(int mode, int prec, int nsteps, int it0, bool scatter) {
+  const bool lazy = mode == NTLP_MODE_BE && nsteps >= kLazyMinSteps;
+  int done = 0;
+  const int rc = run_fused_steps(mode, prec, nsteps, it0, scatter, lazy, done);
+  if (rc != 0 && lazy && done > 0) {
+    const std::string msg = g_err;
+    if (done == nsteps || settle(done) != 0 || hipStreamSynchronize(ctx.stream) != hipSuccess) ctx.state_bad = true;
+    g_err = msg;
+  }
+  if (rc != 0) {
+    ctx.perm_valid = ctx.keys_valid = ctx.amounts_valid = false;
+    ctx.diag_mask = 0;
+  }
+  return rc;
 }
 
 }  // namespace
@@ -713,6 +769,8 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   CHK(dalloc(ctx.src, 6 * ctx.nsrc, "coupling sources"));
   CHK(dalloc(ctx.tally, kTallyRows * kTallyWidth, "counters"));
   CHK(dalloc(ctx.tally_slots, kTallySlots * kTallyWidth, "counter slots"));
+  CHK(dalloc(ctx.tally_totals, kTallyWidth, "counter totals"));
+  CHK(dalloc(ctx.flux, 2 * 3 * (nz + 1), "flux statistics"));
   ctx.ext_elems = (int64_t)(nz + 2) * (grid->nny + 5) * (grid->nnx + 5);
   ctx.init = true;
   CHK(upload_consts());
@@ -720,6 +778,8 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   CHK(fill_f64(ctx.src, 6 * ctx.nsrc, 0.0));
   CHK(launch("fill", [&] { k_fill_u64<<<64, kBlock, 0, ctx.stream>>>(ctx.tally, kTallyRows * kTallyWidth); }));
   CHK(launch("fill", [&] { k_fill_u64<<<8, kBlock, 0, ctx.stream>>>(ctx.tally_slots, kTallySlots * kTallyWidth); }));
+  CHK(launch("fill", [&] { k_fill_u64<<<1, kBlock, 0, ctx.stream>>>(ctx.tally_totals, kTallyWidth); }));
+  CHK(fill_f64(ctx.flux, 2 * 3 * (nz + 1), 0.0));
   HIPCHK(hipStreamSynchronize(ctx.stream));
   return NTLP_OK;
 }
@@ -809,6 +869,7 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
   }
   ctx.n = n;
   ctx.cur = 0;
+  ctx.state_bad = false;
   ctx.home_identity = true;
   ctx.diag_mask = 0;
   ctx.amounts_valid = false;
@@ -842,6 +903,8 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
 int ntlp_gpu_particles_get(double* soa, int64_t* mult, int32_t* pidx, int32_t* procidx, int64_t n) {
   CHK(need_init());
   if (n != ctx.n) return fail(NTLP_E_INVALID, "n does not match the droplet count on the device");
+  if (ctx.state_bad)
+    return fail(NTLP_E_STATE, "a failed step call left the droplets inconsistent: put them again");
   if (soa) CHK(flush_diag());
   if (n == 0) return NTLP_OK;
   const int64_t cap = ctx.cap;
@@ -1329,30 +1392,41 @@ int ntlp_gpu_step(int32_t mode, int32_t prec, int32_t nsteps, int32_t it0) {
 
 int ntlp_gpu_counters(ntlp_counters* out, int32_t reset) {
   CHK(need_init());
-  std::vector<unsigned long long> rows((size_t)kTallyRows * kTallyWidth);
-  HIPCHK(hipMemcpyAsync(rows.data(), ctx.tally, sizeof(unsigned long long) * rows.size(), hipMemcpyDeviceToHost, ctx.stream));
+  unsigned long long row[kTallyWidth];
+  HIPCHK(hipMemcpyAsync(row, ctx.tally_totals, sizeof(row), hipMemcpyDeviceToHost, ctx.stream));
   HIPCHK(hipStreamSynchronize(ctx.stream));
   if (out) {
     memset(out, 0, sizeof(*out));
-    for (int64_t r = 0; r < kTallyRows; ++r) {
-      const unsigned long long* row = rows.data() + r * kTallyWidth;
-      out->num100 += row[kTNum100];
-      out->numimpos += row[kTNumImpos];
-      out->actnum += row[kTActnum];
-      out->denum += row[kTDenum];
-      out->gn_iters += row[kTGnIters];
-      out->lm_iters += row[kTLmIters];
-      out->solves += row[kTSolves];
-      out->steps += row[kTSteps];
-      for (int h = 0; h < NTLP_GN_HIST; ++h) out->gn_hist[h] += row[kTHist0 + h];
-    }
+    out->num100 = row[kTNum100];
+    out->numimpos = row[kTNumImpos];
+    out->actnum = row[kTActnum];
+    out->denum = row[kTDenum];
+    out->gn_iters = row[kTGnIters];
+    out->lm_iters = row[kTLmIters];
+    out->solves = row[kTSolves];
+    out->steps = row[kTSteps];
+    for (int h = 0; h < NTLP_GN_HIST; ++h) out->gn_hist[h] = row[kTHist0 + h];
   }
   if (reset) {
     CHK(launch("fill", [&] { k_fill_u64<<<64, kBlock, 0, ctx.stream>>>(ctx.tally, kTallyRows * kTallyWidth); }));
     CHK(launch("fill", [&] { k_fill_u64<<<8, kBlock, 0, ctx.stream>>>(ctx.tally_slots, kTallySlots * kTallyWidth); }));
+    CHK(launch("fill", [&] { k_fill_u64<<<1, kBlock, 0, ctx.stream>>>(ctx.tally_totals, kTallyWidth); }));
     HIPCHK(hipStreamSynchronize(ctx.stream));
     ctx.step_row = 0;
   }
+  return NTLP_OK;
+}
+
+int ntlp_gpu_be_flux(double* pflux, double* pmassflux, double* penegflux) {
+  CHK(need_init());
+  if (ctx.be_steps == 0) return fail(NTLP_E_STATE, "no backward-Euler step has run");
+  const int nlev = ctx.g.nnz + 1;
+  const double* f = flux_buf(ctx.be_steps - 1);
+  double* outs[3] = {pflux, pmassflux, penegflux};
+  for (int k = 0; k < 3; ++k)
+    if (outs[k])
+      HIPCHK(hipMemcpyAsync(outs[k], f + (int64_t)k * nlev, sizeof(double) * nlev, hipMemcpyDeviceToHost, ctx.stream));
+  HIPCHK(hipStreamSynchronize(ctx.stream));
   return NTLP_OK;
 }
 

@@ -27,8 +27,16 @@ struct Diag {
   double uf[3], Tf, qinf, qstar;
   double vold[3], Tpold, rold;
   double xrhs[3], vrhs[3], Tprhs_s, Tprhs_L, radrhs;
-  int kpt;  // u level of the gather position
+  // u level of the gather position; a BE step packs the first w level above
+  // it (the flux statistics' kc0) into the same register: see pack_levels
+  int kpt;
 };
+
+// kpt (-2..nnz+1) and kc0 (-1..nnz+1) of one droplet in one int (nnz + 2 <=
+// 1024 levels): one register held across the radius solve instead of two.
+__device__ __forceinline__ int pack_levels(int kpt, int kc0) { return (kpt + 2) | ((kc0 + 1) << 11); }
+__device__ __forceinline__ int packed_kpt(int v) { return (v & 2047) - 2; }
+__device__ __forceinline__ int packed_kc0(int v) { return (v >> 11) - 1; }
 
 // Per-thread Newton bookkeeping (particles.f90:3035-3039, 3128, 3147-3174).
 struct Tally {
@@ -555,7 +563,8 @@ __device__ __forceinline__ BeHand be_advance(const DevGrid& g, const DevConsts& 
   o.Tpold = d.Tp;
   o.rold = d.r;
   const Cell c = kguess >= 0 ? cell_near(g, s_z, s_zz, d.x, d.y, d.z, kguess) : cell_of(g, s_z, s_zz, d.x, d.y, d.z);
-  o.kpt = c.kpt;
+  // minloc(z, z > xp3i) - 1 of the pre-step height (flux statistics)
+  o.kpt = pack_levels(c.kpt, (c.kwpt == -2) ? -1 : c.kwpt + 1);
   Fluid f;
   if (ILIN != 1) f = gather6_f64(g, F, s_z, s_zz, d.x, d.y, d.z, c);
   else if (PREC == PREC_MIXED) f = gather_lin_mixed<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
@@ -615,12 +624,49 @@ __device__ __forceinline__ BeHand be_advance(const DevGrid& g, const DevConsts& 
   return h;
 }
 
-// Second half (3103-3206 and the wall reflection of particle_bcs_nonperiodic):
-// radius/temperature solve, q*, residence times, then walls at the new radius.
+// The flux statistics of particle_update_BE (3208-3238) for one droplet,
+// after its advance and radius/temperature update and before the walls.
+// With kc0 / kc1 the first w level above the pre-step / new height
+// (minloc(z, z > .) - 1; kc1 = nnz+1 above zl = z(nnz) and 0 below the
+// ground), a droplet that moved up adds mult, its mass rhop*Volp*mult and its
+// enthalpy rhop*Volp*Cpp*Tp*mult to each level iz = kc0..kc1-1, one that moved
+// down subtracts them from iz = kc1..kc0-1 (kc is monotone in the height, so
+// the direction is the sign of kc1 - kc0).  flux = pflux(0:nnz) |
+// pmassflux(0:nnz) | penegflux(0:nnz); levels outside 0..nnz are skipped.
+// Few droplets cross a level in one step, and they add with FP64 atomics.
+__device__ __forceinline__ void be_flux(const DevGrid& g, const DevConsts& C, const double* s_z, const Drop& d,
+                                        int kc0, int64_t mult, double* flux) {
+  const int len = g.nnz + 2, nlev = g.nnz + 1;
+  int kc1;
+  if (d.z > s_z[g.nnz]) kc1 = g.nnz + 1;
+  else if (d.z < 0.0) kc1 = 0;
+  else if (kc0 >= 1 && kc0 < len && s_z[kc0 - 1] <= d.z && d.z < s_z[kc0]) kc1 = kc0;
+  else {
+    const int p = first_gt(s_z, len, g.nsteps, d.z);
+    kc1 = (p == len) ? -1 : p;
+  }
+  if (kc1 == kc0) return;
+  const double r = d.r;
+  const double Volp = C.pi2 * 2.0 / 3.0 * (r * r * r);
+  const double rhop = (d.m_s + Volp * C.rhow) / Volp;
+  const double m = (double)mult;
+  const double sgn = kc1 > kc0 ? 1.0 : -1.0;
+  const int lo = kc1 > kc0 ? kc0 : kc1, hi = kc1 > kc0 ? kc1 : kc0;
+  for (int iz = max(lo, 0); iz < min(hi, nlev); ++iz) {
+    atomicAdd(&flux[iz], sgn * m);
+    atomicAdd(&flux[nlev + iz], sgn * (rhop * Volp * m));
+    atomicAdd(&flux[2 * nlev + iz], sgn * (rhop * Volp * C.Cpp * d.Tp * m));
+  }
+}
+
+// Second half (3103-3238 and the wall reflection of particle_bcs_nonperiodic):
+// radius/temperature solve, q*, residence times, flux statistics (flux not
+// null; levels: the packed levels be_advance left), then walls at the new
+// radius.
 template <int PREC, bool LAZY = false>
 __device__ __forceinline__ void be_finish(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
                                           const double* s_z, Drop& d, const BeHand& h, double& qstar,
-                                          Tally& tl) {
+                                          Tally& tl, int levels, double* flux = nullptr, int64_t mult = 1) {
   const double dt = C.dt;
   const double esa = (PREC == PREC_FP64) ? mod_magnus(h.Tf) : mod_magnus_fast(h.Tf);
   be_microphysics<PREC>(C, fl, d, h.Tf, h.qinf, h.rhoa, h.diffnorm, h.Rep, esa, tl);
@@ -629,16 +675,19 @@ __device__ __forceinline__ void be_finish(const DevGrid& g, const DevConsts& C, 
     d.res = d.res + dt;
     d.actres = d.actres + dt;
   }
+  if (flux) be_flux(g, C, s_z, d, packed_kc0(levels), mult, flux);
   apply_bcs(g, s_z, d);
 }
 
-// One droplet of particle_update_BE (3043-3206) plus its boundary conditions.
+// One droplet of particle_update_BE (3043-3238) plus its boundary conditions.
 template <int PREC, int ILIN, bool LAZY = false>
 __device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
                                            const DevFields& F, const double* s_z, const double* s_zz,
-                                           Drop& d, int it, Diag& o, Tally& tl, int kguess = -1) {
+                                           Drop& d, int it, Diag& o, Tally& tl, int kguess = -1,
+                                           double* flux = nullptr, int64_t mult = 1) {
   const BeHand h = be_advance<PREC, ILIN>(g, C, fl, F, s_z, s_zz, d, it, o, kguess);
-  be_finish<PREC, LAZY>(g, C, fl, s_z, d, h, o.qstar, tl);
+  be_finish<PREC, LAZY>(g, C, fl, s_z, d, h, o.qstar, tl, o.kpt, flux, mult);
+  o.kpt = packed_kpt(o.kpt);
 }
 
 // ---------------------------------------------------------- RK3 droplet

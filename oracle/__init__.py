@@ -73,6 +73,7 @@ def lib() -> ctypes.CDLL:
             ("or_crit_radius_many", None, [vp, vp, vp, vp, i64, vp]),
             ("or_update_rk3", None, [vp, vp, vp, vp, vp, i64, i32, i32]),
             ("or_update_be", None, [vp, vp, vp, vp, vp, i64, i32, vp]),
+            ("or_update_be_flux", None, [vp, vp, vp, vp, vp, vp, i64, i32, vp, vp]),
             ("or_coupling", None, [vp, vp, vp, vp, vp, i64, vp]),
             ("or_update_be_par", None, [vp, vp, vp, vp, vp, i64, i32, vp]),
             ("or_coupling_par", None, [vp, vp, vp, vp, vp, i64, vp]),
@@ -161,6 +162,17 @@ class Oracle:
         self.L.or_update_be(ctypes.byref(self.g), ctypes.byref(self.c), ctypes.byref(self.f), ctypes.byref(self.fl),
                             _p(soa), soa.shape[1], it, ctypes.byref(cnt))
         return {n: int(getattr(cnt, n)) for n, _ in _Cnt._fields_}
+
+    def update_be_flux(self, soa: np.ndarray, mult: np.ndarray, it: int):
+        """update_be plus the step's flux statistics: (counters, flux (3, nnz+1)
+        = pflux, pmassflux, penegflux over w levels 0..nnz)."""
+        cnt = _Cnt()
+        mult = np.ascontiguousarray(mult, dtype=np.int64)
+        flux = np.zeros((3, self.grid.nnz + 1))
+        self.L.or_update_be_flux(ctypes.byref(self.g), ctypes.byref(self.c), ctypes.byref(self.f),
+                                 ctypes.byref(self.fl), _p(soa), _p(mult), soa.shape[1], it, ctypes.byref(cnt),
+                                 _p(flux))
+        return {n: int(getattr(cnt, n)) for n, _ in _Cnt._fields_}, flux
 
     def update_be_par(self, soa: np.ndarray, it: int) -> dict:
         """update_be over the host's cores (OpenMP): CPU timing baseline."""

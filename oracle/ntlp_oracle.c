@@ -689,14 +689,63 @@ static void be_microphysics(const or_consts *c, const or_flags *f, double *p, do
   p[OR_ACTRES] = p[OR_ACTRES] + dt;
 }
 
-/* particle_update_BE (3000-3280) without the flux statistics (3209-3238), on
- * droplets i0 <= i < i1 of a SoA array with n droplets per column */
+/* The flux statistics of particle_update_BE (3208-3238): after the advance
+ * and the radius/temperature update, before the walls, a droplet that moved
+ * up from below w level z(iz) to above it adds mult, its mass rhop*Volp*mult
+ * and its enthalpy rhop*Volp*Cpp*Tp*mult to level iz, for iz = fluxloci ..
+ * fluxloc-1; moving down it subtracts them for iz = fluxloc .. fluxloci-1.
+ * fluxloc(i) = minloc(z, z > xp(3)) - 1 at the new (initial) height, with
+ * fluxloc = nnz+1 above zl and 0 below the ground; zl is the domain height
+ * z(nnz).  flux = pflux(0:nnz) | pmassflux(0:nnz) | penegflux(0:nnz); levels
+ * outside 0..nnz (the reference's pflux(-1) for a droplet at or above
+ * z(nnz+1) before the step) are not counted. */
+static void be_flux(const or_grid *g, const or_consts *c, const double *p, double xp3i, int64_t mult,
+                    double *flux) {
+  const int len = g->nnz + 2, nlev = g->nnz + 1;
+  const double zl = g->z[g->nnz];
+  int fluxloc, fluxloci;
+  if (p[OR_Z] > zl) {
+    fluxloc = g->nnz + 1;
+    fluxloci = minloc_gt(g->z, len, xp3i) - 1;
+  } else if (p[OR_Z] < 0.0) {
+    fluxloci = minloc_gt(g->z, len, xp3i) - 1;
+    fluxloc = 0;
+  } else {
+    fluxloc = minloc_gt(g->z, len, p[OR_Z]) - 1;
+    fluxloci = minloc_gt(g->z, len, xp3i) - 1;
+  }
+  const double r = p[OR_RADIUS];
+  const double Volp = c->pi2 * 2.0 / 3.0 * (r * r * r);
+  const double rhop = (p[OR_M_S] + Volp * c->rhow) / Volp;
+  const double m = (double)mult;
+  if (xp3i < p[OR_Z]) {
+    for (int iz = fluxloci; iz <= fluxloc - 1; ++iz) {
+      if (iz < 0 || iz >= nlev) continue;
+      flux[iz] = flux[iz] + m;
+      flux[nlev + iz] = flux[nlev + iz] + rhop * Volp * m;
+      flux[2 * nlev + iz] = flux[2 * nlev + iz] + rhop * Volp * c->Cpp * p[OR_TP] * m;
+    }
+  } else if (xp3i > p[OR_Z]) {
+    for (int iz = fluxloc; iz <= fluxloci - 1; ++iz) {
+      if (iz < 0 || iz >= nlev) continue;
+      flux[iz] = flux[iz] - m;
+      flux[nlev + iz] = flux[nlev + iz] - rhop * Volp * m;
+      flux[2 * nlev + iz] = flux[2 * nlev + iz] - rhop * Volp * c->Cpp * p[OR_TP] * m;
+    }
+  }
+}
+
+/* particle_update_BE (3000-3280) on droplets i0 <= i < i1 of a SoA array
+ * with n droplets per column; the flux statistics (3208-3238) are added to
+ * flux when it is not NULL (mult is then the droplets' multiplicity) */
 static void update_be_range(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
-                            double *soa, int64_t n, int64_t i0, int64_t i1, int it, or_counters *cnt) {
+                            double *soa, int64_t n, int64_t i0, int64_t i1, int it, or_counters *cnt,
+                            const int64_t *mult, double *flux) {
   const double dt = c->dt;
   double p[NR];
   for (int64_t i = i0; i < i1; ++i) {
     load_rec(soa, n, i, p);
+    const double xp3i = p[OR_Z];
     p[OR_VXOLD] = p[OR_VX]; p[OR_VYOLD] = p[OR_VY]; p[OR_VZOLD] = p[OR_VZ];
     p[OR_TPOLD] = p[OR_TP];
     p[OR_RADIUSOLD] = p[OR_RADIUS];
@@ -716,6 +765,7 @@ static void update_be_range(const or_grid *g, const or_consts *c, const or_flags
       p[OR_VX + d] = (p[OR_VX + d] + taup_i * dt * corrfac * p[OR_UFX + d] + dt * c->part_grav[d]) /
                      (1 + dt * corrfac * taup_i);
     be_microphysics(c, f, p, rhoa, Rep, rhop, taup_i, cnt);
+    if (flux) be_flux(g, c, p, xp3i, mult[i], flux);
     bcs(g, f, p);
     store_rec(soa, n, i, p);
   }
@@ -723,7 +773,14 @@ static void update_be_range(const or_grid *g, const or_consts *c, const or_flags
 
 void or_update_be(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
                   double *soa, int64_t n, int it, or_counters *cnt) {
-  update_be_range(g, c, f, fl, soa, n, 0, n, it, cnt);
+  update_be_range(g, c, f, fl, soa, n, 0, n, it, cnt, NULL, NULL);
+}
+
+void or_update_be_flux(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
+                       double *soa, const int64_t *mult, int64_t n, int it, or_counters *cnt, double *flux) {
+  const int nlev = g->nnz + 1;
+  for (int k = 0; k < 3 * nlev; ++k) flux[k] = 0.0; /* 3030-3032 */
+  update_be_range(g, c, f, fl, soa, n, 0, n, it, cnt, mult, flux);
 }
 
 /* The same over the host's cores (OpenMP; a CPU timing baseline, not a
@@ -736,7 +793,7 @@ void or_update_be_par(const or_grid *g, const or_consts *c, const or_flags *f, c
     or_counters local;
     memset(&local, 0, sizeof(local));
     const int64_t nt = omp_get_num_threads(), t = omp_get_thread_num();
-    update_be_range(g, c, f, fl, soa, n, n * t / nt, n * (t + 1) / nt, it, &local);
+    update_be_range(g, c, f, fl, soa, n, n * t / nt, n * (t + 1) / nt, it, &local, NULL, NULL);
 #pragma omp critical
     {
       int64_t *a = (int64_t *)cnt;

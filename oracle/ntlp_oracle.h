@@ -108,6 +108,11 @@ void or_update_rk3(const or_grid *g, const or_consts *c, const or_flags *f, cons
 /* one BE step incl. BCs (particle_update_BE, 3000-3280) */
 void or_update_be(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
                   double *soa, int64_t n, int it, or_counters *cnt);
+/* the same step with the flux statistics of 3030-3032, 3208-3238:
+ * flux[3][nnz+1] = pflux(0:nnz), pmassflux(0:nnz), penegflux(0:nnz), zeroed
+ * first; mult[n] the droplets' multiplicities */
+void or_update_be_flux(const or_grid *g, const or_consts *c, const or_flags *f, const or_fields *fl,
+                       double *soa, const int64_t *mult, int64_t n, int it, or_counters *cnt, double *flux);
 /* particle_coupling_update (1151-1276): src[6][(nnz+2)*(nny+1)*(nnx+1)] */
 void or_coupling(const or_grid *g, const or_consts *c, const or_flags *f, const double *soa,
                  const int64_t *mult, int64_t n, double *src);

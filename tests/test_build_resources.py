@@ -14,8 +14,8 @@ import pytest
 
 from ntlp_amd import build
 
-STEP_MIXED_BE = ("_ZN4ntlp6k_stepILi2ELi1ELi1ELb0EEEvNS_8StepArgsE",   # library step
-                 "_ZN4ntlp6k_stepILi2ELi1ELi1ELb1EEEvNS_8StepArgsE")   # serving step
+STEP_MIXED_BE = ("_ZN4ntlp6k_stepILi2ELi1ELi1ELb0ELb1EEEvNS_8StepArgsE",   # library step (multi-step calls)
+                 "_ZN4ntlp6k_stepILi2ELi1ELi1ELb1ELb0EEEvNS_8StepArgsE")   # serving step
 SCATTER_BOX = "_ZN4ntlp13k_scatter_boxILi8ELi8ELi8EEEvNS_8TileArgsE"
 
 

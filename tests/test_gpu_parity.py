@@ -168,15 +168,86 @@ def test_restart_file_roundtrip(tmp_path):
     s.close()
 
 
+COUNTERS = ("num100", "numimpos", "actnum", "denum", "solves")
+
+
+def add_counts(total, cnt):
+    for k in COUNTERS:
+        total[k] = total.get(k, 0) + cnt[k]
+    return total
+
+
+def _record_columns():
+    """Particles column -> (member of `type particle`, component)."""
+    m = {}
+    for vec, cols in (("xp", ("x", "y", "z")), ("vp", ("vx", "vy", "vz")), ("uf", ("ufx", "ufy", "ufz")),
+                      ("xrhs", ("xrhsx", "xrhsy", "xrhsz")), ("vrhs", ("vrhsx", "vrhsy", "vrhsz")),
+                      ("vp_old", ("vxold", "vyold", "vzold"))):
+        for k, c in enumerate(cols):
+            m[c] = (vec, k)
+    for c in ("Tp", "Tprhs_s", "Tprhs_L", "Tf", "radius", "radrhs", "qinf", "qstar", "res", "m_s", "kappa_s", "rc",
+              "actres", "numact"):
+        m[c] = (c, None)
+    m["Tpold"], m["radiusold"] = ("Tp_old", None), ("radius_old", None)
+    return m
+
+
+def test_restart_records_follow_the_type_particle_layout(tmp_path):
+    """A restart file whose 352-byte records come from a numpy dtype built from
+    the declaration of `type particle` (particles.f90:76-85, the MPI datatype
+    of 1929-1950: 4 int32, 39 real(8), 3 int64), with a distinct value in every
+    member (tests/golden/part_res_7.bin), read through ntlp_gpu_read_part_res:
+    every member lands in its column, the records outside 0 < x < xl,
+    0 < y < yl are dropped, and save_particles writes the kept records back
+    with every member at its declared offset."""
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_restart_fixture import PARTICLE
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "part_res_7.bin")
+    raw = np.fromfile(path, dtype=np.uint8)
+    n = int(raw[:4].view(np.int32)[0])
+    rec = raw[4:].view(PARTICLE)
+    assert n == 7 and len(rec) == 7
+    grid, consts, flags, fields, p, ora = small_case(seed=0, n=10)
+    s = make_step(grid, consts, flags, fields, p)
+    kept = s.read_part_res(path)
+    inside = (rec["xp"][:, 0] > 0) & (rec["xp"][:, 0] < grid.xl) & (rec["xp"][:, 1] > 0) & (rec["xp"][:, 1] < grid.yl)
+    assert kept == 5 == int(inside.sum())
+    got = s.get()
+    want = rec[inside]
+    for c, (member, k) in _record_columns().items():
+        col = want[member] if k is None else want[member][:, k]
+        assert np.array_equal(got[c], col), c
+    assert np.array_equal(got.mult, want["mult"]) and np.array_equal(got.pidx, want["pidx"])
+    assert np.array_equal(got.procidx, want["procidx"])
+    out = str(tmp_path / "again.res")
+    s.save_particles(out)
+    back = np.fromfile(out, dtype=np.uint8)
+    assert int(back[:4].view(np.int32)[0]) == 5
+    brec = back[4:].view(PARTICLE)
+    for member in PARTICLE.names:
+        if member in ("dist", "u_sub", "sigm_s", "prev", "next", "nbr_pidx", "nbr_procidx"):
+            continue  # members of the stochastic subgrid model and the list links are not kept
+        assert np.array_equal(brec[member], want[member]), member
+    s.close()
+
+
 @pytest.mark.parametrize("kind", ["uniform", "stretched"])
 def test_particle_update_be_fp64(kind):
     grid, consts, flags, fields, p, ora = small_case(seed=6, kind=kind)
     s = make_step(grid, consts, flags, fields, p)
     want = p.copy()
+    s.counters(reset=True)
+    cnt_o = {}
     for it in (1, 2):
         s.particle_update_BE(it=it)
-        cnt_o = ora.update_be(want.soa, it)
+        add_counts(cnt_o, ora.update_be(want.soa, it))
     got = s.get()
+    cnt = s.counters()
+    assert {k: cnt[k] for k in COUNTERS} == cnt_o
     bad = compare(got, want, ["x", "y", "z", "vx", "vy", "vz", "ufx", "ufy", "ufz", "Tf", "qinf", "Tp",
                               "radius", "qstar", "res", "actres", "numact"])
     assert not bad, bad
@@ -278,7 +349,10 @@ def test_fused_steps_match_single_steps(mode):
             for st in (1, 2, 3):
                 ora.update_rk3(want.soa, st, it)
     src_o = ora.coupling(want.soa, want.mult)
-    cols = ["x", "y", "z", "vx", "vy", "vz", "Tp", "radius", "res", "ufx", "Tf", "qinf", "qstar"]
+    cols = ["x", "y", "z", "vx", "vy", "vz", "Tp", "radius", "res", "actres", "numact", "ufx", "Tf", "qinf",
+            "qstar"]
+    if mode == MODE_BE:
+        assert want["numact"].sum() > 0  # an activation resets actres (particles.f90:3172)
     tol = dict(TOL64)
     tol["radius"] = (1e-8, 1e-6)
     bad = compare(got, want, cols, tol)
@@ -391,10 +465,11 @@ def test_fused_steps_across_reorderings_match_oracle():
     want = p.copy()
     for it in range(2, 22):
         ora.update_be(want.soa, it)
+    assert want["numact"].sum() > 0  # activations restart actres inside the lazy 12- and 8-step calls
     tol = dict(TOL64)
     tol.update({"x": (1e-12, 1.0), "y": (1e-12, 1.0), "z": (1e-12, 1.0), "vx": (1e-9, 0.1), "vy": (1e-9, 0.1),
                 "vz": (1e-9, 0.1), "Tp": (1e-10, 300.0), "radius": (1e-8, 1e-6)})
-    bad = compare(got, want, ["x", "y", "z", "vx", "vy", "vz", "Tp", "radius", "res", "numact"], tol)
+    bad = compare(got, want, ["x", "y", "z", "vx", "vy", "vz", "Tp", "radius", "res", "actres", "numact"], tol)
     assert not bad, bad
     assert np.array_equal(got.mult, p.mult) and np.array_equal(got.pidx, p.pidx)
     s.close()

@@ -490,34 +490,7 @@ def test_statistics_match_numpy_bincounts():
     assert np.allclose(lwp.ravel(), want_lwp, rtol=1e-12, atol=0)
 
 
-# ----------------------------------------------------------------- walls
-
-def test_boundary_conditions_hand_cases():
-    g = Grid.uniform(8, 8, 8, 0.02, 0.02, 0.02)
-    c = Consts(dt=1e-3, radius_init=1e-5)
-    o = ora_for(g, c, Flags(ievap=0))
-    zero = np.zeros((g.nnx, g.nny, g.nnz + 2))
-    o.set_fields(zero, zero, zero, zero + 300.0, zero + 0.01)
-    p = Particles.empty(4)
-    r = 1e-5
-    p["radius"], p["m_s"], p["Tp"] = r, 1e-18, 300.0
-    p["x"] = [0.05, g.xl - 1e-4, 1e-4, g.xl]
-    p["y"] = [0.05, 0.05, 0.05, 0.05]
-    p["z"] = [g.z[g.nnz] - 2 * r, 1.5 * r, 0.08, 0.08]
-    p["vx"] = [0.0, 1.0, -1.0, 0.0]
-    p["vz"] = [3.0, -3.0, 0.0, 0.0]
-    o.f.icouple = 0
-    o.update_be(p.soa, 2)
-    top = g.z[g.nnz] - r
-    # droplet 0 crossed the top wall: mirrored and vz negated
-    assert p["z"][0] < top and p["vz"][0] < 0
-    # droplet 1 crossed the bottom wall (z < r): mirrored about z = r
-    assert p["z"][1] > r and p["vz"][1] > 0
-    # droplets 1 and 2 wrapped through x = xl and x = 0
-    assert 0 <= p["x"][1] < g.xl / 2 and g.xl / 2 < p["x"][2] <= g.xl
-    # x == xl is not wrapped (strict >, particles.f90:2826) when it stays put
-    assert p["x"][3] == g.xl
-
+# --------------------------------------------------------------- parallel
 
 def test_parallel_oracle_matches_sequential():
     """The OpenMP oracle the bench times as its all-core CPU baseline computes
