@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -34,40 +35,59 @@ def log(*a):
 
 def kernel_bytes(name: str, n: int, grid, precision: int) -> float | None:
     """Algorithmic HBM bytes of one launch of a fused-step kernel over n
-    droplets (each byte the kernel must move once).
+    droplets (each byte the kernel must move once).  `am` is the size of one
+    coupling amount: 4 bytes where a mixed-precision step stores them, else 8.
 
-    step_be (in place): reads 12 of the 14 FP64 hot columns (the residence
-      times stay in HBM until the end of the call), mult (8) and the u-cell
-      key (4); writes the 8 columns that change every step (numact and actres
-      only when a droplet activates), the 6 FP64 coupling amounts and the new
-      key; reads once the gathered fields over the ext array (u, v, w in FP32
-      for mixed precision, else FP64; T, q FP64).
-    step_be_sorted (the step that reorders into cell order): reads also the
-      sort permutation (4) and home (4), writes all 14 hot columns, mult and
-      home into the other buffer.
+    step_be (in place, residence times left in HBM until the end of the call):
+      reads 12 of the 14 FP64 hot columns, mult (8) and the u-cell key (4);
+      writes the 8 columns that change every step (numact and actres only when
+      a droplet activates), the 6 coupling amounts and the new key; reads once
+      the gathered fields over the ext array (u, v, w in FP32 for mixed
+      precision, else FP64; T, q FP64).
+    step_rk3 (in place): reads all 14 hot columns, mult and the key; writes 10
+      columns (the 8 above and the residence times), the amounts and the key.
+    *_sorted (the step that reorders into cell order): reads also the sort
+      permutation (4) and home (4), writes all 14 hot columns, mult and home
+      into the other buffer.
     res_settle (once per call): reads and writes the two residence-time
       columns.
     *_last: the last step of a call also writes the 11 BE diagnostics (the
-      gathered fluid state and the pre-step values), 88 bytes per droplet.
+      gathered fluid state and the pre-step values), 88 bytes per droplet (RK3
+      adds the 9 stage registers).
     scatter: reads the sort permutation, the key, x, y, z and the 6 amounts of
       each droplet; writes each of the 6 FP64 source fields once.
     """
     ext = (grid.nnz + 2) * (grid.nny + 5) * (grid.nnx + 5)
     nodes = (grid.nnz + 2) * (grid.nny + 1) * (grid.nnx + 1)
     per_cell = (3 * 4 + 2 * 8) if precision == 1 else 5 * 8
-    if name.startswith("step_be"):
+    am = 4 if precision == 1 else 8
+    if name.startswith("step_be") or name.startswith("step_rk3"):
+        rk3 = name.startswith("step_rk3")
         if "_sorted" in name:
-            per = (4 + 14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4 + 6 * 8 + 4)
+            per = (4 + 14 * 8 + 8 + 4 + 4) + (14 * 8 + 8 + 4 + 6 * am + 4)
+        elif rk3:
+            per = (14 * 8 + 8 + 4) + (10 * 8 + 6 * am + 4)
         else:
-            per = (12 * 8 + 8 + 4) + (8 * 8 + 6 * 8 + 4)
+            per = (12 * 8 + 8 + 4) + (8 * 8 + 6 * am + 4)
         if name.endswith("_last"):
-            per += 11 * 8
+            per += (20 if rk3 else 11) * 8
         return float(n) * per + float(ext) * per_cell
     if name == "res_settle":
         return float(n) * 4 * 8
     if name == "scatter":
-        return float(n) * (4 + 4 + 3 * 8 + 6 * 8) + 6 * 8 * float(nodes)
+        return float(n) * (4 + 4 + 3 * 8 + 6 * am) + 6 * 8 * float(nodes)
     return None
+
+
+def cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
 
 
 def pmc_traffic(kernel: str, workload: str):
@@ -86,27 +106,35 @@ def pmc_traffic(kernel: str, workload: str):
 
 
 def prepare(wl_id: int, n: int | None, seed: int):
+    """The workload's grid, constants and fields (host); its droplets are drawn
+    on the device (synth.ensemble)."""
     from ntlp_amd import synth
 
     t = time.time()
-    wl = synth.workload(wl_id, n=n, seed=seed)
-    log(f"[bench] workload {wl_id}: {wl.name}; inputs built in {time.time() - t:.1f}s")
+    wl = synth.workload(wl_id, n=n, seed=seed, droplets=False)
+    log(f"[bench] workload {wl_id}: {wl.name}; fields built in {time.time() - t:.1f}s")
     return wl
 
 
-def init_device(wl):
-    from ntlp_amd import DropletStep
+def workload_size(wl_id: int, n: int | None) -> int:
+    return n if n is not None else {1: 100_000, 2: 2_000_000, 3: 20_000_000, 4: 2_000_000, 5: 100_000_000}[wl_id]
 
-    s = DropletStep(wl.grid, wl.consts, wl.flags)
+
+def init_device(wl, wl_id: int, n: int, seed: int, flags=None):
+    """Fields resident and n droplets of the workload's ensemble created on the
+    device (vp = uf, Tp = Tf, rc as create_particle sets them)."""
+    from ntlp_amd import DropletStep, synth
+
+    s = DropletStep(wl.grid, wl.consts, flags or wl.flags)
     s.set_fields(*wl.fields)
-    s.put(wl.droplets)
-    s.interp(set_vp=True, set_tp=True)  # vp = uf, Tp = Tf at creation
-    s.crit_radius()
+    s.generate(synth.ensemble(wl_id, seed=seed), n)
     return s
 
 
 def run_steps(s, init_state, mode, prec, steps, warmup, it0=2):
-    s.put(init_state)
+    """Times `steps` fused steps (one call) after `warmup` untimed ones, from
+    init_state (a Particles, or a (ensemble, n) pair drawn on the device)."""
+    restore(s, init_state)
     s.counters(reset=True)
     if warmup:
         s.step(mode=mode, precision=prec, nsteps=warmup, it0=it0)
@@ -116,6 +144,35 @@ def run_steps(s, init_state, mode, prec, steps, warmup, it0=2):
     s.sync()
     dt = time.perf_counter() - t0
     return dt
+
+
+def restore(s, init_state):
+    if isinstance(init_state, tuple):
+        s.generate(*init_state)
+    else:
+        s.put(init_state)
+
+
+def kernel_timing(s, init_state, mode, prec, steps, warmup):
+    """Per-kernel HIP-event times of the same steps (ms per launch)."""
+    restore(s, init_state)
+    if warmup:
+        s.step(mode=mode, precision=prec, nsteps=warmup, it0=2)
+    s.timing(True)
+    s.step(mode=mode, precision=prec, nsteps=steps, it0=2 + warmup)
+    kt = s.kernel_times()
+    s.timing(False)
+    return kt
+
+
+def roofline_of(kt, n, grid, prec):
+    per_kernel = {}
+    for k, (nl, tot) in kt.items():
+        b = kernel_bytes(k, n, grid, prec)
+        if b is not None:
+            per_kernel[k] = {"avg_ms": tot / nl, "launches": nl, "algorithmic_bytes": b,
+                             "GBps": b / (tot / nl * 1e-3) / 1e9, "frac_of_peak": b / (tot / nl * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return per_kernel
 
 
 def cpu_baseline(wl_id: int, seed: int, n_full: int, target_s: float = 8.0):
@@ -170,56 +227,37 @@ def cpu_baseline(wl_id: int, seed: int, n_full: int, target_s: float = 8.0):
 
 
 def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budget_bytes: int = 32 << 30,
-                n_templates: int = 4, nmax: int = 4_000_000):
+                nmax: int = 4_000_000):
     """The serving stream against resident 128^3 fields: Poisson arrivals at
     `rate` req/s for `duration_s`, request sizes log-uniform on [1e3, 4e6]
-    droplets, each request advancing `steps` mixed-precision BE steps.
-
-    Request payloads are windows (random offset, the request's size) of
-    `n_templates` droplet ensembles generated up front in pinned memory, each
-    with its own seed, radius median drawn from 2-20 um and aerosol median
-    from 0.02-0.2 um (generating every request in Python at 200 req/s would
-    measure numpy, not the server).  Results land in a pinned FIFO ring of
-    3 x 4M rows.  One loop serves three phases:
+    droplets, each request its own ensemble -- its own seed, radius median
+    drawn from 2-20 um (sigma_ln 0.4), aerosol median from 0.02-0.2 um
+    (sigma_ln 0.5) -- drawn on the device at admission
+    (ntlp_serve_submit_ensemble) and advancing `steps` mixed-precision BE
+    steps.  Results (14 columns and mult) land in a pinned FIFO ring of 3 x 4M
+    rows.  Phases:
 
       stream     Poisson arrivals; p50/p99 latency from submission to result
                  in host memory;
       saturated  a second set of requests all arriving at t = 0 (a closed
                  loop bounded by the result ring): sustained throughput;
-      static     the same requests in static batches (each filling the
-                 result ring, submitted together, all waited for before the
-                 next): the batched throughput the continuous loop is
-                 compared with.
-
-    The device-resident batched step (no host transfers) on the saturated
-    pool size is reported beside them: results of 120 bytes per droplet over
-    the host link bound any served throughput well below it."""
+      batched    the same requests through the plain entry points, in batches
+                 that fill the result ring: the batch's ensembles generated
+                 together (ntlp_gpu_particles_generate), `steps` fused steps,
+                 the 14 result columns and mult copied into the same pinned
+                 ring -- the comparator of the sustained throughput.
+    """
+    import ctypes
     import queue
     import threading
 
-    from ntlp_amd import (MODE_BE, PREC_MIXED, DropletStep, Flags, Particles, PinnedArray, ServeLoop, SERVE_IN,
-                          synth)
+    from ntlp_amd import (COL, MODE_BE, PREC_MIXED, SERVE_OUT, DropletStep, Ensemble, Flags, PinnedArray, ServeLoop,
+                          SALT_DRY_LOGNORMAL, synth)
 
-    wl = synth.workload(4, n=1, seed=seed)
+    wl = synth.workload(4, n=1, seed=seed, droplets=False)
     flags = Flags(icouple=0, iTcouple=0, iHcouple=0)
     s = DropletStep(wl.grid, wl.consts, flags)
     s.set_fields(*wl.fields)
-    g = wl.grid
-    t = time.time()
-    bank = PinnedArray((n_templates, 6, nmax))
-    bmult = PinnedArray((n_templates, nmax), np.int64)
-    for k in range(n_templates):
-        rng = np.random.default_rng(1000 * seed + k)
-        r_med, a_med = rng.uniform(2e-6, 20e-6), rng.uniform(0.02e-6, 0.2e-6)
-        b = bank.array[k]
-        b[0] = rng.uniform(0, g.xl, nmax)
-        b[1] = rng.uniform(0, g.yl, nmax)
-        b[2] = rng.uniform(0, g.zl, nmax)
-        b[3] = synth.lognormal(rng, nmax, r_med, 0.4)
-        b[4] = synth.dry_mass(synth.lognormal(rng, nmax, a_med, 0.5))
-        b[5] = 1.2
-        bmult.array[k] = 1
-    log(f"[serve] request bank: {n_templates} ensembles x {nmax} droplets in {time.time() - t:.1f}s")
     ring_rows = 3 * nmax
     outbuf = PinnedArray((14, ring_rows))
     outmult = PinnedArray((ring_rows,), np.int64)
@@ -243,15 +281,21 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
                 self.blocks.pop(0)
                 self.cv.notify_all()
 
+    rng = np.random.default_rng(seed + 77)
+
+    def draw(k, n0):
+        sz = np.exp(rng.uniform(np.log(1e3), np.log(4e6), k)).astype(np.int64)
+        ens = [Ensemble(seed=1_000_003 * seed + n0 + i, radius_p=(rng.uniform(2e-6, 20e-6), 0.4),
+                        salt_kind=SALT_DRY_LOGNORMAL, salt_p=(rng.uniform(0.02e-6, 0.2e-6), 0.5)) for i in range(k)]
+        return sz, ens
+
     loop = ServeLoop(s, budget_bytes, steps=steps, precision=PREC_MIXED)
 
-    def submit(ring, i, n, o):
-        k = i % n_templates
+    def submit(ring, e, n):
         row = ring.take(n)
-        return loop.submit(bank.array[k][:, o:o + n], bmult.array[k][o:o + n], outbuf.array[:, row:row + n],
-                           outmult.array[row:row + n])
+        return loop.submit_ensemble(e, n, outbuf.array[:, row:row + n], outmult.array[row:row + n])
 
-    def run_stream(arrivals, sizes, offs):
+    def run_stream(arrivals, sizes, ens):
         ring = OutRing()
         done_q = queue.Queue()
         lat, qd, comp = [], [], []
@@ -270,43 +314,23 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
         th = threading.Thread(target=waiter)
         th.start()
         t0 = time.perf_counter()
-        for i, (ta, n, o) in enumerate(zip(arrivals, sizes, offs)):
+        for ta, n, e in zip(arrivals, sizes, ens):
             now = time.perf_counter() - t0
             if ta > now:
                 time.sleep(ta - now)
-            done_q.put(submit(ring, i, int(n), int(o)))
+            done_q.put(submit(ring, e, int(n)))
         done_q.put(None)
         th.join()
         return np.array(lat), np.array(qd), t0, max(comp)
 
-    def run_static(sizes, offs):
-        ring = OutRing()
-        t0 = time.perf_counter()
-        i, lat = 0, []
-        while i < len(sizes):
-            batch, rows = [], 0
-            while i < len(sizes) and rows + sizes[i] <= ring_rows:
-                batch.append(submit(ring, i, int(sizes[i]), int(offs[i])))
-                rows += int(sizes[i])
-                i += 1
-            for tk in batch:
-                lat.append(loop.wait(tk)[0])
-                ring.give()
-        return np.array(lat), t0, time.perf_counter()
-
     def delta(a, b):
         return {k: b[k] - a[k] for k in ("iterations", "droplet_steps", "graph_launches")}
 
-    def draw(n):
-        sz = np.exp(rng.uniform(np.log(1e3), np.log(4e6), n)).astype(np.int64)
-        return sz, np.array([rng.integers(0, nmax - m + 1) for m in sz])
-
-    rng = np.random.default_rng(seed + 77)
     nreq = max(1, int(rate * duration_s))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
-    sizes, offs = draw(nreq)
+    sizes, ens = draw(nreq, 0)
     st0 = loop.stats()
-    lat, qd, t0, t1 = run_stream(arrivals, sizes, offs)
+    lat, qd, t0, t1 = run_stream(arrivals, sizes, ens)
     st1 = loop.stats()
     d = delta(st0, st1)
     work = float(sizes.sum()) * steps
@@ -319,51 +343,92 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
         f"{stream['throughput_du_s']:.3g} du/s")
 
     nsat = max(16, nreq // 4)
-    sizes_s, offs_s = draw(nsat)
-    lat_s, _, t0, t1 = run_stream(np.zeros(nsat), sizes_s, offs_s)
+    sizes_s, ens_s = draw(nsat, 10_000_000)
+    lat_s, _, t0, t1 = run_stream(np.zeros(nsat), sizes_s, ens_s)
     st2 = loop.stats()
     d = delta(st1, st2)
     sat_tp = float(sizes_s.sum()) * steps / (t1 - t0)
     mean_pool = d["droplet_steps"] / max(d["iterations"], 1)
-    lat_b, t0, t1 = run_static(sizes_s, offs_s)
-    st3 = loop.stats()
-    static_tp = float(sizes_s.sum()) * steps / (t1 - t0)
     tot = loop.close()
-    log(f"[serve] saturated {sat_tp:.3g} du/s (mean pool {mean_pool:.3g}) vs static batches {static_tp:.3g} du/s")
+    log(f"[serve] saturated {sat_tp:.3g} du/s (mean pool {mean_pool:.3g}); engine thread (ms) {tot['engine_ms']}")
 
-    # the device-resident batched step on a pool of the saturated size
-    nb = int(min(max(mean_pool, 1e4), nmax))
-    p = Particles.empty(nb)
-    for c, name in enumerate(SERVE_IN):
-        p[name] = bank.array[0][c, :nb]
-    s.put(p)
-    s.interp(set_vp=True, set_tp=True)
-    s.crit_radius()
-    s.step(MODE_BE, PREC_MIXED, nsteps=2, it0=2)
-    s.sync()
+    # the same requests through the plain entry points, in batches filling the ring
+    lib = s.lib
+    cols = [COL[c] for c in SERVE_OUT]
     tb = time.perf_counter()
-    s.step(MODE_BE, PREC_MIXED, nsteps=steps, it0=4)
-    s.sync()
-    device = nb * steps / (time.perf_counter() - tb)
+    i, nb = 0, 0
+    while i < nsat:
+        j, rows = i, 0
+        while j < nsat and rows + sizes_s[j] <= ring_rows:
+            rows += int(sizes_s[j])
+            j += 1
+        s.generate(ens_s[i:j], sizes_s[i:j])
+        s.step(MODE_BE, PREC_MIXED, nsteps=steps, it0=2)
+        for k, c in enumerate(cols):
+            s._check(lib.ntlp_gpu_particles_get_column(c, outbuf.array[k].ctypes.data_as(ctypes.c_void_p), rows))
+        s._check(lib.ntlp_gpu_particles_get(None, outmult.array.ctypes.data_as(ctypes.c_void_p), None, None, rows))
+        i, nb = j, nb + 1
+    batched = float(sizes_s.sum()) * steps / (time.perf_counter() - tb)
+    log(f"[serve] plain-path batches {batched:.3g} du/s ({nb} batches)")
     saturated = {"requests": int(nsat), "throughput_du_s": sat_tp, "mean_pool": mean_pool,
                  "iterations": d["iterations"],
                  "p50_ms": float(np.percentile(lat_s, 50)), "p99_ms": float(np.percentile(lat_s, 99)),
-                 "static_batches_du_s": static_tp, "static_p50_ms": float(np.percentile(lat_b, 50)),
-                 "static_p99_ms": float(np.percentile(lat_b, 99)),
-                 "sustained_over_batched": sat_tp / static_tp,
-                 "device_batched_du_s": device, "device_batched_droplets": nb,
-                 "sustained_over_device_batched": sat_tp / device,
+                 "batched_du_s": batched, "batches": nb,
+                 "sustained_over_batched": sat_tp / batched,
                  "host_link_bound_du_s": steps * 56e9 / (14 * 8 + 8)}
-    log(f"[serve] device-resident batched {device:.3g} du/s on {nb} droplets; engine thread (ms) {tot['engine_ms']}")
     s.close()
-    bank.free()
-    bmult.free()
     outbuf.free()
     outmult.free()
-    return {"workload": "128^3 resident fields, Poisson arrivals, sizes log-uniform 1e3-4e6, "
+    return {"workload": "128^3 resident fields, Poisson arrivals, sizes log-uniform 1e3-4e6, each request its own "
+                        f"seed, radius median U(2,20) um and aerosol median U(0.02,0.2) um drawn on the device, "
                         f"{steps} BE steps each, mixed precision, HBM budget {budget_bytes >> 30} GiB",
             "stream": stream, "saturated": saturated, "graphs": tot["graphs"], "capacity": tot["capacity"],
             "newton": tot["newton"]}
+
+
+def rk3_bench(seed: int, steps: int = 10, warmup: int = 2):
+    """RK3 (ipart_method = 1, three gather + advance stages per step) on the
+    128^3 workload's 2M droplets, FP64 and mixed, at dt = 2e-5 s: at the
+    workload's own 5e-4 s the explicit drag of droplets below ~4 um has
+    dt/tau_p > 2.5 and the scheme diverges (in the reference as here), so the
+    throughput is measured where RK3 is stable for the whole population."""
+    from ntlp_amd import MODE_RK3, PREC_FP64, PREC_MIXED, synth
+
+    wl = prepare(2, None, seed)
+    n = workload_size(2, None)
+    s = init_device(wl, 2, n, seed)
+    s.set_dt(2e-5)
+    init = (synth.ensemble(2, seed=seed), n)
+    out = {"workload": f"{wl.name} at dt = 2e-5 s (RK3 stable for every droplet)", "droplets": n}
+    for prec, nm in ((PREC_MIXED, "mixed"), (PREC_FP64, "fp64")):
+        t = run_steps(s, init, MODE_RK3, prec, steps, warmup)
+        kt = kernel_timing(s, init, MODE_RK3, prec, steps, warmup)
+        out[nm] = {"value": n * steps / t, "ms_per_step": 1e3 * t / steps,
+                   "kernels": roofline_of(kt, n, wl.grid, prec)}
+        log(f"[rk3] {nm}: {n * steps / t:.3g} du/s, " + ", ".join(f"{k}={v[1] / v[0]:.3f}ms" for k, v in kt.items()))
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    s.close()
+    return out
+
+
+def config5_bench(seed: int, steps: int = 20, warmup: int = 2):
+    """The large single-GPU case: 512x512x256, 100M droplets (lognormal 10 um,
+    trimodal salt) drawn on the device, `steps` mixed BE steps with coupling,
+    HBM GB/s per kernel."""
+    from ntlp_amd import MODE_BE, PREC_MIXED, synth
+
+    wl = prepare(5, None, seed)
+    n = workload_size(5, None)
+    s = init_device(wl, 5, n, seed)
+    init = (synth.ensemble(5, seed=seed), n)
+    t = run_steps(s, init, MODE_BE, PREC_MIXED, steps, warmup)
+    kt = kernel_timing(s, init, MODE_BE, PREC_MIXED, steps, warmup)
+    s.close()
+    out = {"workload": wl.name, "droplets": n, "steps": steps, "value": n * steps / t,
+           "ms_per_step": 1e3 * t / steps, "kernels": roofline_of(kt, n, wl.grid, PREC_MIXED)}
+    log(f"[config5] {out['value']:.3g} du/s, {out['ms_per_step']:.2f} ms/step; " +
+        ", ".join(f"{k}={v['GBps']:.0f}GB/s" for k, v in out["kernels"].items()))
+    return out
 
 
 def main():
@@ -377,6 +442,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-rk3", action="store_true")
+    ap.add_argument("--no-config5", action="store_true")
+    ap.add_argument("--no-onestep", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -400,49 +468,36 @@ def main():
               "droplets with an all-reduce of the coupling fields; that path is not built).", file=sys.stderr)
         sys.exit(2)
 
-    from ntlp_amd import MODE_BE, PREC_FP64, PREC_MIXED
+    from ntlp_amd import MODE_BE, PREC_FP64, PREC_MIXED, synth
 
     if args.serve_only:
         print(json.dumps(serve_bench(args.seed, args.serve_seconds, args.serve_rate)), flush=True)
         return
 
     wl = prepare(args.workload, args.droplets, args.seed)
-    s = init_device(wl)
-    init_state = s.get()
-    n = init_state.n
+    n = workload_size(args.workload, args.droplets)
+    s = init_device(wl, args.workload, n, args.seed)
+    init = (synth.ensemble(args.workload, seed=args.seed), n)
     g = wl.grid
-    log(f"[bench] {n} droplets on {g.nnx}x{g.nny}x{g.nnz}; device ready")
+    log(f"[bench] {n} droplets on {g.nnx}x{g.nny}x{g.nnz} drawn on the device")
 
     # mixed precision: the headline
-    t_mixed = run_steps(s, init_state, MODE_BE, PREC_MIXED, args.steps, args.warmup)
+    t_mixed = run_steps(s, init, MODE_BE, PREC_MIXED, args.steps, args.warmup)
     cnt_mixed = s.counters()
     rows = s.step_counters()
-    end_mixed = s.get()
+    end_mixed = {c: s.column(c) for c in ("radius", "Tp", "z")}
     log(f"[bench] mixed: {args.steps} steps in {t_mixed:.3f}s")
-
-    # same steps with per-kernel HIP-event timing for the roofline
-    s.put(init_state)
-    if args.warmup:
-        s.step(mode=MODE_BE, precision=PREC_MIXED, nsteps=args.warmup, it0=2)
-    s.timing(True)
-    s.step(mode=MODE_BE, precision=PREC_MIXED, nsteps=args.steps, it0=2 + args.warmup)
-    ktimes = s.kernel_times()
-    s.timing(False)
+    ktimes = kernel_timing(s, init, MODE_BE, PREC_MIXED, args.steps, args.warmup)
     log("[bench] kernel times (mixed): " + ", ".join(f"{k}={v[1] / v[0]:.3f}ms x{v[0]}" for k, v in ktimes.items()))
 
     fp64 = None
+    accuracy = None
     if not args.no_fp64:
-        t_fp64 = run_steps(s, init_state, MODE_BE, PREC_FP64, args.steps, args.warmup)
+        t_fp64 = run_steps(s, init, MODE_BE, PREC_FP64, args.steps, args.warmup)
         cnt_fp64 = s.counters()
-        end_fp64 = s.get()
+        end_fp64 = {c: s.column(c) for c in ("radius", "Tp", "z")}
         log(f"[bench] fp64: {args.steps} steps in {t_fp64:.3f}s")
-        s.put(init_state)
-        if args.warmup:
-            s.step(mode=MODE_BE, precision=PREC_FP64, nsteps=args.warmup, it0=2)
-        s.timing(True)
-        s.step(mode=MODE_BE, precision=PREC_FP64, nsteps=args.steps, it0=2 + args.warmup)
-        ktimes64 = s.kernel_times()
-        s.timing(False)
+        ktimes64 = kernel_timing(s, init, MODE_BE, PREC_FP64, args.steps, args.warmup)
         log("[bench] kernel times (fp64): " + ", ".join(f"{k}={v[1] / v[0]:.3f}ms x{v[0]}" for k, v in ktimes64.items()))
         rr = np.abs(end_mixed["radius"] / end_fp64["radius"] - 1.0)
         fp64 = {
@@ -455,18 +510,30 @@ def main():
                     "radius_p99_rel_err_mixed_vs_fp64": float(np.quantile(rr, 0.99)),
                     "Tp_max_abs_err_K": float(np.abs(end_mixed["Tp"] - end_fp64["Tp"]).max()),
                     "z_max_abs_err_m": float(np.abs(end_mixed["z"] - end_fp64["z"]).max())}
-    else:
-        accuracy = None
+
+    # the drop-in path: one particle_update_BE (+ particle_coupling_update) call per step
+    onestep = None
+    if not args.no_onestep:
+        onestep = {}
+        for prec, nm in ((PREC_MIXED, "mixed"), (PREC_FP64, "fp64")):
+            restore(s, init)
+            s.particle_update_BE(it=2, precision=prec)
+            s.sync()
+            k1 = 10
+            t0 = time.perf_counter()
+            for k in range(k1):
+                s.particle_update_BE(it=3 + k, precision=prec)
+                s.particle_coupling_update(deterministic=0, out=False)
+            s.sync()
+            t1 = (time.perf_counter() - t0) / k1
+            onestep[nm] = {"ms_per_call": 1e3 * t1, "value": n / t1}
+            log(f"[bench] one-step calls ({nm}): {1e3 * t1:.3f} ms per particle_update_BE + coupling")
 
     # roofline of the dominant kernel (mixed precision): algorithmic bytes per
     # launch over its average HIP-event duration on the library's stream
     dom_name, (dom_launches, dom_ms) = max(ktimes.items(), key=lambda kv: kv[1][1])
     avg_ms = dom_ms / dom_launches
-    per_kernel = {}
-    for k, (nl, tot) in ktimes.items():
-        b = kernel_bytes(k, n, g, PREC_MIXED)
-        if b is not None:
-            per_kernel[k] = {"avg_ms": tot / nl, "algorithmic_bytes": b, "GBps": b / (tot / nl * 1e-3) / 1e9}
+    per_kernel = roofline_of(ktimes, n, g, PREC_MIXED)
     traffic_alg = kernel_bytes(dom_name, n, g, PREC_MIXED)
     roofline = {"kernel": dom_name, "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                 "avg_launch_ms": avg_ms, "launches": dom_launches, "achieved": None, "frac": None,
@@ -482,6 +549,10 @@ def main():
         if roofline["achieved"]:
             roofline["frac_of_copy"] = roofline["achieved"] / copy_gbps
     roofline["kernels"] = per_kernel
+    s.close()
+
+    rk3 = None if args.no_rk3 else rk3_bench(args.seed)
+    config5 = None if args.no_config5 else config5_bench(args.seed)
 
     cpu = None
     if not args.no_cpu:
@@ -491,12 +562,13 @@ def main():
 
     serve = None
     if not args.no_serve:
-        s.close()
         log("[bench] serving stream ...")
         serve = serve_bench(args.seed, args.serve_seconds, args.serve_rate)
 
     value = n * args.steps / t_mixed
     per_step_iters = (rows[:, 4] / np.maximum(rows[:, 6], 1)).tolist() if len(rows) else []
+    hist = rows[-args.steps:, 8:8 + 64] if len(rows) else np.zeros((0, 64), dtype=np.int64)
+    last = int(np.max(np.nonzero(hist.sum(axis=0))[0])) + 1 if hist.size and hist.sum() else 1
     out = {
         "metric": "droplet-updates/s per RK3 step, FP64 vs mixed; serve p50/p99 latency",
         "value": value,
@@ -509,7 +581,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64+f32",
-        "data": "synthetic",
+        "data": "synthetic (fields on the host, droplets drawn on the device)",
         "config": {"workload": wl.name, "droplets": n, "grid": f"{g.nnx}x{g.nny}x{g.nnz}",
                    "mode": "BE (ipart_method=2)", "precision": "mixed", "dt": wl.consts.dt,
                    "coupling": "on", "parallelism": "1 GPU"},
@@ -519,9 +591,15 @@ def main():
         "newton": {"iters_per_solve": cnt_mixed["gn_iters"] / max(cnt_mixed["solves"], 1),
                    "lm_fallbacks": cnt_mixed["num100"], "failed_solves": cnt_mixed["numimpos"],
                    "activations": cnt_mixed["actnum"], "deactivations": cnt_mixed["denum"],
-                   "iters_per_solve_per_step": per_step_iters[-args.steps:]},
+                   "iters_per_solve_per_step": per_step_iters[-args.steps:],
+                   "gn_iteration_histogram_per_step": {"columns": "solves whose Gauss-Newton ran k iterations, "
+                                                                  "k = 0 (fell back to LM), 1, 2, ...",
+                                                       "rows": hist[:, :last].tolist()}},
+        "onestep_calls": onestep,
         "kernel_ms": {k: v[1] / v[0] for k, v in ktimes.items()},
         "roofline": roofline,
+        "rk3": rk3,
+        "config5": config5,
         "cpu_baseline": cpu,
         "serve": serve,
     }

@@ -143,6 +143,43 @@ int ntlp_gpu_particles_get(double *real_soa, int64_t *mult, int32_t *pidx, int32
 int ntlp_gpu_particles_put_records(const void *records, int64_t n);
 int ntlp_gpu_particles_get_records(void *records, int64_t n);
 int64_t ntlp_gpu_particles_count(void);
+/* One column (enum ntlp_column) of every droplet, in put order. */
+int ntlp_gpu_particles_get_column(int32_t column, double *out, int64_t n);
+
+/* A droplet ensemble drawn on the device.  It has no counterpart in the
+ * reference, whose particle_init/create_particle (particles.f90:2226-2300,
+ * 2440-2520) draw droplets one by one on the host with ran2/gasdev.  Droplet i
+ * takes its random numbers from the counter-based generator Philox4x32-10
+ * keyed by `seed` at counter first + i, so an ensemble is the same however it
+ * is split.  Position: uniform over [0,xl) x [0,yl), height r + u (z(nnz) - 2r).
+ * Radius: radius_kind with radius_p, or with probability frac2 radius2_kind
+ * with radius2_p; NTLP_DIST_LOGNORMAL: p = (median, sigma_ln);
+ * NTLP_DIST_LOGUNIFORM: p = (lo, hi).  Salt: NTLP_SALT_FIXED m_s = salt_p[0];
+ * NTLP_SALT_DRY_LOGNORMAL: dry radius lognormal (median salt_p[0], sigma_ln
+ * salt_p[1]), m_s = rhos (4/3) pi r_d^3 (particles.f90:2498-2499);
+ * NTLP_SALT_CHOICE3: one of salt_p[0..2] with equal probability.  Then, as
+ * create_particle does: vp = uf(xp), Tp = Tf(xp), rc = crit_radius; res,
+ * actres, numact and the other record members zero; pidx = first + i + 1. */
+#define NTLP_DIST_LOGNORMAL 0
+#define NTLP_DIST_LOGUNIFORM 1
+#define NTLP_SALT_FIXED 0
+#define NTLP_SALT_DRY_LOGNORMAL 1
+#define NTLP_SALT_CHOICE3 2
+typedef struct ntlp_ensemble {
+  uint64_t seed;
+  int32_t radius_kind, radius2_kind;
+  double radius_p[2], radius2_p[2], frac2;
+  int32_t salt_kind, reserved;
+  double salt_p[3];
+  double kappa_s;
+  int64_t mult;
+  int64_t first;
+} ntlp_ensemble;
+/* Replaces the droplets on the device by counts[0] droplets of ens[0], then
+ * counts[1] of ens[1], ... (nens <= NTLP_MAX_ENSEMBLES; the fields must be
+ * set). */
+#define NTLP_MAX_ENSEMBLES 65536
+int ntlp_gpu_particles_generate(const ntlp_ensemble *ens, const int64_t *counts, int32_t nens);
 /* save_particles / read_part_res (particles.f90:1988-2146) for one rank: the
  * file is an int32 droplet count followed by one 352-byte record per droplet.
  * Reading keeps the records with 0 < x < xl and 0 < y < yl (the reference's
@@ -315,6 +352,11 @@ int ntlp_serve_open(const ntlp_serve_config *cfg, int64_t *capacity);
  * out (0 means n). */
 int ntlp_serve_submit(const double *in, int64_t ld_in, const int64_t *in_mult, int64_t n, double *out,
                       int64_t ld_out, int64_t *out_mult, int64_t *ticket);
+/* Queues a request whose n droplets are drawn on the device from `ens`
+ * (ntlp_gpu_particles_generate's ensemble, created at admission): nothing is
+ * uploaded.  out / out_mult as for ntlp_serve_submit. */
+int ntlp_serve_submit_ensemble(const ntlp_ensemble *ens, int64_t n, double *out, int64_t ld_out, int64_t *out_mult,
+                               int64_t *ticket);
 /* 1 when the ticket's result is in place, 0 while it is queued or running. */
 int ntlp_serve_poll(int64_t ticket);
 /* Blocks until the result is in place; times_ms (may be NULL) receives the

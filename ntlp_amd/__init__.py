@@ -21,7 +21,8 @@ import numpy as np
 from . import grid as _grid  # noqa: F401  (re-exported helpers)
 from .grid import Grid
 
-__all__ = ["Grid", "Consts", "Flags", "Particles", "DropletStep", "ServeLoop", "PinnedArray", "NtlpError", "load_library",
+__all__ = ["Grid", "Consts", "Flags", "Particles", "Ensemble", "DropletStep", "ServeLoop", "PinnedArray", "NtlpError",
+           "load_library", "DIST_LOGNORMAL", "DIST_LOGUNIFORM", "SALT_FIXED", "SALT_DRY_LOGNORMAL", "SALT_CHOICE3",
            "NREAL", "COL", "MODE_RK3", "MODE_BE", "PREC_FP64", "PREC_MIXED", "PREC_FP64_AJ",
            "LAYOUT_EXT", "LAYOUT_CORE", "LAYOUT_XPENCIL", "LIB_PATH"]
 
@@ -40,6 +41,8 @@ PREC_FP64, PREC_MIXED, PREC_FP64_AJ = 0, 1, 2
 LAYOUT_EXT, LAYOUT_CORE, LAYOUT_XPENCIL = 0, 1, 2
 GN_HIST = 64
 STEP_ROW = 72
+DIST_LOGNORMAL, DIST_LOGUNIFORM = 0, 1
+SALT_FIXED, SALT_DRY_LOGNORMAL, SALT_CHOICE3 = 0, 1, 2
 
 NTLP_ERRORS = {-1: "NTLP_E_INVALID", -2: "NTLP_E_NOT_INIT", -3: "NTLP_E_HIP", -4: "NTLP_E_NOMEM",
                -5: "NTLP_E_RANGE", -6: "NTLP_E_UNSUPPORTED", -7: "NTLP_E_STATE"}
@@ -172,6 +175,47 @@ class Particles:
                          np.arange(1, n + 1, dtype=np.int32), np.zeros(n, dtype=np.int32))
 
 
+class _CEnsemble(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("radius_kind", ctypes.c_int32), ("radius2_kind", ctypes.c_int32),
+                ("radius_p", ctypes.c_double * 2), ("radius2_p", ctypes.c_double * 2), ("frac2", ctypes.c_double),
+                ("salt_kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("salt_p", ctypes.c_double * 3),
+                ("kappa_s", ctypes.c_double), ("mult", ctypes.c_int64), ("first", ctypes.c_int64)]
+
+
+@dataclass
+class Ensemble:
+    """A droplet ensemble drawn on the device (ntlp_ensemble): radius
+    distribution (DIST_LOGNORMAL: (median, sigma_ln); DIST_LOGUNIFORM: (lo,
+    hi)), optionally a second one taken with probability frac2, salt mass
+    (SALT_FIXED: m_s; SALT_DRY_LOGNORMAL: (median dry radius, sigma_ln);
+    SALT_CHOICE3: three masses), kappa_s and multiplicity; droplet i uses the
+    Philox counter first + i under the key seed."""
+    seed: int = 0
+    radius_kind: int = DIST_LOGNORMAL
+    radius_p: tuple = (10e-6, 0.3)
+    radius2_kind: int = DIST_LOGNORMAL
+    radius2_p: tuple = (1e-6, 0.0)
+    frac2: float = 0.0
+    salt_kind: int = SALT_FIXED
+    salt_p: tuple = (1e-18, 0.0, 0.0)
+    kappa_s: float = 1.2
+    mult: int = 1
+    first: int = 0
+
+    def to_c(self) -> _CEnsemble:
+        e = _CEnsemble()
+        e.seed = int(self.seed) & 0xFFFFFFFFFFFFFFFF
+        e.radius_kind, e.radius2_kind = int(self.radius_kind), int(self.radius2_kind)
+        e.radius_p = (ctypes.c_double * 2)(*self.radius_p)
+        e.radius2_p = (ctypes.c_double * 2)(*self.radius2_p)
+        e.frac2 = float(self.frac2)
+        e.salt_kind = int(self.salt_kind)
+        sp = tuple(self.salt_p) + (0.0,) * (3 - len(self.salt_p))
+        e.salt_p = (ctypes.c_double * 3)(*sp)
+        e.kappa_s, e.mult, e.first = float(self.kappa_s), int(self.mult), int(self.first)
+        return e
+
+
 class _CServeConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("steps", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32), ("reserved", ctypes.c_int32)]
@@ -210,6 +254,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_particles_put_records": (ctypes.c_int, [vp, i64]),
         "ntlp_gpu_particles_get_records": (ctypes.c_int, [vp, i64]),
         "ntlp_gpu_particles_count": (i64, []),
+        "ntlp_gpu_particles_get_column": (ctypes.c_int, [i32, vp, i64]),
+        "ntlp_gpu_particles_generate": (ctypes.c_int, [vp, vp, i32]),
+        "ntlp_serve_submit_ensemble": (ctypes.c_int, [ctypes.POINTER(_CEnsemble), i64, vp, i64, vp,
+                                                       ctypes.POINTER(i64)]),
         "ntlp_gpu_save_particles": (ctypes.c_int, [ctypes.c_char_p]),
         "ntlp_gpu_read_part_res": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(i64)]),
         "ntlp_gpu_interp": (ctypes.c_int, [i32, i32]),
@@ -304,6 +352,24 @@ class DropletStep:
         self._check(self.lib.ntlp_gpu_particles_get(_ptr(p.soa), _ptr(p.mult), _ptr(p.pidx), _ptr(p.procidx), n))
         return p
 
+    def generate(self, ens, n) -> None:
+        """Droplets drawn on the device (ntlp_gpu_particles_generate), replacing
+        those there: n droplets of the Ensemble ens, or, with lists, n[k]
+        droplets of ens[k] one ensemble after the other; vp = uf, Tp = Tf and
+        rc set as create_particle does."""
+        if isinstance(ens, Ensemble):
+            ens, n = [ens], [n]
+        arr = (_CEnsemble * len(ens))(*[e.to_c() for e in ens])
+        counts = np.ascontiguousarray(n, dtype=np.int64)
+        self._check(self.lib.ntlp_gpu_particles_generate(ctypes.cast(arr, ctypes.c_void_p), _ptr(counts), len(ens)))
+        self.n = int(counts.sum())
+
+    def column(self, name: str) -> np.ndarray:
+        """One record column of every droplet, in put order."""
+        out = np.empty(self.n)
+        self._check(self.lib.ntlp_gpu_particles_get_column(COL[name], _ptr(out), self.n))
+        return out
+
     def save_particles(self, path: str) -> None:
         """save_particles (particles.f90:1988-2046): int32 count + 352-byte records."""
         self._check(self.lib.ntlp_gpu_save_particles(os.fsencode(path)))
@@ -341,15 +407,16 @@ class DropletStep:
     def particle_update_BE(self, it: int = 2, precision: int = PREC_FP64) -> None:
         self._check(self.lib.ntlp_gpu_particle_update_be_prec(it, precision))
 
-    def particle_coupling_update(self, deterministic: int = 0) -> np.ndarray:
+    def particle_coupling_update(self, deterministic: int = 0, out: bool = True):
         """Returns partsrc_t(:,:,:,1:3), partTsrc_t, partHsrc_t, partTEsrc_t as
-        an array (6, nnx+1, nny+1, nnz+2) (C order; z fastest).  deterministic
-        picks the summation: 0 (or False) LDS node boxes, 1 (or True) a fixed
-        order per node, 2 wavefront-segmented run sums with global atomics."""
+        an array (6, nnx+1, nny+1, nnz+2) (C order; z fastest), or None with
+        out=False (the sources stay on the device).  deterministic picks the
+        summation: 0 (or False) LDS node boxes, 1 (or True) a fixed order per
+        node, 2 wavefront-segmented run sums with global atomics."""
         g = self.grid
-        out = np.empty((6, g.nnx + 1, g.nny + 1, g.nnz + 2))
-        self._check(self.lib.ntlp_gpu_particle_coupling_update(_ptr(out), int(deterministic)))
-        return out
+        arr = np.empty((6, g.nnx + 1, g.nny + 1, g.nnz + 2)) if out else None
+        self._check(self.lib.ntlp_gpu_particle_coupling_update(_ptr(arr), int(deterministic)))
+        return arr
 
     def sources(self) -> np.ndarray:
         """The resident coupling sources, (6, nnx+1, nny+1, nnz+2)."""
@@ -506,6 +573,19 @@ class ServeLoop:
         self._check(self.lib.ntlp_serve_submit(_ptr(inp), inp.strides[0] // 8, _ptr(mult), n, _ptr(out),
                                                out.strides[0] // 8, _ptr(out_mult), ctypes.byref(t)))
         self._held[t.value] = (inp, mult, out, out_mult)
+        return int(t.value)
+
+    def submit_ensemble(self, ens: Ensemble, n: int, out: np.ndarray, out_mult: np.ndarray) -> int:
+        """Queues n droplets of `ens`, drawn on the device at admission (no
+        upload); out (14, n) and out_mult as for submit."""
+        if out.shape != (14, n) or out.dtype != np.float64 or out.strides[1] != 8 or out.strides[0] % 8:
+            raise ValueError(f"serving output must be float64 of shape (14, {n}) with contiguous rows")
+        if out_mult.shape != (n,) or out_mult.dtype != np.int64 or not out_mult.flags.c_contiguous:
+            raise ValueError(f"multiplicities must be contiguous int64 of shape ({n},)")
+        t = ctypes.c_int64()
+        self._check(self.lib.ntlp_serve_submit_ensemble(ctypes.byref(ens.to_c()), int(n), _ptr(out),
+                                                        out.strides[0] // 8, _ptr(out_mult), ctypes.byref(t)))
+        self._held[t.value] = (out, out_mult)
         return int(t.value)
 
     def poll(self, ticket: int) -> bool:

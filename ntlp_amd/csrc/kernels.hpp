@@ -14,6 +14,7 @@
 //                    touched through home[t] (diagnostics, RK3 stage
 //                    registers, pre-step values for the coupling).
 #pragma once
+#include "ensemble.hpp"
 #include "physics.hpp"
 
 namespace ntlp {
@@ -77,7 +78,7 @@ struct StepArgs {
   DropletBufs src, dst;  // fused step reads src through perm, writes dst in slot order
   const int32_t* perm;   // sorted index -> src slot (nullptr: identity)
   double* cold;          // put-order cold columns
-  double* amounts;       // [6][cap] coupling amounts in dst slot order
+  double* amounts;       // [6][cap] coupling amounts in dst slot order (FP32 in mixed precision)
   const int32_t* keys_in; // u-cell key of the current position (src slot order), or null
   int32_t* keys;          // [cap] u-cell key of the new position (dst slot order)
   int32_t* counts;       // u-cell histogram (must be zero on entry)
@@ -409,8 +410,14 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
       if (!SERVE) {
         double am[6];
         coupling_amounts<PREC == PREC_MIXED>(C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
+        if (PREC == PREC_MIXED) {  // stored in FP32, summed in FP64 by the scatter
+          float* af = reinterpret_cast<float*>(a.amounts);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];
+          for (int k = 0; k < 6; ++k) af[aix(k, t)] = (float)am[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];
+        }
       }
       store_dynamic(a.dst.hot, t, d, !inplace || d.numact != numact0, !LAZY);
       if (LAZY) {
@@ -517,11 +524,13 @@ __global__ void __launch_bounds__(kBlock) k_amounts(StepArgs a) {
 
 // -------------------------------------------------------- coupling scatter
 
+// The scatters read the per-droplet amounts as AT: double, or float where a
+// mixed-precision step stored them (their sums stay FP64).
 struct ScatterArgs {
   DevGrid g;
   const int32_t* keys;    // u-cell keys of the positions (slot order)
   const double* hot;      // positions (slot order)
-  const double* amounts;  // [6][cap]
+  const void* amounts;    // [6][cap] of AT
   int64_t n, cap;
   double* src;            // 6 fields (0:nnz+1, 1:nny+1, 1:nnx+1)
   int64_t nsrc;           // elements per field
@@ -610,7 +619,9 @@ __device__ __forceinline__ Deposit deposit_of(const DevGrid& g, const double* s_
 //  3. a run whose cell sits directly above the previous run's cell shares its
 //     bottom nodes with that run's top nodes, which are folded in first.
 // Each wave then issues at most 48 FP64 global atomic instructions.
+template <typename AT>
 __global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
+  const AT* amounts = static_cast<const AT*>(a.amounts);
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
   __shared__ int s_tail[kBlock];
   stage_grid(a.g, s_z, s_zz);
@@ -660,7 +671,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
   key_nodes(a.g, act ? skey : 0, node);
   for (int f = 0; f < 6; ++f) {
     if (!((a.fmask >> f) & 1u)) continue;
-    const double am = valid ? a.amounts[aix(f, t)] : 0.0;
+    const double am = valid ? (double)amounts[aix(f, t)] : 0.0;
     double v[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) v[c] = am * dp.w[c];
@@ -694,7 +705,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
 struct PullArgs {
   DevGrid g;
   const double* hot;
-  const double* amounts;
+  const void* amounts;     // [6][cap] of AT
   const int32_t* perm;     // sorted index -> slot
   const int32_t* offsets;  // u-cell offsets
   int64_t cap;
@@ -703,7 +714,9 @@ struct PullArgs {
   unsigned fmask;
 };
 
+template <typename AT>
 __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
+  const AT* amounts = static_cast<const AT*>(a.amounts);
   __shared__ double s_zz[kMaxLevels];
   for (int k = threadIdx.x; k < a.g.nnz + 2; k += blockDim.x) s_zz[k] = a.g.zz[k];
   __syncthreads();
@@ -741,7 +754,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
                                          a.hot[hix(kHX, s)], a.hot[hix(kHY, s)], a.hot[hix(kHZ, s)]);
 #pragma unroll
           for (int f = 0; f < 6; ++f)
-            if ((a.fmask >> f) & 1u) acc[f] += a.amounts[aix(f, s)] * w;
+            if ((a.fmask >> f) & 1u) acc[f] += (double)amounts[aix(f, s)] * w;
         }
       }
 #pragma unroll
@@ -767,7 +780,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
 struct TileArgs {
   DevGrid g;
   const double* hot;       // positions (slot order)
-  const double* amounts;   // [6][cap]
+  const void* amounts;     // [6][cap] of AT
   const int32_t* keys;     // u-cell keys (slot order)
   const int32_t* perm;     // sorted index -> slot
   const int32_t* offsets;  // u-cell offsets of the sort
@@ -800,8 +813,9 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* s_wave, int* tot
   return base + inc - v;
 }
 
-template <int TI, int TJ, int TK>
+template <int TI, int TJ, int TK, typename AT>
 __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
+  const AT* amounts = static_cast<const AT*>(a.amounts);
   // node (li, lj, lk) of the box at (li*TJ + lj)*TKP + lk (25 KB of LDS for
   // 8x8x8: six blocks fit a CU; an odd pitch, against bank conflicts, made
   // it 28 KB and five blocks, which measured slower)
@@ -881,7 +895,7 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
       y[u] = a.hot[hix(kHY, t)];
       z[u] = a.hot[hix(kHZ, t)];
 #pragma unroll
-      for (int f = 0; f < 6; ++f) am[u][f] = ((a.fmask >> f) & 1u) ? a.amounts[aix(f, t)] : 0.0;
+      for (int f = 0; f < 6; ++f) am[u][f] = ((a.fmask >> f) & 1u) ? (double)amounts[aix(f, t)] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
@@ -1501,8 +1515,9 @@ struct AdmitArgs {
   DevConsts C;
   DevFlags fl;
   DevFields F;
-  const double* in;        // [6][m]: x, y, z, radius, m_s, kappa_s
+  const double* in;        // [6][m]: x, y, z, radius, m_s, kappa_s (null: from ens)
   const int64_t* in_mult;  // [m]
+  DevEnsemble ens;         // the request's ensemble when in is null
   int64_t m, cap, base;
   int32_t tag;             // request slot << kServeIdxBits
   double* hot;
@@ -1521,8 +1536,26 @@ __global__ void __launch_bounds__(kBlock) k_serve_admit(AdmitArgs a) {
   const bool valid = t < a.m;
   int key = -1;
   if (valid) {
-    const double x = a.in[t], y = a.in[a.m + t], z = a.in[2 * a.m + t];
-    const double m_s = a.in[4 * a.m + t], kappa = a.in[5 * a.m + t];
+    double x, y, z, r, m_s, kappa;
+    int64_t mult;
+    if (a.in) {
+      x = a.in[t];
+      y = a.in[a.m + t];
+      z = a.in[2 * a.m + t];
+      r = a.in[3 * a.m + t];
+      m_s = a.in[4 * a.m + t];
+      kappa = a.in[5 * a.m + t];
+      mult = a.in_mult[t];
+    } else {
+      const NewDroplet nd = make_droplet(a.ens, a.g, s_z, t, a.C.rhos, a.C.pi2);
+      x = nd.x;
+      y = nd.y;
+      z = nd.z;
+      r = nd.r;
+      m_s = nd.m_s;
+      kappa = a.ens.kappa_s;
+      mult = a.ens.mult;
+    }
     const Cell c = cell_of(a.g, s_z, s_zz, x, y, z);
     const Fluid f = (ILIN == 1) ? gather_lin_f64<true>(a.g, a.F, s_z, s_zz, x, y, z, c)
                                 : gather6_f64(a.g, a.F, s_z, s_zz, x, y, z, c);
@@ -1534,7 +1567,7 @@ __global__ void __launch_bounds__(kBlock) k_serve_admit(AdmitArgs a) {
     d.vy = f.u[1];
     d.vz = f.u[2];
     d.Tp = f.T;
-    d.r = a.in[3 * a.m + t];
+    d.r = r;
     d.m_s = m_s;
     d.kappa = kappa;
     d.rc = crit_radius_of(a.C, a.fl, m_s, kappa, f.T);
@@ -1543,12 +1576,77 @@ __global__ void __launch_bounds__(kBlock) k_serve_admit(AdmitArgs a) {
     d.numact = 0.0;
     const int64_t s = a.base + t;
     store_drop(a.hot, s, d);
-    a.mult[s] = a.in_mult[t];
+    a.mult[s] = mult;
     a.home[s] = a.tag | (int32_t)t;
     key = ukey_of(a.g, c);
     a.keys[s] = key;
   }
   count_key(a.counts, key, valid);
+}
+
+// ntlp_gpu_particles_generate: droplets 0..n-1 of ensemble e created on the
+// device (create_particle, particles.f90:2226-2300: vp = uf, Tp = Tf from the
+// gather at the new position, rc = crit_radius), in put order.
+struct GenArgs {
+  DevGrid g;
+  DevConsts C;
+  DevFlags fl;
+  DevFields F;
+  const DevEnsemble* ens;  // [nens]
+  const int64_t* start;    // [nens + 1]: first droplet of each ensemble, then n
+  int nens;
+  int64_t n, cap;
+  double* hot;
+  int64_t* mult;
+  int32_t* home;
+  double* cold;
+  int32_t* pidx;
+  int32_t* procidx;
+};
+
+template <int ILIN>
+__global__ void __launch_bounds__(kBlock) k_generate(GenArgs a) {
+  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  stage_grid(a.g, s_z, s_zz);
+  __syncthreads();
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= a.n) return;
+  int lo = 0, hi = a.nens;  // the ensemble of droplet t: start[lo] <= t < start[lo + 1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a.start[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  const DevEnsemble& e = a.ens[lo];
+  const int64_t i = t - a.start[lo];
+  const NewDroplet nd = make_droplet(e, a.g, s_z, i, a.C.rhos, a.C.pi2);
+  const Cell c = cell_of(a.g, s_z, s_zz, nd.x, nd.y, nd.z);
+  const Fluid f = (ILIN == 1) ? gather_lin_f64<true>(a.g, a.F, s_z, s_zz, nd.x, nd.y, nd.z, c)
+                              : gather6_f64(a.g, a.F, s_z, s_zz, nd.x, nd.y, nd.z, c);
+  Drop d;
+  d.x = nd.x;
+  d.y = nd.y;
+  d.z = nd.z;
+  d.vx = f.u[0];
+  d.vy = f.u[1];
+  d.vz = f.u[2];
+  d.Tp = f.T;
+  d.r = nd.r;
+  d.m_s = nd.m_s;
+  d.kappa = e.kappa_s;
+  d.rc = crit_radius_of(a.C, a.fl, nd.m_s, e.kappa_s, f.T);
+  d.res = d.actres = d.numact = 0.0;
+  store_drop(a.hot, t, d);
+  a.mult[t] = e.mult;
+  a.home[t] = (int32_t)t;
+  a.pidx[t] = (int32_t)(e.first + i + 1);
+  a.procidx[t] = 0;
+  for (int k = 0; k < kNCold; ++k) a.cold[(int64_t)k * a.cap + t] = 0.0;
+  a.cold[kCUFX * a.cap + t] = f.u[0];
+  a.cold[kCUFY * a.cap + t] = f.u[1];
+  a.cold[kCUFZ * a.cap + t] = f.u[2];
+  a.cold[kCTF * a.cap + t] = f.T;
+  a.cold[kCQINF * a.cap + t] = f.q;
 }
 
 // a plain column into hot column c (slot order)

@@ -72,6 +72,7 @@ struct Ctx {
   int32_t* procidx = nullptr;
   double* amounts = nullptr;
   bool amounts_valid = false;  // amounts belong to the last step of the current slots
+  bool amounts_f32 = false;    // ... and a mixed-precision step stored them in FP32
   double* diag = nullptr;      // last step's diagnostics in slot order (kNCold columns)
   unsigned diag_mask = 0;      // cold columns waiting in diag
   int32_t* keys[2] = {};
@@ -444,7 +445,10 @@ int scatter_atomic() {
   s.src = ctx.src;
   s.nsrc = ctx.nsrc;
   s.fmask = src_mask();
-  return launch("scatter_seg", [&] { k_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(s); });
+  return launch("scatter_seg", [&] {
+    if (ctx.amounts_f32) k_scatter<float><<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(s);
+    else k_scatter<double><<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(s);
+  });
 }
 
 // atomic-free deterministic scatter over the current cell sort (perm and
@@ -462,7 +466,10 @@ int pull_sorted() {
   p.src = ctx.src;
   p.nsrc = ctx.nsrc;
   p.fmask = src_mask();
-  return launch("scatter_pull", [&] { k_scatter_pull<<<grid8(ctx.nsrc), kBlock, 0, ctx.stream>>>(p); });
+  return launch("scatter_pull", [&] {
+    if (ctx.amounts_f32) k_scatter_pull<float><<<grid8(ctx.nsrc), kBlock, 0, ctx.stream>>>(p);
+    else k_scatter_pull<double><<<grid8(ctx.nsrc), kBlock, 0, ctx.stream>>>(p);
+  });
 }
 
 int scatter_pull() {
@@ -495,7 +502,10 @@ int tile_sorted() {
   t.ntz = (ctx.g.nnz + 2 + kTileK - 1) / kTileK;
   const int64_t nt = (int64_t)t.ntx * t.nty * t.ntz;
   const unsigned grid = (unsigned)((nt + 7) / 8 * 8);
-  return launch("scatter", [&] { k_scatter_box<kTileI, kTileJ, kTileK><<<grid, kBlock, 0, ctx.stream>>>(t); });
+  return launch("scatter", [&] {
+    if (ctx.amounts_f32) k_scatter_box<kTileI, kTileJ, kTileK, float><<<grid, kBlock, 0, ctx.stream>>>(t);
+    else k_scatter_box<kTileI, kTileJ, kTileK, double><<<grid, kBlock, 0, ctx.stream>>>(t);
+  });
 }
 
 int scatter_tile() {
@@ -612,6 +622,8 @@ int run_fused_steps(int mode, int prec, int nsteps, int it0, bool scatter, bool 
   const unsigned mask = mode == NTLP_MODE_RK3 ? kDiagRK3 : kDiagBE;
   if (ctx.diag_mask & ~mask) CHK(flush_diag());
   if (!ctx.perm_valid || !ctx.keys_valid) CHK(bin_current(nullptr, nullptr));
+  ctx.amounts_valid = false;
+  ctx.amounts_f32 = prec == PREC_MIXED;  // the step kernels' amounts
   for (int k = 0; k < nsteps; ++k) {
     const bool reorder = !ctx.slots_sorted || ctx.since_reorder + 1 >= kReorderEvery;
     StepArgs a = base_args();
@@ -643,7 +655,7 @@ int run_fused_steps(int mode, int prec, int nsteps, int it0, bool scatter, bool 
     done = k + 1;
     CHK(close_step(mode == NTLP_MODE_BE));
     CHK(finish_bin(nullptr, false));
-    if (scatter) CHK(tile_sorted());
+    if (scatter && src_mask()) CHK(tile_sorted());  // no coupling flag: the sources stay zero
   }
   if (lazy) CHK(settle(nsteps));  // the call's residence-time additions
   ctx.diag_mask = mask;
@@ -844,11 +856,10 @@ int ntlp_gpu_set_fields(const double* u, const double* v, const double* w, const
   return NTLP_OK;
 }
 
-int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t* pidx, const int32_t* procidx,
-                           int64_t n) {
-  CHK(need_init());
-  if (n < 0 || (n > 0 && (!soa || !mult))) return fail(NTLP_E_INVALID, "bad droplet arrays");
-  if (n >= (int64_t)INT32_MAX) return fail(NTLP_E_RANGE, "more than 2^31-1 droplets");
+namespace {
+// device buffers for n droplets (kept when they are large enough); resets the
+// slot bookkeeping to put order
+int ensure_capacity(int64_t n) {
   HIPCHK(hipStreamSynchronize(ctx.stream));
   if (n > ctx.cap) {
     for (int b = 0; b < 2; ++b) {
@@ -876,6 +887,61 @@ int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t
   ctx.slots_sorted = false;
   ctx.perm_valid = false;
   ctx.keys_valid = false;
+  return 0;
+}
+
+// the critical-radius table of setup_radval (particles.f90:5294-5305)
+int upload_radval(hipStream_t stream) {
+  static double table[1024];
+  const double radstart = -8.0, radend = -3.0, dr = (radstart - radend) / 1024.0;
+  for (int i = 1; i <= 1024; ++i) table[i - 1] = std::pow(10.0, radstart - (double)(i - 1) * dr);
+  HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_radval), table, sizeof(table), 0, hipMemcpyHostToDevice, stream));
+  return 0;
+}
+
+int check_ensemble(const ntlp_ensemble* e) {
+  if (!e) return fail(NTLP_E_INVALID, "null ensemble");
+  auto radius_ok = [](int kind, const double* p) {
+    if (kind == NTLP_DIST_LOGNORMAL) return p[0] > 0 && p[1] >= 0;
+    if (kind == NTLP_DIST_LOGUNIFORM) return p[0] > 0 && p[1] >= p[0];
+    return false;
+  };
+  if (!radius_ok(e->radius_kind, e->radius_p)) return fail(NTLP_E_INVALID, "bad radius distribution");
+  if (e->frac2 < 0 || e->frac2 > 1) return fail(NTLP_E_INVALID, "frac2 must lie in [0, 1]");
+  if (e->frac2 > 0 && !radius_ok(e->radius2_kind, e->radius2_p))
+    return fail(NTLP_E_INVALID, "bad second radius distribution");
+  if (e->salt_kind != NTLP_SALT_FIXED && e->salt_kind != NTLP_SALT_DRY_LOGNORMAL && e->salt_kind != NTLP_SALT_CHOICE3)
+    return fail(NTLP_E_INVALID, "unknown salt distribution");
+  if (e->mult < 1 || e->first < 0) return fail(NTLP_E_INVALID, "mult must be >= 1 and first >= 0");
+  return 0;
+}
+
+DevEnsemble dev_ensemble(const ntlp_ensemble* e) {
+  DevEnsemble d;
+  memset(&d, 0, sizeof(d));
+  d.seed = e->seed;
+  d.radius_kind = e->radius_kind;
+  d.radius2_kind = e->radius2_kind;
+  for (int k = 0; k < 2; ++k) {
+    d.radius_p[k] = e->radius_p[k];
+    d.radius2_p[k] = e->radius2_p[k];
+  }
+  d.frac2 = e->frac2;
+  d.salt_kind = e->salt_kind;
+  for (int k = 0; k < 3; ++k) d.salt_p[k] = e->salt_p[k];
+  d.kappa_s = e->kappa_s;
+  d.mult = e->mult;
+  d.first = e->first;
+  return d;
+}
+}  // namespace
+
+int ntlp_gpu_particles_put(const double* soa, const int64_t* mult, const int32_t* pidx, const int32_t* procidx,
+                           int64_t n) {
+  CHK(need_init());
+  if (n < 0 || (n > 0 && (!soa || !mult))) return fail(NTLP_E_INVALID, "bad droplet arrays");
+  if (n >= (int64_t)INT32_MAX) return fail(NTLP_E_RANGE, "more than 2^31-1 droplets");
+  CHK(ensure_capacity(n));
   if (n == 0) return NTLP_OK;
   for (int c = 0; c < NTLP_NREAL; ++c) {
     if (hot_of(c) >= 0) {
@@ -1060,13 +1126,87 @@ int ntlp_gpu_read_part_res(const char* path, int64_t* kept) {
   return NTLP_OK;
 }
 
+int ntlp_gpu_particles_generate(const ntlp_ensemble* ens, const int64_t* counts, int32_t nens) {
+  CHK(need_fields());
+  if (!ens || !counts || nens < 1 || nens > NTLP_MAX_ENSEMBLES)
+    return fail(NTLP_E_INVALID, "1 to " + std::to_string(NTLP_MAX_ENSEMBLES) + " ensembles with their counts");
+  std::vector<DevEnsemble> de((size_t)nens);
+  std::vector<int64_t> start((size_t)nens + 1, 0);
+  for (int k = 0; k < nens; ++k) {
+    CHK(check_ensemble(&ens[k]));
+    if (counts[k] < 0) return fail(NTLP_E_INVALID, "negative droplet count");
+    de[k] = dev_ensemble(&ens[k]);
+    start[k + 1] = start[k] + counts[k];
+  }
+  const int64_t n = start[nens];
+  if (n < 1 || n >= (int64_t)INT32_MAX) return fail(NTLP_E_RANGE, "the ensembles must hold 1 .. 2^31-2 droplets");
+  CHK(ensure_capacity(n));
+  CHK(upload_radval(ctx.stream));
+  DevEnsemble* d_ens = nullptr;
+  int64_t* d_start = nullptr;
+  CHK(dalloc(d_ens, nens, "ensembles"));
+  int rc = dalloc(d_start, nens + 1, "ensemble starts");
+  if (!rc && (hipMemcpyAsync(d_ens, de.data(), sizeof(DevEnsemble) * nens, hipMemcpyHostToDevice, ctx.stream) !=
+                  hipSuccess ||
+              hipMemcpyAsync(d_start, start.data(), sizeof(int64_t) * (nens + 1), hipMemcpyHostToDevice, ctx.stream) !=
+                  hipSuccess))
+    rc = fail(NTLP_E_HIP, "ensemble upload failed");
+  GenArgs a;
+  a.g = ctx.g;
+  a.C = ctx.C;
+  a.fl = ctx.fl;
+  a.F = fields_view();
+  a.ens = d_ens;
+  a.start = d_start;
+  a.nens = nens;
+  a.n = n;
+  a.cap = ctx.cap;
+  a.hot = ctx.hot[0];
+  a.mult = ctx.mult[0];
+  a.home = ctx.home[0];
+  a.cold = ctx.cold;
+  a.pidx = ctx.pidx;
+  a.procidx = ctx.procidx;
+  const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+  const bool lin = ctx.fl.ilin == 1;
+  if (!rc)
+    rc = launch("generate", [&] {
+      if (lin) k_generate<1><<<grid, kBlock, 0, ctx.stream>>>(a);
+      else k_generate<0><<<grid, kBlock, 0, ctx.stream>>>(a);
+    });
+  if (hipStreamSynchronize(ctx.stream) != hipSuccess && !rc) rc = fail(NTLP_E_HIP, "ensemble generation failed");
+  dfree(d_ens);
+  dfree(d_start);
+  return rc;
+}
+
+int ntlp_gpu_particles_get_column(int32_t column, double* out, int64_t n) {
+  CHK(need_init());
+  if (n != ctx.n || !out) return fail(NTLP_E_INVALID, "n does not match the droplet count, or null output");
+  if (column < 0 || column >= NTLP_NREAL) return fail(NTLP_E_INVALID, "unknown column");
+  if (ctx.state_bad)
+    return fail(NTLP_E_STATE, "a failed step call left the droplets inconsistent: put them again");
+  if (n == 0) return NTLP_OK;
+  if (hot_of(column) >= 0) {
+    const double* hot = ctx.hot[ctx.cur];
+    const int hc = hot_of(column);
+    double* tmp = ctx.col_tmp;
+    const int32_t* hm = ctx.home_identity ? nullptr : ctx.home[ctx.cur];
+    CHK(launch("get", [&] { k_unpack_hot<<<grid_flat(n), kBlock, 0, ctx.stream>>>(hot, hc, hm, n, tmp); }));
+    HIPCHK(hipMemcpyAsync(out, tmp, sizeof(double) * n, hipMemcpyDeviceToHost, ctx.stream));
+  } else {
+    CHK(flush_diag());
+    HIPCHK(hipMemcpyAsync(out, ctx.cold + (int64_t)cold_of(column) * ctx.cap, sizeof(double) * n,
+                          hipMemcpyDeviceToHost, ctx.stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return NTLP_OK;
+}
+
 int ntlp_gpu_crit_radius(void) {
   CHK(need_init());
   if (ctx.n <= 0) return fail(NTLP_E_STATE, "no droplets");
-  static double table[1024];
-  const double radstart = -8.0, radend = -3.0, dr = (radstart - radend) / 1024.0;
-  for (int i = 1; i <= 1024; ++i) table[i - 1] = std::pow(10.0, radstart - (double)(i - 1) * dr);
-  HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_radval), table, sizeof(table), 0, hipMemcpyHostToDevice, ctx.stream));
+  CHK(upload_radval(ctx.stream));
   StepArgs a = base_args();
   CHK(launch("crit_radius", [&] { k_crit_radius<<<grid_flat(ctx.n), kBlock, 0, ctx.stream>>>(a); }));
   HIPCHK(hipStreamSynchronize(ctx.stream));
@@ -1131,6 +1271,7 @@ int ntlp_gpu_particle_coupling_update(double* src_out, int32_t deterministic) {
     StepArgs a = base_args();
     CHK(launch("amounts", [&] { k_amounts<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(a); }));
     ctx.amounts_valid = true;
+    ctx.amounts_f32 = false;
   }
   if (deterministic == 1) CHK(scatter_pull());
   else if (deterministic == 2) CHK(scatter_atomic());

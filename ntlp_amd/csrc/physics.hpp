@@ -383,17 +383,11 @@ __device__ __forceinline__ double rad_solver2_f32(const DevConsts& C, double Tf,
   return (double)guess * 1e-6;
 }
 
-// Squared residual of the unchanged state below which the mixed solve starts
-// there without forming the Koehler guess.
-constexpr double kStartResid2 = 1e-4;
-
 // Radius/temperature block of particle_update_BE (3103-3178) on a droplet
 // whose velocity is already the new one.  Updates d.r, d.Tp, activation
 // bookkeeping and the tally.  PREC_FP64: reference residual and FD
 // Jacobian.  Otherwise the closed-form residual with its exact Jacobian; in
-// mixed precision the Koehler guess is formed in FP32 and the solve starts
-// from whichever of that guess and the unchanged state (1, Tf/Tp) has the
-// smaller residual.
+// mixed precision the Koehler guess that starts the solve is formed in FP32.
 template <int PREC>
 __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFlags& fl, Drop& d,
                                                 double Tf, double qinf, double rhoa, double diffnorm,
@@ -460,24 +454,13 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
       P.Cc = C.Lv3_Cpp * d.r * iTp;
       const AJEval ev{C, P};
       if (PREC == PREC_MIXED) {
-        // Start from the unchanged state (1, Tf/Tp) when its residual is
-        // already small; otherwise from the better of it and the FP32 Koehler
-        // guess (the guess, with its log, acos/cos or cbrt, costs as much as a
-        // residual evaluation and rarely wins over a step of a few 1e-4 s).
-        rt_start[0] = 1.0;
-        rt_start[1] = Tf * iTp;
-        double fb[2];
-        ev.f(rt_start[0], rt_start[1], fb);
-        const double nb = fb[0] * fb[0] + fb[1] * fb[1];
-        if (!(nb < kStartResid2)) {
-          const double guess = rad_solver2_f32(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
-          const double gs[2] = {mflag == 0 ? guess * ir : 1.0, mflag == 0 ? Tf * iTp : 1.0};
-          double fa[2];
-          ev.f(gs[0], gs[1], fa);
-          if (fa[0] * fa[0] + fa[1] * fa[1] <= nb) {
-            rt_start[0] = gs[0];
-            rt_start[1] = gs[1];
-          }
+        // the reference's start, from the Koehler guess formed in FP32: the
+        // BE equations can have more than one solution, and which one Newton
+        // (or LM) reaches depends on where it starts
+        const double guess = rad_solver2_f32(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
+        if (mflag == 0) {
+          rt_start[0] = guess * ir;
+          rt_start[1] = Tf * iTp;
         }
       }
       flag = gauss_newton(ev, rt_start, rt_zeroes, it1);

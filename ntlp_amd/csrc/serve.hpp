@@ -99,6 +99,8 @@ struct SReq {
   int64_t n = 0;
   const double* in = nullptr;
   const int64_t* in_mult = nullptr;
+  bool generated = false;  // created on the device from ens (no upload)
+  DevEnsemble ens{};
   double* out = nullptr;
   int64_t* out_mult = nullptr;
   int64_t ld_in = 0, ld_out = 0;
@@ -353,6 +355,10 @@ int serve_retire(bool block) {
 int serve_upload(std::vector<SReq*>& accepted) {
   HostTimer w(srv.ms_copy_calls);
   for (SReq* r : accepted) {
+    if (r->generated) {  // nothing to upload: admitted at once
+      srv.loading.push_back(r);
+      continue;
+    }
     SHIP(hipMemcpy2DAsync(srv.in + 6 * r->in_row, sizeof(double) * r->n, r->in, sizeof(double) * r->ld_in,
                           sizeof(double) * r->n, 6, hipMemcpyHostToDevice, srv.up));
     SHIP(hipMemcpyAsync(srv.in_mult + r->in_row, r->in_mult, sizeof(int64_t) * r->n, hipMemcpyHostToDevice, srv.up));
@@ -368,7 +374,7 @@ int serve_upload(std::vector<SReq*>& accepted) {
 int serve_admit() {
   while (!srv.loading.empty()) {
     SReq* r = srv.loading.front();
-    const hipError_t q = hipEventQuery(r->ev_in);
+    const hipError_t q = r->generated ? hipSuccess : hipEventQuery(r->ev_in);
     if (q == hipErrorNotReady) {
       if (!srv.active.empty()) break;
       HostTimer w(srv.ms_wait_upload);
@@ -376,14 +382,15 @@ int serve_admit() {
     } else if (q != hipSuccess) {
       return serve_fail(NTLP_E_HIP, std::string("serving upload failed: ") + hipGetErrorString(q));
     }
-    SHIP(hipStreamWaitEvent(srv.stream, r->ev_in, 0));
+    if (!r->generated) SHIP(hipStreamWaitEvent(srv.stream, r->ev_in, 0));
     AdmitArgs a;
     a.g = ctx.g;
     a.C = ctx.C;
     a.fl = ctx.fl;
     a.F = fields_view();
-    a.in = srv.in + 6 * r->in_row;
-    a.in_mult = srv.in_mult + r->in_row;
+    a.in = r->generated ? nullptr : srv.in + 6 * r->in_row;
+    a.in_mult = r->generated ? nullptr : srv.in_mult + r->in_row;
+    a.ens = r->ens;
     a.m = r->n;
     a.cap = srv.cap;
     a.base = srv.hw;
@@ -396,11 +403,13 @@ int serve_admit() {
     if (ctx.fl.ilin == 1) k_serve_admit<1><<<grid8(r->n), kBlock, 0, srv.stream>>>(a);
     else k_serve_admit<0><<<grid8(r->n), kBlock, 0, srv.stream>>>(a);
     SHIP(hipGetLastError());
-    hipEvent_t ev;
-    SHIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    SHIP(hipEventRecord(ev, srv.stream));
-    srv.admitted.push_back(ev);
-    (void)hipEventDestroy(r->ev_in);
+    if (!r->generated) {  // its input-ring rows are free once the admission has run
+      hipEvent_t ev;
+      SHIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      SHIP(hipEventRecord(ev, srv.stream));
+      srv.admitted.push_back(ev);
+    }
+    if (r->ev_in) (void)hipEventDestroy(r->ev_in);
     r->ev_in = nullptr;
     srv.loading.pop_front();
     srv.hw += r->n;
@@ -527,7 +536,7 @@ void serve_loop() {
             break;
           }
         if (slot < 0 || r->n > srv.cap - srv.hw - srv.reserved) break;
-        const int64_t row = srv.in_ring.alloc(r->n);
+        const int64_t row = r->generated ? 0 : srv.in_ring.alloc(r->n);
         if (row < 0) break;
         srv.queue.pop_front();
         srv.slot_used[slot] = true;
@@ -608,11 +617,7 @@ int ntlp_serve_open(const ntlp_serve_config* cfg, int64_t* capacity) {
     HIPCHK(hipMemsetAsync(srv.tally, 0, sizeof(unsigned long long) * kTallySlots * kTallyWidth, srv.stream));
     HIPCHK(hipMemsetAsync(srv.keys[0], 0xff, sizeof(int32_t) * cap, srv.stream));
     HIPCHK(hipMemsetAsync(srv.keys[1], 0xff, sizeof(int32_t) * cap, srv.stream));
-    // the critical-radius table (setup_radval, particles.f90:5294-5305)
-    static double table[1024];
-    const double radstart = -8.0, radend = -3.0, dr = (radstart - radend) / 1024.0;
-    for (int i = 1; i <= 1024; ++i) table[i - 1] = std::pow(10.0, radstart - (double)(i - 1) * dr);
-    HIPCHK(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_radval), table, sizeof(table), 0, hipMemcpyHostToDevice, srv.stream));
+    CHK(upload_radval(srv.stream));
     HIPCHK(hipStreamSynchronize(srv.stream));
     return 0;
   };
@@ -646,6 +651,41 @@ int ntlp_serve_submit(const double* in, int64_t ld_in, const int64_t* in_mult, i
   r->out = out;
   r->out_mult = out_mult;
   r->ld_in = ld_in;
+  r->ld_out = ld_out;
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(srv.mu);
+    if (srv.err) {
+      delete r;
+      return fail(srv.err, srv.err_msg);
+    }
+    r->ticket = srv.next_ticket++;
+    srv.reqs[r->ticket] = r;
+    srv.queue.push_back(r);
+    srv.submitted++;
+  }
+  srv.cv_work.notify_one();
+  *ticket = r->ticket;
+  return NTLP_OK;
+}
+
+int ntlp_serve_submit_ensemble(const ntlp_ensemble* ens, int64_t n, double* out, int64_t ld_out, int64_t* out_mult,
+                               int64_t* ticket) {
+  if (!srv.open) return fail(NTLP_E_STATE, "the serving loop is not open");
+  CHK(check_ensemble(ens));
+  if (!out || !out_mult || !ticket) return fail(NTLP_E_INVALID, "null argument");
+  if (n < 1) return fail(NTLP_E_INVALID, "a request needs at least one droplet");
+  if (ld_out == 0) ld_out = n;
+  if (ld_out < n) return fail(NTLP_E_INVALID, "column stride smaller than the request");
+  if (n > srv.cap || n > kServeIdxMask + 1)
+    return fail(NTLP_E_RANGE, "request larger than the pool (" + std::to_string(srv.cap) + " droplets) or 2^23");
+  SReq* r = new SReq();
+  r->n = n;
+  r->generated = true;
+  r->ens = dev_ensemble(ens);
+  r->out = out;
+  r->out_mult = out_mult;
+  r->ld_in = n;
   r->ld_out = ld_out;
   r->t_submit = Clock::now();
   {

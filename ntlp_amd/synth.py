@@ -92,6 +92,27 @@ def dry_mass(r_dry, rhos=2000.0):
     return rhos * (4.0 / 3.0) * np.pi * r_dry ** 3
 
 
+def ensemble(k: int, seed: int = 0):
+    """The droplet ensemble of workload k as drawn on the device
+    (ntlp_gpu_particles_generate): the same distributions as workload()'s
+    host draws, from the counter-based generator instead of numpy."""
+    from . import (DIST_LOGNORMAL, DIST_LOGUNIFORM, SALT_CHOICE3, SALT_DRY_LOGNORMAL, SALT_FIXED, Ensemble)
+
+    if k == 1:
+        return Ensemble(seed=seed, radius_kind=DIST_LOGNORMAL, radius_p=(10e-6, 0.3), salt_kind=SALT_FIXED,
+                        salt_p=(1e-18,))
+    if k in (2, 4):
+        return Ensemble(seed=seed, radius_kind=DIST_LOGNORMAL, radius_p=(8e-6, 0.4), salt_kind=SALT_DRY_LOGNORMAL,
+                        salt_p=(0.05e-6, 0.5))
+    if k == 3:
+        return Ensemble(seed=seed, radius_kind=DIST_LOGUNIFORM, radius_p=(5e-6, 20e-6), radius2_kind=DIST_LOGUNIFORM,
+                        radius2_p=(1e-6, 2e-6), frac2=0.6, salt_kind=SALT_FIXED, salt_p=(5e-18,))
+    if k == 5:
+        return Ensemble(seed=seed, radius_kind=DIST_LOGNORMAL, radius_p=(10e-6, 0.35), salt_kind=SALT_CHOICE3,
+                        salt_p=(1e-19, 1e-17, 1e-15))
+    raise ValueError(f"unknown workload {k}")
+
+
 @dataclass
 class Workload:
     name: str
@@ -104,12 +125,17 @@ class Workload:
     mode: int = 2
 
 
-def workload(k: int, n: int | None = None, seed: int = 0, make_fields_: bool = True) -> Workload:
+def workload(k: int, n: int | None = None, seed: int = 0, make_fields_: bool = True,
+             droplets: bool = True) -> Workload:
     """The five workloads: 1 = 64^3 / 100k / 1 step; 2 = 128^3 / 2M / 100
     steps; 3 = 256x256x128 tanh-stretched / 20M (60% haze) / 50 steps;
     4 = 128^3 serving fields; 5 = 512x512x256 / 100M / 20 steps.  `n`
-    overrides the droplet count (parity tests at reduced size)."""
+    overrides the droplet count (parity tests at reduced size); droplets=False
+    leaves them to be drawn on the device (ensemble(k))."""
     rng = np.random.default_rng(seed)
+    # the fields have their own stream, so they are the same with or without
+    # host-drawn droplets
+    rng_f = np.random.default_rng([seed, 1])
     flags = Flags()
     if k == 1:
         grid = Grid.uniform(64, 64, 64, 0.0156, 0.0156, 0.0156)
@@ -117,39 +143,39 @@ def workload(k: int, n: int | None = None, seed: int = 0, make_fields_: bool = T
         n = 100_000 if n is None else n
         r = lognormal(rng, n, 10e-6, 0.3)
         ms = np.full(n, 1e-18)
-        fl = make_fields(grid, c, rng) if make_fields_ else None
-        d = make_droplets(grid, rng, n, r, ms)
+        fl = make_fields(grid, c, rng_f) if make_fields_ else None
+        d = make_droplets(grid, rng, n, r, ms) if droplets else None
         return Workload("64^3 periodic, 100k droplets, 1 step", grid, c, flags, fl, d, 1)
     if k in (2, 4):
         grid = Grid.uniform(128, 128, 128, 0.0156, 0.0156, 0.0156)
         c = Consts(dt=5.0e-4, radius_init=8e-6)
         n = 2_000_000 if n is None else n
-        r = lognormal(rng, n, 8e-6, 0.4)
-        ms = dry_mass(lognormal(rng, n, 0.05e-6, 0.5))
-        fl = make_fields(grid, c, rng) if make_fields_ else None
-        d = make_droplets(grid, rng, n, r, ms)
+        r = lognormal(rng, n, 8e-6, 0.4) if droplets else None
+        ms = dry_mass(lognormal(rng, n, 0.05e-6, 0.5)) if droplets else None
+        fl = make_fields(grid, c, rng_f) if make_fields_ else None
+        d = make_droplets(grid, rng, n, r, ms) if droplets else None
         return Workload("128^3 periodic, 2M droplets, 100 steps", grid, c, flags, fl, d, 100)
     if k == 3:
         grid = Grid.tanh_stretched(256, 256, 128, 0.05, 0.05, 0.01, 0.10)
         c = Consts(dt=5.0e-4, radius_init=1.5e-6)
         n = 20_000_000 if n is None else n
-        nh = int(round(0.6 * n))
-        r = np.empty(n)
-        r[:nh] = np.exp(rng.uniform(np.log(1e-6), np.log(2e-6), nh))
-        r[nh:] = np.exp(rng.uniform(np.log(5e-6), np.log(20e-6), n - nh))
-        r = r[rng.permutation(n)]
+        if droplets:
+            nh = int(round(0.6 * n))
+            r = np.empty(n)
+            r[:nh] = np.exp(rng.uniform(np.log(1e-6), np.log(2e-6), nh))
+            r[nh:] = np.exp(rng.uniform(np.log(5e-6), np.log(20e-6), n - nh))
+            r = r[rng.permutation(n)]
         ms = np.full(n, 5e-18)  # critical radius ~1.4 um: the haze sits at activation
-        fl = (make_fields(grid, c, rng, T0=295.0, T_sd=0.0, T_grad=0.02, rh_uniform=(99.8, 100.2))
+        fl = (make_fields(grid, c, rng_f, T0=295.0, T_sd=0.0, T_grad=0.02, rh_uniform=(99.8, 100.2))
               if make_fields_ else None)
-        d = make_droplets(grid, rng, n, r, ms)
+        d = make_droplets(grid, rng, n, r, ms) if droplets else None
         return Workload("256x256x128 tanh-stretched, 20M droplets (60% haze), 50 steps", grid, c, flags, fl, d, 50)
     if k == 5:
         grid = Grid.uniform(512, 512, 256, 0.0156, 0.0156, 0.0156)
         c = Consts(dt=5.0e-4, radius_init=10e-6)
         n = 100_000_000 if n is None else n
-        r = lognormal(rng, n, 10e-6, 0.35)
-        ms = rng.choice(np.array([1e-19, 1e-17, 1e-15]), size=n)
-        fl = make_fields(grid, c, rng) if make_fields_ else None
-        d = make_droplets(grid, rng, n, r, ms)
+        r = lognormal(rng, n, 10e-6, 0.35) if droplets else None
+        fl = make_fields(grid, c, rng_f) if make_fields_ else None
+        d = make_droplets(grid, rng, n, r, rng.choice(np.array([1e-19, 1e-17, 1e-15]), size=n)) if droplets else None
         return Workload("512x512x256 periodic, 100M droplets, 20 steps", grid, c, flags, fl, d, 20)
     raise ValueError(f"unknown workload {k}")

@@ -16,7 +16,8 @@ from ntlp_amd import build
 
 STEP_MIXED_BE = ("_ZN4ntlp6k_stepILi2ELi1ELi1ELb0ELb1EEEvNS_8StepArgsE",   # library step (multi-step calls)
                  "_ZN4ntlp6k_stepILi2ELi1ELi1ELb1ELb0EEEvNS_8StepArgsE")   # serving step
-SCATTER_BOX = "_ZN4ntlp13k_scatter_boxILi8ELi8ELi8EEEvNS_8TileArgsE"
+SCATTER_BOX = ("_ZN4ntlp13k_scatter_boxILi8ELi8ELi8EdEEvNS_8TileArgsE",   # FP64 amounts
+               "_ZN4ntlp13k_scatter_boxILi8ELi8ELi8EfEEvNS_8TileArgsE")   # FP32 amounts (mixed precision)
 
 
 @pytest.fixture(scope="module")
@@ -42,7 +43,8 @@ def test_mixed_be_step_keeps_four_waves_per_simd(resources, name):
     assert r["ScratchSize"] == 0 and r["VGPRs Spill"] == 0, r
 
 
-def test_scatter_box_fits_six_workgroups_per_cu(resources):
-    r = resources[SCATTER_BOX]
+@pytest.mark.parametrize("name", SCATTER_BOX)
+def test_scatter_box_fits_six_workgroups_per_cu(resources, name):
+    r = resources[name]
     assert r["VGPRs"] <= 80, r
     assert r["ScratchSize"] == 0 and r["VGPRs Spill"] == 0, r
