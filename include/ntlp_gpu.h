@@ -263,6 +263,20 @@ int ntlp_gpu_calc_tauc_min(double *tauc_min);
  * Any output may be NULL.  num100, numimpos, actnum and denum come from
  * ntlp_gpu_counters. */
 int ntlp_gpu_particle_xy_stats(double *s0, double *s1, double *lwp);
+/* ntlp_gpu_particle_xy_stats for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It has no counterpart in the reference.  s0[21], s1[24*(nnz+2)]
+ * and lwp[nnx*nny] are DEVICE pointers, all three required; the per-workgroup
+ * partial rows live in a static device workspace of 12.6 MB per GPU (held
+ * until the process ends; its address is looked up by ntlp_gpu_init on each
+ * device, so call that first with the device current).  Sizes the workspace
+ * cannot hold return NTLP_E_RANGE; with nnz + 2 <= 1024 every grid fits.  The
+ * call only enqueues work on the library stream: the outputs are ready after
+ * ntlp_gpu_sync().  Every call on one GPU shares the workspace, so calls that
+ * overlap on one GPU (several host threads or processes) race.  Results equal
+ * ntlp_gpu_particle_xy_stats' except for the summation order of lwp (global
+ * atomics). */
+int ntlp_gpu_particle_xy_stats_noalloc(double *s0, double *s1, double *lwp);
 
 /* The flux statistics of the last particle_update_BE step (particles.f90:
  * 3030-3032, 3208-3238): pflux(0:nnz), pmassflux(0:nnz), penegflux(0:nnz),

@@ -275,6 +275,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_num_ucells": (i64, []),
         "ntlp_gpu_calc_tauc_min": (ctypes.c_int, [ctypes.POINTER(dp)]),
         "ntlp_gpu_particle_xy_stats": (ctypes.c_int, [vp, vp, vp]),
+        "ntlp_gpu_particle_xy_stats_noalloc": (ctypes.c_int, [vp, vp, vp]),
         "ntlp_gpu_step": (ctypes.c_int, [i32, i32, i32, i32]),
         "ntlp_gpu_be_flux": (ctypes.c_int, [vp, vp, vp]),
         "ntlp_gpu_counters": (ctypes.c_int, [ctypes.POINTER(_CCounters), i32]),

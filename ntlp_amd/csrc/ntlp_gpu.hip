@@ -535,6 +535,102 @@ int flush_diag() {
   return 0;
 }
 
+// ---------------------------------------------------- droplet statistics
+
+// particle_xy_stats runs in passes of nq_max per-level quantities; every
+// workgroup (row) writes kStats0 whole-domain values and nq_max*(nnz+2)
+// per-level values per pass, and k_xy_reduce sums the rows in row order.
+constexpr int kStatsRowsMax = 256;
+constexpr int kStatsWidthMax = 6144;  // nq_max * (nnz+2) of one pass (nnz+2 <= kMaxLevels)
+constexpr int64_t kStatsScratch = (int64_t)kStatsRowsMax * (kStats0 + kStatsWidthMax);
+
+// Workspace of ntlp_gpu_particle_xy_stats_noalloc: its partial rows, 12.6 MB
+// per GPU, held until the process ends.  Host code never passes this name to
+// a kernel (on the host it is only a stand-in for the device symbol): it uses
+// the device address that ntlp_gpu_init looks up on each device.
+static __device__ __align__(256) double scratch[kStatsScratch];
+
+constexpr int kMaxDevices = 64;
+double* g_scratch_dev[kMaxDevices] = {};  // device address of `scratch` on each device
+
+// Looks up the device address of `scratch` on the current device (once per
+// device; called by ntlp_gpu_init with the device current).
+int setup_scratch() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(NTLP_E_RANGE, "device ordinal beyond the workspace table");
+  if (g_scratch_dev[dev]) return 0;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(NTLP_E_HIP, "hipGetSymbolAddress returned a null workspace address");
+  g_scratch_dev[dev] = static_cast<double*>(p);
+  return 0;
+}
+
+int stats_scratch(double** ws) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch_dev[dev])
+    return fail(NTLP_E_STATE, "no workspace address on this device: call ntlp_gpu_init with it current");
+  *ws = g_scratch_dev[dev];
+  return 0;
+}
+
+struct StatsShape {
+  int nz2, rows, nq_max;
+};
+
+StatsShape stats_shape() {
+  StatsShape sh;
+  sh.nz2 = ctx.g.nnz + 2;
+  sh.rows = (int)std::min<int64_t>(kStatsRowsMax, std::max<int64_t>(1, (ctx.n + kBlock - 1) / kBlock));
+  sh.nq_max = std::max(1, std::min(24, kStatsWidthMax / sh.nz2));
+  return sh;
+}
+
+// Enqueues the statistics passes: partial rows into part0 [rows][kStats0] and
+// part1 [rows][nq_max][nnz+2], results into out0 [kStats0], out1 [24][nnz+2]
+// and lw [nny][nnx] (all device memory).  timed: through launch() (HIP events
+// when timing is on); otherwise the kernels are launched directly, so that the
+// call creates nothing.
+int stats_launch(const StatsShape& sh, double* part0, double* part1, double* out0, double* out1, double* lw,
+                 bool timed) {
+  auto go = [&](const char* name, auto&& f) -> int {
+    if (timed) return launch(name, f);
+    f();
+    HIPCHK(hipGetLastError());
+    return 0;
+  };
+  const int64_t nlw = (int64_t)ctx.g.nnx * ctx.g.nny;
+  CHK(go("fill", [&] { k_fill_f64<<<grid_flat(nlw), kBlock, 0, ctx.stream>>>(lw, nlw, 0.0); }));
+  for (int q0 = 0; q0 < 24; q0 += sh.nq_max) {
+    StatsArgs a;
+    a.g = ctx.g;
+    a.C = ctx.C;
+    a.hot = ctx.hot[ctx.cur];
+    a.mult = ctx.mult[ctx.cur];
+    a.home = ctx.home[ctx.cur];
+    a.cold = ctx.cold;
+    a.n = ctx.n;
+    a.cap = ctx.cap;
+    a.q0 = q0;
+    a.nq = std::min(sh.nq_max, 24 - q0);
+    a.part0 = part0;
+    a.part1 = part1;
+    a.lwp = lw;
+    const size_t lds = sizeof(double) * a.nq * sh.nz2;
+    const int width = a.nq * sh.nz2;
+    double* dst = out1 + (int64_t)q0 * sh.nz2;
+    CHK(go("xy_stats", [&] { k_xy_stats<<<sh.rows, kBlock, lds, ctx.stream>>>(a); }));
+    CHK(go("xy_stats", [&] {
+      k_xy_reduce<<<(width + kBlock - 1) / kBlock, kBlock, 0, ctx.stream>>>(part1, sh.rows, width, dst, width, width);
+    }));
+    if (q0 == 0)
+      CHK(go("xy_stats", [&] { k_xy_reduce<<<1, kBlock, 0, ctx.stream>>>(part0, sh.rows, kStats0, out0, 9, 15); }));
+  }
+  return 0;
+}
+
 template <int MODE, int PREC, int ILIN>
 void launch_update_t(const StepArgs& a, unsigned grid) {
   k_update<MODE, PREC, ILIN><<<grid, kBlock, 0, ctx.stream>>>(a);
@@ -702,6 +798,7 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   const int64_t cells_u = (int64_t)(grid->nnx + 1) * (grid->nny + 1) * (grid->nnz + 1);
   if (cells_u >= (int64_t)INT32_MAX) return fail(NTLP_E_RANGE, "grid has too many cells for 32-bit cell keys");
   release_all();
+  CHK(setup_scratch());
   HIPCHK(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
   const int nz = grid->nnz;
   const int nlev = nz + 2, nd = nz + 3;
@@ -1470,45 +1567,16 @@ int ntlp_gpu_calc_tauc_min(double* tauc_min) {
 int ntlp_gpu_particle_xy_stats(double* s0, double* s1, double* lwp) {
   CHK(need_drops());
   CHK(flush_diag());
-  const int nz2 = ctx.g.nnz + 2;
-  const int rows = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (ctx.n + kBlock - 1) / kBlock));
-  const int nq_max = std::max(1, std::min(24, 6144 / nz2));
+  const StatsShape sh = stats_shape();
   double *part0 = nullptr, *part1 = nullptr, *out0 = nullptr, *out1 = nullptr, *lw = nullptr;
-  int rc = dalloc(part0, (int64_t)rows * kStats0, "statistics rows");
-  if (!rc) rc = dalloc(part1, (int64_t)rows * nq_max * nz2, "statistics rows");
+  int rc = dalloc(part0, (int64_t)sh.rows * kStats0, "statistics rows");
+  if (!rc) rc = dalloc(part1, (int64_t)sh.rows * sh.nq_max * sh.nz2, "statistics rows");
   if (!rc) rc = dalloc(out0, kStats0, "statistics");
-  if (!rc) rc = dalloc(out1, 24 * nz2, "statistics");
+  if (!rc) rc = dalloc(out1, 24 * sh.nz2, "statistics");
   if (!rc) rc = dalloc(lw, (int64_t)ctx.g.nnx * ctx.g.nny, "LWP");
-  if (!rc && hipMemsetAsync(lw, 0, sizeof(double) * ctx.g.nnx * ctx.g.nny, ctx.stream) != hipSuccess)
-    rc = fail(NTLP_E_HIP, "statistics set-up failed");
-  for (int q0 = 0; q0 < 24 && !rc; q0 += nq_max) {
-    StatsArgs a;
-    a.g = ctx.g;
-    a.C = ctx.C;
-    a.hot = ctx.hot[ctx.cur];
-    a.mult = ctx.mult[ctx.cur];
-    a.home = ctx.home[ctx.cur];
-    a.cold = ctx.cold;
-    a.n = ctx.n;
-    a.cap = ctx.cap;
-    a.q0 = q0;
-    a.nq = std::min(nq_max, 24 - q0);
-    a.part0 = part0;
-    a.part1 = part1;
-    a.lwp = lw;
-    const size_t lds = sizeof(double) * a.nq * nz2;
-    const int width = a.nq * nz2;
-    double* dst = out1 + (int64_t)q0 * nz2;
-    rc = launch("xy_stats", [&] { k_xy_stats<<<rows, kBlock, lds, ctx.stream>>>(a); });
-    if (!rc)
-      rc = launch("xy_stats", [&] {
-        k_xy_reduce<<<(width + kBlock - 1) / kBlock, kBlock, 0, ctx.stream>>>(part1, rows, width, dst, width, width);
-      });
-    if (!rc && q0 == 0)
-      rc = launch("xy_stats", [&] { k_xy_reduce<<<1, kBlock, 0, ctx.stream>>>(part0, rows, kStats0, out0, 9, 15); });
-  }
+  if (!rc) rc = stats_launch(sh, part0, part1, out0, out1, lw, true);
   if (!rc && ((s0 && hipMemcpyAsync(s0, out0, sizeof(double) * kStats0, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess) ||
-              (s1 && hipMemcpyAsync(s1, out1, sizeof(double) * 24 * nz2, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess) ||
+              (s1 && hipMemcpyAsync(s1, out1, sizeof(double) * 24 * sh.nz2, hipMemcpyDeviceToHost, ctx.stream) != hipSuccess) ||
               (lwp && hipMemcpyAsync(lwp, lw, sizeof(double) * ctx.g.nnx * ctx.g.nny, hipMemcpyDeviceToHost,
                                      ctx.stream) != hipSuccess) ||
               hipStreamSynchronize(ctx.stream) != hipSuccess))
@@ -1519,6 +1587,25 @@ int ntlp_gpu_particle_xy_stats(double* s0, double* s1, double* lwp) {
   dfree(out1);
   dfree(lw);
   return rc;
+}
+
+int ntlp_gpu_particle_xy_stats_noalloc(double* s0, double* s1, double* lwp) {
+  CHK(need_drops());
+  if (!s0 || !s1 || !lwp) return fail(NTLP_E_INVALID, "null output (all three device outputs are required)");
+  double* ws = nullptr;
+  CHK(stats_scratch(&ws));
+  const StatsShape sh = stats_shape();
+  const int64_t need = (int64_t)sh.rows * (kStats0 + (int64_t)sh.nq_max * sh.nz2);
+  if (need > kStatsScratch)
+    return fail(NTLP_E_RANGE, "statistics rows need " + std::to_string(need * 8) + " bytes of workspace; the static "
+                "workspace holds " + std::to_string(kStatsScratch * 8));
+  if (ctx.diag_mask) {  // pending diagnostics (uf, Tf, qinf, qstar) into the put-order columns
+    k_flush_diag<<<grid_flat(ctx.n), kBlock, 0, ctx.stream>>>(ctx.diag, ctx.home[ctx.cur], ctx.n, ctx.cap,
+                                                              ctx.diag_mask, ctx.cold);
+    HIPCHK(hipGetLastError());
+    ctx.diag_mask = 0;
+  }
+  return stats_launch(sh, ws, ws + (int64_t)sh.rows * kStats0, s0, s1, lwp, false);
 }
 
 int64_t ntlp_gpu_num_ucells(void) { return ctx.init ? ctx.ncell : 0; }
