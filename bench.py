@@ -468,7 +468,7 @@ def main():
               "droplets with an all-reduce of the coupling fields; that path is not built).", file=sys.stderr)
         sys.exit(2)
 
-    from ntlp_amd import MODE_BE, PREC_FP64, PREC_MIXED, synth
+    from ntlp_amd import MODE_BE, PREC_FP64, PREC_FP64_AJ, PREC_MIXED, synth
 
     if args.serve_only:
         print(json.dumps(serve_bench(args.seed, args.serve_seconds, args.serve_rate)), flush=True)
@@ -485,7 +485,7 @@ def main():
     t_mixed = run_steps(s, init, MODE_BE, PREC_MIXED, args.steps, args.warmup)
     cnt_mixed = s.counters()
     rows = s.step_counters()
-    end_mixed = {c: s.column(c) for c in ("radius", "Tp", "z")}
+    end_mixed = {c: s.column(c) for c in ("radius", "Tp", "z", "vz")}
     log(f"[bench] mixed: {args.steps} steps in {t_mixed:.3f}s")
     ktimes = kernel_timing(s, init, MODE_BE, PREC_MIXED, args.steps, args.warmup)
     log("[bench] kernel times (mixed): " + ", ".join(f"{k}={v[1] / v[0]:.3f}ms x{v[0]}" for k, v in ktimes.items()))
@@ -495,11 +495,18 @@ def main():
     if not args.no_fp64:
         t_fp64 = run_steps(s, init, MODE_BE, PREC_FP64, args.steps, args.warmup)
         cnt_fp64 = s.counters()
-        end_fp64 = {c: s.column(c) for c in ("radius", "Tp", "z")}
+        end_fp64 = {c: s.column(c) for c in ("radius", "Tp", "z", "vz")}
         log(f"[bench] fp64: {args.steps} steps in {t_fp64:.3f}s")
         ktimes64 = kernel_timing(s, init, MODE_BE, PREC_FP64, args.steps, args.warmup)
         log("[bench] kernel times (fp64): " + ", ".join(f"{k}={v[1] / v[0]:.3f}ms x{v[0]}" for k, v in ktimes64.items()))
+        # the same steps with the exact-Jacobian FP64 solve (the mixed path's
+        # solver in FP64): it tells the droplets where the reference's
+        # FD-Jacobian solve failed (and reset the droplet) from precision effects
+        run_steps(s, init, MODE_BE, PREC_FP64_AJ, args.steps, args.warmup)
+        end_aj = s.column("radius")
         rr = np.abs(end_mixed["radius"] / end_fp64["radius"] - 1.0)
+        r_aj = np.abs(end_mixed["radius"] / end_aj - 1.0)
+        fd_fail = np.abs(end_fp64["radius"] / end_aj - 1.0) > 1e-5
         fp64 = {
             "value": n * args.steps / t_fp64, "ms_per_step": 1e3 * t_fp64 / args.steps,
             "newton_iters_per_solve": cnt_fp64["gn_iters"] / max(cnt_fp64["solves"], 1),
@@ -508,8 +515,12 @@ def main():
         }
         accuracy = {"radius_max_rel_err_mixed_vs_fp64": float(rr.max()),
                     "radius_p99_rel_err_mixed_vs_fp64": float(np.quantile(rr, 0.99)),
+                    "radius_max_rel_err_mixed_vs_fp64_fd_solve_converged": float(rr[~fd_fail].max()),
+                    "droplets_where_fp64_fd_solve_failed": int(fd_fail.sum()),
+                    "radius_max_rel_err_mixed_vs_fp64_exact_jacobian": float(r_aj.max()),
                     "Tp_max_abs_err_K": float(np.abs(end_mixed["Tp"] - end_fp64["Tp"]).max()),
-                    "z_max_abs_err_m": float(np.abs(end_mixed["z"] - end_fp64["z"]).max())}
+                    "z_max_abs_err_m": float(np.abs(end_mixed["z"] - end_fp64["z"]).max()),
+                    "vz_max_abs_err_m_s": float(np.abs(end_mixed["vz"] - end_fp64["vz"]).max())}
 
     # the drop-in path: one particle_update_BE (+ particle_coupling_update) call per step
     onestep = None

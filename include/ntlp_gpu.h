@@ -315,6 +315,14 @@ int ntlp_gpu_kernel_times(char *names, int64_t *launches, double *total_ms, int3
  * in GB/s of bytes read plus bytes written.  Allocates and frees its two
  * buffers. */
 int ntlp_gpu_copy_bandwidth(int64_t bytes, int32_t reps, double *gbps);
+/* Options of the fused step and the serving loop; they have no counterpart
+ * in the reference.  NTLP_OPT_STAGED_GATHER: 1 stages each workgroup's
+ * trilinear field nodes (the corner columns of its droplets' cells) in LDS
+ * before the gathers, 0 (the default) gathers straight through the caches;
+ * both give the same bits, and the direct gather measured faster on MI355X.
+ * Not while a serving loop is open. */
+#define NTLP_OPT_STAGED_GATHER 1
+int ntlp_gpu_set_option(int32_t option, int64_t value);
 /* Waits for all work of the context. */
 int ntlp_gpu_sync(void);
 

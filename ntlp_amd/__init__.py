@@ -23,7 +23,7 @@ from .grid import Grid
 
 __all__ = ["Grid", "Consts", "Flags", "Particles", "Ensemble", "DropletStep", "ServeLoop", "PinnedArray", "NtlpError",
            "load_library", "DIST_LOGNORMAL", "DIST_LOGUNIFORM", "SALT_FIXED", "SALT_DRY_LOGNORMAL", "SALT_CHOICE3",
-           "NREAL", "COL", "MODE_RK3", "MODE_BE", "PREC_FP64", "PREC_MIXED", "PREC_FP64_AJ",
+           "NREAL", "COL", "MODE_RK3", "MODE_BE", "PREC_FP64", "PREC_MIXED", "PREC_FP64_AJ", "OPT_STAGED_GATHER",
            "LAYOUT_EXT", "LAYOUT_CORE", "LAYOUT_XPENCIL", "LIB_PATH"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libntlp_gpu.so")
@@ -38,6 +38,7 @@ NREAL = len(COLUMNS)
 
 MODE_RK3, MODE_BE = 1, 2
 PREC_FP64, PREC_MIXED, PREC_FP64_AJ = 0, 1, 2
+OPT_STAGED_GATHER = 1
 LAYOUT_EXT, LAYOUT_CORE, LAYOUT_XPENCIL = 0, 1, 2
 GN_HIST = 64
 STEP_ROW = 72
@@ -284,6 +285,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "ntlp_gpu_kernel_times": (ctypes.c_int, [vp, vp, vp, i32]),
         "ntlp_gpu_copy_bandwidth": (ctypes.c_int, [i64, i32, vp]),
         "ntlp_gpu_sync": (ctypes.c_int, []),
+        "ntlp_gpu_set_option": (ctypes.c_int, [i32, i64]),
         "ntlp_serve_open": (ctypes.c_int, [ctypes.POINTER(_CServeConfig), ctypes.POINTER(i64)]),
         "ntlp_serve_submit": (ctypes.c_int, [vp, i64, vp, i64, vp, i64, vp, ctypes.POINTER(i64)]),
         "ntlp_host_alloc": (ctypes.c_int, [i64, ctypes.POINTER(vp)]),
@@ -526,6 +528,12 @@ class DropletStep:
             nm = names.raw[32 * i:32 * (i + 1)].split(b"\0", 1)[0].decode()
             out[nm] = (int(launches[i]), float(total[i]))
         return out
+
+    def set_option(self, option: int, value: int) -> None:
+        """ntlp_gpu_set_option: OPT_STAGED_GATHER 1 stages each workgroup's
+        field nodes in LDS before the trilinear gathers (0, the default:
+        gathers through the caches)."""
+        self._check(self.lib.ntlp_gpu_set_option(int(option), int(value)))
 
     def sync(self) -> None:
         self._check(self.lib.ntlp_gpu_sync())

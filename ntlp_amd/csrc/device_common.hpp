@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+
 namespace ntlp {
 
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
@@ -49,8 +51,6 @@ struct DevConsts {
   double irhos;           // 1 / rhos
   double nuf18;           // 18 nuf
   double isurf_rho;       // 1 / surf_rho
-  double Mw2Gam_Ru_rhow_um;  // Mw2Gam_Ru_rhow in micrometre units (x 1e6)
-  double isol_um3;        // isol in micrometre units (x 1e18)
   // 0.6 Sc^(1/3), 0.6 Pra^(1/3) of Sherwood/Nusselt: read from the constants
   // (scalar registers) rather than kept per droplet in vector registers
   double sc6, pr6;
@@ -172,14 +172,53 @@ struct Fluid {
   double T, q;
 };
 
+// Node values of the trilinear gathers: the ext fields in HBM, or a
+// workgroup's box of them staged in LDS.  The corner column (i, j) of a
+// droplet's cell starts at element col00 + sj*j + si*i and level k sits at
+// +k.  Element indices are 32-bit and the byte offset of an element is
+// formed in 32 bits (the ext arrays are limited to 4 GB), so that a load is
+// a scalar base plus one 32-bit vector offset instead of a 64-bit vector
+// address per corner.
+struct NodeView {
+  const double *u, *v, *w, *T, *q;
+  const float *u32, *v32, *w32;
+  unsigned sj, si;
+};
+
+template <typename T>
+__device__ __forceinline__ T node_at(const T* base, unsigned idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (unsigned)sizeof(T));
+}
+
+__device__ __forceinline__ NodeView hbm_view(const DevFields& F) {
+  NodeView V;
+  V.u = F.u;
+  V.v = F.v;
+  V.w = F.w;
+  V.T = F.T;
+  V.q = F.q;
+  V.u32 = F.u32;
+  V.v32 = F.v32;
+  V.w32 = F.w32;
+  V.sj = (unsigned)F.sj;
+  V.si = (unsigned)F.si;
+  return V;
+}
+
+// element index of the droplet's (0, 0) corner column in the ext arrays
+__device__ __forceinline__ unsigned hbm_col00(const DevGrid& g, const DevFields& F, const Cell& c) {
+  const int ic = clampi(c.ipt, -1, g.nnx + 2), jc = clampi(c.jpt, -1, g.nny + 2);
+  return (unsigned)F.sj * (unsigned)(jc + 1) + (unsigned)F.si * (unsigned)(ic + 1);
+}
+
 // uf_interp_lin (particles.f90:767-847) in FP64, same product order
 // ((f*wtx)*wty)*wtz and (i,j,k) accumulation order.  The zeroth-order wall
 // override of 831-842 leaves the node (ipt+1, jpt+1) values in uf(1:2), Tf and
 // qinf.  Indices are clamped only where the reference would leave its arrays.
 template <bool TQ>
-__device__ __forceinline__ Fluid gather_lin_f64(const DevGrid& g, const DevFields& F,
-                                                const double* s_z, const double* s_zz, double x,
-                                                double y, double z, const Cell& c) {
+__device__ __forceinline__ Fluid gather_lin_f64_v(const DevGrid& g, const NodeView& V, unsigned col00,
+                                                  const double* s_z, const double* s_zz, double x, double y,
+                                                  double z, const Cell& c) {
   const int nnz = g.nnz, kmax = nnz + 1;
   const int kpt = c.kpt, kwpt = c.kwpt;
   const double dzu1 = g.dzu[clampi(kpt + 1, 0, nnz + 2)];
@@ -194,35 +233,41 @@ __device__ __forceinline__ Fluid gather_lin_f64(const DevGrid& g, const DevField
     wtx[k] = 1.0 - fabs(x - g.dx * (double)(k + c.ipt - 1)) / g.dx;
     wty[k] = 1.0 - fabs(y - g.dy * (double)(k + c.jpt - 1)) / g.dy;
   }
-  const int ic = clampi(c.ipt, -1, g.nnx + 2), jc = clampi(c.jpt, -1, g.nny + 2);
   Fluid f = {{0.0, 0.0, 0.0}, 0.0, 0.0};
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int64_t col = F.sj * (int64_t)(jc + j + 1) + F.si * (int64_t)(ic + i + 1);
+      const unsigned col = col00 + V.sj * j + V.si * i;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        const int64_t o = col + ku[k], ow = col + kw[k];
-        f.u[0] = f.u[0] + F.u[o] * wtx[i] * wty[j] * wtz[k];
-        f.u[1] = f.u[1] + F.v[o] * wtx[i] * wty[j] * wtz[k];
-        f.u[2] = f.u[2] + F.w[ow] * wtx[i] * wty[j] * wtw[k];
+        const unsigned o = col + ku[k], ow = col + kw[k];
+        f.u[0] = f.u[0] + node_at(V.u, o) * wtx[i] * wty[j] * wtz[k];
+        f.u[1] = f.u[1] + node_at(V.v, o) * wtx[i] * wty[j] * wtz[k];
+        f.u[2] = f.u[2] + node_at(V.w, ow) * wtx[i] * wty[j] * wtw[k];
         if (TQ) {
-          f.T = f.T + F.T[o] * wtx[i] * wty[j] * wtz[k];
-          f.q = f.q + F.q[o] * wtx[i] * wty[j] * wtz[k];
+          f.T = f.T + node_at(V.T, o) * wtx[i] * wty[j] * wtz[k];
+          f.q = f.q + node_at(V.q, o) * wtx[i] * wty[j] * wtz[k];
         }
       }
     }
   if (kpt == nnz || kpt == 0) {
-    const int64_t o = ext_off(F, kpt == 0 ? 1 : kpt, jc + 1, ic + 1);
-    f.u[0] = F.u[o];
-    f.u[1] = F.v[o];
+    const unsigned o = col00 + V.sj + V.si + (unsigned)(kpt == 0 ? 1 : kpt);
+    f.u[0] = node_at(V.u, o);
+    f.u[1] = node_at(V.v, o);
     if (TQ) {
-      f.T = F.T[o];
-      f.q = F.q[o];
+      f.T = node_at(V.T, o);
+      f.q = node_at(V.q, o);
     }
   }
   return f;
+}
+
+template <bool TQ>
+__device__ __forceinline__ Fluid gather_lin_f64(const DevGrid& g, const DevFields& F,
+                                                const double* s_z, const double* s_zz, double x,
+                                                double y, double z, const Cell& c) {
+  return gather_lin_f64_v<TQ>(g, hbm_view(F), hbm_col00(g, F, c), s_z, s_zz, x, y, z, c);
 }
 
 // Mixed-precision trilinear gather: FP32 fractional weights (computed from the
@@ -232,9 +277,9 @@ __device__ __forceinline__ Fluid gather_lin_f64(const DevGrid& g, const DevField
 // trilinear weights sum to one, and keeps the FP32 weight rounding off the
 // 300 K offset of T.  Same wall override as the FP64 gather.
 template <bool TQ>
-__device__ __forceinline__ Fluid gather_lin_mixed(const DevGrid& g, const DevFields& F,
-                                                  const double* s_z, const double* s_zz,
-                                                  double x, double y, double z, const Cell& c) {
+__device__ __forceinline__ Fluid gather_lin_mixed_v(const DevGrid& g, const NodeView& V, unsigned col00,
+                                                    const double* s_z, const double* s_zz,
+                                                    double x, double y, double z, const Cell& c) {
   const int nnz = g.nnz, kmax = nnz + 1;
   const int kpt = c.kpt, kwpt = c.kwpt;
   const float dzu1 = (float)g.dzu[clampi(kpt + 1, 0, nnz + 2)];
@@ -247,27 +292,25 @@ __device__ __forceinline__ Fluid gather_lin_mixed(const DevGrid& g, const DevFie
   const float fw = (float)(z - s_z[kw0]) / dzw1;
   const float wtx[2] = {1.0f - fx, fx}, wty[2] = {1.0f - fy, fy};
   const float wtz[2] = {1.0f - fz, fz}, wtw[2] = {1.0f - fw, fw};
-  const int ic = clampi(c.ipt, -1, g.nnx + 2), jc = clampi(c.jpt, -1, g.nny + 2);
   float u0 = 0.f, u1 = 0.f, u2 = 0.f;
   double T = 0.0, q = 0.0, T0 = 0.0, q0 = 0.0;
-  const int64_t col00 = F.sj * (int64_t)(jc + 1) + F.si * (int64_t)(ic + 1);
   if (TQ) {
-    T0 = F.T[col00 + ku0];
-    q0 = F.q[col00 + ku0];
+    T0 = node_at(V.T, col00 + ku0);
+    q0 = node_at(V.q, col00 + ku0);
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int64_t col = col00 + F.sj * j + F.si * i;
+      const unsigned col = col00 + V.sj * j + V.si * i;
       const float wxy = wtx[i] * wty[j];
       const float a0 = wxy * wtz[0], a1 = wxy * wtz[1];
-      u0 += F.u32[col + ku0] * a0 + F.u32[col + ku1] * a1;
-      u1 += F.v32[col + ku0] * a0 + F.v32[col + ku1] * a1;
-      u2 += F.w32[col + kw0] * (wxy * wtw[0]) + F.w32[col + kw1] * (wxy * wtw[1]);
+      u0 += node_at(V.u32, col + ku0) * a0 + node_at(V.u32, col + ku1) * a1;
+      u1 += node_at(V.v32, col + ku0) * a0 + node_at(V.v32, col + ku1) * a1;
+      u2 += node_at(V.w32, col + kw0) * (wxy * wtw[0]) + node_at(V.w32, col + kw1) * (wxy * wtw[1]);
       if (TQ) {
-        T += (F.T[col + ku0] - T0) * (double)a0 + (F.T[col + ku1] - T0) * (double)a1;
-        q += (F.q[col + ku0] - q0) * (double)a0 + (F.q[col + ku1] - q0) * (double)a1;
+        T += (node_at(V.T, col + ku0) - T0) * (double)a0 + (node_at(V.T, col + ku1) - T0) * (double)a1;
+        q += (node_at(V.q, col + ku0) - q0) * (double)a0 + (node_at(V.q, col + ku1) - q0) * (double)a1;
       }
     }
   Fluid f;
@@ -277,15 +320,163 @@ __device__ __forceinline__ Fluid gather_lin_mixed(const DevGrid& g, const DevFie
   f.T = T0 + T;
   f.q = q0 + q;
   if (kpt == nnz || kpt == 0) {
-    const int64_t o = ext_off(F, kpt == 0 ? 1 : kpt, jc + 1, ic + 1);
-    f.u[0] = F.u32[o];
-    f.u[1] = F.v32[o];
+    const unsigned o = col00 + V.sj + V.si + (unsigned)(kpt == 0 ? 1 : kpt);
+    f.u[0] = node_at(V.u32, o);
+    f.u[1] = node_at(V.v32, o);
     if (TQ) {
-      f.T = F.T[o];
-      f.q = F.q[o];
+      f.T = node_at(V.T, o);
+      f.q = node_at(V.q, o);
     }
   }
   return f;
+}
+
+template <bool TQ>
+__device__ __forceinline__ Fluid gather_lin_mixed(const DevGrid& g, const DevFields& F,
+                                                  const double* s_z, const double* s_zz,
+                                                  double x, double y, double z, const Cell& c) {
+  return gather_lin_mixed_v<TQ>(g, hbm_view(F), hbm_col00(g, F, c), s_z, s_zz, x, y, z, c);
+}
+
+// ------------------------------------------------- LDS-staged gather
+
+// The field box of a workgroup: the ext nodes of the corner columns of every
+// cell its droplets occupy, staged in LDS before the gather.  Box column
+// (ii, jj) holds ext column ipt = i0 + 1 + ii, jpt = j0 + 1 + jj (i0, j0: the
+// smallest cell indices ipt-1, jpt-1 of the workgroup's droplets) at levels
+// k0 .. k0 + nk - 1, level fastest.  ni == 0: no box (every droplet gathers
+// from HBM).
+struct FieldBox {
+  int i0, j0, k0, ni, nj, nk;
+  NodeView V;  // the box's node arrays in LDS (sj = nk, si = nj * nk)
+};
+
+// bytes per staged node: FP32 u, v, w and FP64 T, q (mixed), or five FP64
+__host__ __device__ constexpr int box_node_bytes(bool mixed) { return mixed ? 28 : 40; }
+
+// True when every node the trilinear gather of cell c reads is in the box.
+__device__ __forceinline__ bool box_has(const FieldBox& b, const DevGrid& g, const Cell& c) {
+  const int ii = c.ipt - 1 - b.i0, jj = c.jpt - 1 - b.j0;
+  const int kmax = g.nnz + 1, k1 = b.k0 + b.nk - 1;
+  const int ku0 = clampi(c.kpt, 0, kmax), ku1 = clampi(c.kpt + 1, 0, kmax);
+  const int kw0 = clampi(c.kwpt, 0, kmax), kw1 = clampi(c.kwpt + 1, 0, kmax);
+  return b.ni > 0 && ii >= 0 && ii <= b.ni - 2 && jj >= 0 && jj <= b.nj - 2 && ku0 >= b.k0 && ku1 <= k1 &&
+         kw0 >= b.k0 && kw1 <= k1;
+}
+
+// Trilinear gather of cell c from the box when it holds the cell's nodes,
+// from HBM otherwise; both read the same node values with the same
+// arithmetic, so the result does not depend on where the nodes came from.
+template <bool MIXED>
+__device__ __forceinline__ Fluid gather_lin_staged(const DevGrid& g, const DevFields& F, const FieldBox& b,
+                                                   const double* s_z, const double* s_zz, double x, double y,
+                                                   double z, const Cell& c) {
+  // one path per wave: the box when it holds the cells of all the wave's
+  // droplets (in the workgroup that the box was made for, every in-domain
+  // droplet's), HBM otherwise
+  if (__all(box_has(b, g, c))) {
+    const unsigned col00 = (unsigned)(((c.ipt - 1 - b.i0) * b.nj + (c.jpt - 1 - b.j0)) * b.nk - b.k0);
+    return MIXED ? gather_lin_mixed_v<true>(g, b.V, col00, s_z, s_zz, x, y, z, c)
+                 : gather_lin_f64_v<true>(g, b.V, col00, s_z, s_zz, x, y, z, c);
+  }
+  return MIXED ? gather_lin_mixed<true>(g, F, s_z, s_zz, x, y, z, c) : gather_lin_f64<true>(g, F, s_z, s_zz, x, y, z, c);
+}
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Stages the field box of the workgroup's droplets (u-cell keys, k fastest;
+// key < 0 or !live: no droplet) into s_box (box_bytes of LDS).  All threads
+// of the workgroup call it (it holds two barriers); s_mm: 6 * (blockDim/64)
+// ints of LDS.  Returns ni == 0 when the box would not fit.
+template <bool MIXED>
+__device__ FieldBox stage_box(const DevGrid& g, const DevFields& F, int key, bool live, double* s_box,
+                              int box_bytes, int* s_mm) {
+  const int nz1 = g.nnz + 1, ny1 = g.nny + 1;
+  int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX, INT_MAX};
+  if (live && key >= 0) {
+    const int k = key % nz1, rest = key / nz1;
+    const int j = rest % ny1, i = rest / ny1;
+    v[0] = i, v[1] = -i, v[2] = j, v[3] = -j, v[4] = k, v[5] = -k;
+  }
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;  // waves of the workgroup
+#pragma unroll
+  for (int q = 0; q < 6; ++q) v[q] = wave_min_i32(v[q]);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s_mm[w * 6 + q] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    int m = s_mm[q];
+    for (int k = 1; k < nw; ++k) m = min(m, s_mm[k * 6 + q]);
+    v[q] = __builtin_amdgcn_readfirstlane(m);
+  }
+  FieldBox b;
+  b.ni = 0;
+  if (v[0] == INT_MAX) return b;  // no droplet: the same for every thread
+  b.i0 = v[0];
+  b.j0 = v[2];
+  b.ni = -v[1] - v[0] + 2;
+  b.nj = -v[3] - v[2] + 2;
+  b.k0 = max(v[4] - 1, 0);
+  b.nk = min(-v[5] + 1, g.nnz + 1) - b.k0 + 1;
+  // in 64 bits: a workgroup whose droplets sit at both ends of the domain (a
+  // droplet that crossed the periodic boundary since the slots were last
+  // sorted) spans the whole grid, whose byte count overflows 32 bits
+  const int64_t box_size = (int64_t)b.ni * b.nj * b.nk * box_node_bytes(MIXED);
+  if (box_size > box_bytes) {
+    b.ni = 0;
+    return b;
+  }
+  const int nodes = b.ni * b.nj * b.nk;
+  double* sT = s_box;
+  double* sq = s_box + nodes;
+  b.V.T = sT;
+  b.V.q = sq;
+  b.V.sj = (unsigned)b.nk;
+  b.V.si = (unsigned)(b.nj * b.nk);
+  float *su32 = nullptr, *sv32 = nullptr, *sw32 = nullptr;
+  double *su = nullptr, *sv = nullptr, *sw = nullptr;
+  if (MIXED) {
+    su32 = reinterpret_cast<float*>(s_box + 2 * nodes);
+    sv32 = su32 + nodes;
+    sw32 = sv32 + nodes;
+  } else {
+    su = s_box + 2 * nodes;
+    sv = su + nodes;
+    sw = sv + nodes;
+  }
+  b.V.u32 = su32;
+  b.V.v32 = sv32;
+  b.V.w32 = sw32;
+  b.V.u = su;
+  b.V.v = sv;
+  b.V.w = sw;
+  // consecutive threads take consecutive levels of a box column (coalesced)
+  const unsigned gsj = (unsigned)F.sj, gsi = (unsigned)F.si;
+  const unsigned g0 = gsj * (unsigned)(b.j0 + 2) + gsi * (unsigned)(b.i0 + 2) + (unsigned)b.k0;
+  for (int e = threadIdx.x; e < nodes; e += blockDim.x) {
+    const int col = e / b.nk, l = e - col * b.nk;
+    const int ii = col / b.nj, jj = col - ii * b.nj;
+    const unsigned o = g0 + gsj * (unsigned)jj + gsi * (unsigned)ii + (unsigned)l;
+    sT[e] = node_at(F.T, o);
+    sq[e] = node_at(F.q, o);
+    if (MIXED) {
+      su32[e] = node_at(F.u32, o);
+      sv32[e] = node_at(F.v32, o);
+      sw32[e] = node_at(F.w32, o);
+    } else {
+      su[e] = node_at(F.u, o);
+      sv[e] = node_at(F.v, o);
+      sw[e] = node_at(F.w, o);
+    }
+  }
+  __syncthreads();
+  return b;
 }
 
 // uf_interp (particles.f90:554-765): 6th-order Lagrange on 6x6x6 nodes with

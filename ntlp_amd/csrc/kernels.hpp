@@ -14,6 +14,8 @@
 //                    touched through home[t] (diagnostics, RK3 stage
 //                    registers, pre-step values for the coupling).
 #pragma once
+#include <algorithm>
+
 #include "ensemble.hpp"
 #include "physics.hpp"
 
@@ -94,7 +96,18 @@ struct StepArgs {
   double* diag;          // [kNCold][cap] diagnostics of the final step in dst slot order
   int lazy_j;            // BE library steps: index of this step within its call
   double* flux;          // BE flux statistics [3][nnz+1] (zero on entry), or null
+  int box_bytes;         // LDS for the staged field box of k_step (0: gather from HBM)
 };
+
+// Dynamic LDS of k_step: z and zz, then the field box.  The box is sized so
+// that four 256-thread workgroups (four waves per SIMD, what the step's
+// register use allows) share a CU's 160 KB.
+constexpr int kStepLdsPerBlock = 38912;
+inline int step_box_bytes(int nnz, bool staged) {
+  const int rest = kStepLdsPerBlock - 16 * (nnz + 2) - 4 * (kTallyWidth + 1) - 4 * 6 * (kBlock / 64);
+  return staged ? std::max(0, std::min(32768, rest)) : 0;
+}
+inline size_t step_lds_bytes(int nnz, int box_bytes) { return (size_t)16 * (nnz + 2) + (size_t)box_bytes; }
 
 // Workgroup -> droplet chunk mapping that keeps consecutive chunks on one XCD:
 // gfx950 deals workgroups to its 8 XCDs round-robin, so workgroup b lands on
@@ -371,8 +384,12 @@ __device__ __forceinline__ void count_key(int* counts, int key, bool valid) {
 template <int MODE, int PREC, int ILIN, bool SERVE = false, bool LAZY = false>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   const DevConsts& C = *a.Cp;
-  __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  extern __shared__ double s_dyn[];  // z(0:nnz+1) | zz(0:nnz+1) | field box (a.box_bytes)
   __shared__ int s_red[kTallyWidth + 1];
+  __shared__ int s_mm[6 * (kBlock / 64)];
+  const int nlev = a.g.nnz + 2;
+  double* s_z = s_dyn;
+  double* s_zz = s_dyn + nlev;
   stage_grid(a.g, s_z, s_zz);
   zero_tally_lds(s_red);
   __syncthreads();
@@ -382,24 +399,39 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   int key = -1;
   const bool inplace = !SERVE && a.inplace;
   static_assert(!LAZY || (MODE == 2 && !SERVE), "lazy residence times: library BE steps only");
+  // the droplet's loads are issued before the field box is staged, so that
+  // they are in flight while the box loads
+  int64_t s = t;
+  int kin = -1;
+  Drop d;
+  int64_t mult = 0;
+  int32_t home = 0;
   if (valid) {
-    const int64_t s = (a.perm && !inplace) ? (int64_t)a.perm[t] : t;
-    Drop d = load_drop(a.src.hot, s, !LAZY);
+    s = (a.perm && !inplace) ? (int64_t)a.perm[t] : t;
+    if (a.keys_in) kin = a.keys_in[s];
+    d = load_drop(a.src.hot, s, !LAZY);
+    mult = a.src.mult[s];
+    home = inplace ? 0 : a.src.home[s];
+  }
+  // LDS-staged gather: the trilinear field nodes of the workgroup's cells
+  FieldBox bx;
+  bx.ni = 0;
+  if (ILIN == 1 && a.box_bytes > 0 && a.keys_in)
+    bx = stage_box<PREC == PREC_MIXED>(a.g, a.F, kin, valid, s_dyn + 2 * nlev, a.box_bytes, s_mm);
+  if (valid) {
     const double numact0 = d.numact;
-    const int64_t mult = a.src.mult[s];
-    const int32_t home = inplace ? 0 : a.src.home[s];
     Diag o;
-    const int kg = a.keys_in ? a.keys_in[s] % (a.g.nnz + 1) : -1;
+    const int kg = a.keys_in ? kin % (a.g.nnz + 1) : -1;
     if (MODE == 1) {  // three particle_update_rk3 stages; stage registers stay in VGPRs
       o.xrhs[0] = o.xrhs[1] = o.xrhs[2] = 0.0;  // zetas(1) = 0 discards the stale values
       o.vrhs[0] = o.vrhs[1] = o.vrhs[2] = 0.0;
       o.Tprhs_s = o.Tprhs_L = o.radrhs = 0.0;
       o.kpt = kg;
       for (int st = 1; st <= 3; ++st)
-        rk3_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, st, a.it, o, o.kpt);
+        rk3_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, st, a.it, o, o.kpt, &bx);
     } else {
       be_droplet<PREC, ILIN, LAZY>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg, SERVE ? nullptr : a.flux,
-                                   mult);
+                                   mult, &bx);
     }
     if (SERVE && a.serve->last[home >> kServeIdxBits]) {
       const int64_t row = a.serve->out_off[home >> kServeIdxBits] + (home & kServeIdxMask);
@@ -829,6 +861,9 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
   __shared__ int s_wave[kBlock / 64];
   const int64_t b = xcd_block();
   if (b >= (int64_t)a.ntx * a.nty * a.ntz) return;  // whole block
+  // boxes with k fastest: consecutive boxes of one XCD share their droplets'
+  // runs and node columns in L2 (the orders with x or y fastest, and boxes
+  // of 1-2 x 8 x 32-130 nodes, measured 1.6-3.7x slower)
   const int kt = (int)(b % a.ntz);
   const int jt = (int)((b / a.ntz) % a.nty);
   const int it = (int)(b / ((int64_t)a.ntz * a.nty));

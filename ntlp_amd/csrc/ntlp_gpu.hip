@@ -108,6 +108,9 @@ struct Ctx {
   bool state_bad = false;
   // a serving loop is open on these fields
   bool serving = false;
+  // the fused step stages each workgroup's trilinear field nodes in LDS
+  // (NTLP_OPT_STAGED_GATHER); off, every gather reads through the caches
+  bool stage_gather = false;
   // timing
   bool timing = false;
   std::vector<Timed> pending;
@@ -324,6 +327,7 @@ StepArgs base_args() {
   a.keys = ctx.keys[ctx.cur];
   a.counts = ctx.counts;
   a.tally = ctx.tally_slots;
+  a.box_bytes = step_box_bytes(ctx.g.nnz, ctx.stage_gather && ctx.fl.ilin == 1);
   return a;
 }
 
@@ -350,8 +354,6 @@ void derive_consts(DevConsts& C) {
   C.irhos = 1.0 / C.rhos;
   C.nuf18 = 18.0 * C.nuf;
   C.isurf_rho = 1.0 / C.surf_rho;
-  C.Mw2Gam_Ru_rhow_um = C.Mw2Gam_Ru_rhow * 1e6;
-  C.isol_um3 = C.isol * 1e18;
   C.sc6 = 0.6 * C.Sc13;
   C.pr6 = 0.6 * C.Pra13;
 }
@@ -637,13 +639,14 @@ void launch_update_t(const StepArgs& a, unsigned grid) {
 }
 template <int MODE, int PREC, int ILIN>
 void launch_step_t(const StepArgs& a, unsigned grid, bool lazy) {
+  const size_t lds = step_lds_bytes(a.g.nnz, a.box_bytes);
   if constexpr (MODE == 2) {
     if (lazy) {
-      k_step<MODE, PREC, ILIN, false, true><<<grid, kBlock, 0, ctx.stream>>>(a);
+      k_step<MODE, PREC, ILIN, false, true><<<grid, kBlock, lds, ctx.stream>>>(a);
       return;
     }
   }
-  k_step<MODE, PREC, ILIN><<<grid, kBlock, 0, ctx.stream>>>(a);
+  k_step<MODE, PREC, ILIN><<<grid, kBlock, lds, ctx.stream>>>(a);
 }
 
 template <int MODE>
@@ -797,6 +800,9 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   if (f->icase != 0) return fail(NTLP_E_UNSUPPORTED, "only icase=0 (reflecting walls) is supported");
   const int64_t cells_u = (int64_t)(grid->nnx + 1) * (grid->nny + 1) * (grid->nnz + 1);
   if (cells_u >= (int64_t)INT32_MAX) return fail(NTLP_E_RANGE, "grid has too many cells for 32-bit cell keys");
+  // the trilinear gathers address the ext arrays with 32-bit byte offsets
+  if ((int64_t)(grid->nnz + 2) * (grid->nny + 5) * (grid->nnx + 5) * 8 > (int64_t)UINT32_MAX)
+    return fail(NTLP_E_RANGE, "ext field arrays beyond 4 GB each (nnx*nny*nnz above about 5.3e8)");
   release_all();
   CHK(setup_scratch());
   HIPCHK(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
@@ -1729,6 +1735,19 @@ int ntlp_gpu_copy_bandwidth(int64_t bytes, int32_t reps, double* gbps) {
   std::sort(ms.begin(), ms.end());
   *gbps = 2.0 * 16.0 * (double)n / (ms[ms.size() / 2] * 1e-3) / 1e9;
   return NTLP_OK;
+}
+
+int ntlp_gpu_set_option(int32_t option, int64_t value) {
+  CHK(need_init());
+  if (ctx.serving) return fail(NTLP_E_STATE, "close the serving loop first");
+  switch (option) {
+    case NTLP_OPT_STAGED_GATHER:
+      if (value != 0 && value != 1) return fail(NTLP_E_INVALID, "NTLP_OPT_STAGED_GATHER takes 0 or 1");
+      ctx.stage_gather = value == 1;
+      return NTLP_OK;
+    default:
+      return fail(NTLP_E_INVALID, "unknown option " + std::to_string(option));
+  }
 }
 
 int ntlp_gpu_sync(void) {

@@ -350,44 +350,15 @@ __device__ __forceinline__ double rad_solver2(const DevConsts& C, double Tf, dou
   return guess;
 }
 
-// The same initial guess in FP32 (mixed precision; the guess only seeds the
-// FP64 solve).  Radii in micrometres keep a^3, c and R^2 inside the FP32
-// range; ln S is formed as log1p(S - 1) from the FP64 difference.
-__device__ __forceinline__ double rad_solver2_f32(const DevConsts& C, double Tf, double rhoa, double qinf,
-                                                  double kappa, double m_s, double r, double esa, int& mflag) {
-  mflag = 0;
-  const double Sm1 = C.Ru_Mw * Tf * rhoa * qinf * rcp64(esa) - 1.0;
-  const float L = log1pf((float)Sm1);
-  const float iL = __builtin_amdgcn_rcpf(L);
-  const float a = -(float)(C.Mw2Gam_Ru_rhow_um * rcp64(Tf)) * iL;
-  const float c = (float)(kappa * m_s * C.isol_um3) * iL;
-  const float Q = (a * a) * (1.0f / 9.0f);
-  const float R = (2.0f * (a * a * a) + 27.0f * c) * (1.0f / 54.0f);
-  const float Q3 = Q * Q * Q;
-  const float M = R * R - Q3;
-  const float pi2 = (float)C.pi2;
-  float guess;
-  if (M < 0.0f) {
-    const float theta = acosf(R * __builtin_amdgcn_rsqf(Q3));
-    const float sQ = sqrtf(Q);
-    guess = -(2.0f * sQ * cosf((theta - pi2) * (1.0f / 3.0f))) - a * (1.0f / 3.0f);
-    if (guess < 0.0f) guess = -(2.0f * sQ * cosf((theta + pi2) * (1.0f / 3.0f))) - a * (1.0f / 3.0f);
-  } else {
-    const float S = -copysignf(1.0f, R) * cbrtf(fabsf(R) + sqrtf(M));
-    guess = S + fdiv32(Q, S) - a * (1.0f / 3.0f);
-    if (guess < 0.0f) {
-      mflag = 1;
-      return r;
-    }
-  }
-  return (double)guess * 1e-6;
-}
-
 // Radius/temperature block of particle_update_BE (3103-3178) on a droplet
 // whose velocity is already the new one.  Updates d.r, d.Tp, activation
 // bookkeeping and the tally.  PREC_FP64: reference residual and FD
-// Jacobian.  Otherwise the closed-form residual with its exact Jacobian; in
-// mixed precision the Koehler guess that starts the solve is formed in FP32.
+// Jacobian.  Otherwise the closed-form residual with its exact Jacobian.
+// Every precision starts from the reference's FP64 Koehler guess: the BE
+// equations can have more than one solution near activation, and which one
+// Newton (or Levenberg-Marquardt) reaches depends on where it starts; an FP32
+// guess (Cardano's discriminant cancels near the Koehler maximum) sent a few
+// hundred droplets per step of the stiff activation case into failed solves.
 template <int PREC>
 __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFlags& fl, Drop& d,
                                                 double Tf, double qinf, double rhoa, double diffnorm,
@@ -399,7 +370,7 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
     // reciprocals of the closed-form solve (the reference-order FP64 solve divides)
     const double ir = (PREC == PREC_FP64) ? 0.0 : rcp64(d.r);
     const double iTp = (PREC == PREC_FP64) ? 0.0 : rcp64(d.Tp);
-    if (PREC != PREC_MIXED) {  // the reference's Koehler start (mixed: below)
+    {  // the reference's start: the FP64 Koehler guess
       const double guess = rad_solver2(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
       if (mflag == 0) {
         rt_start[0] = (PREC == PREC_FP64) ? guess / d.r : guess * ir;
@@ -453,16 +424,6 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
       P.hB = C.hB0 * rhoa * iTp;
       P.Cc = C.Lv3_Cpp * d.r * iTp;
       const AJEval ev{C, P};
-      if (PREC == PREC_MIXED) {
-        // the reference's start, from the Koehler guess formed in FP32: the
-        // BE equations can have more than one solution, and which one Newton
-        // (or LM) reaches depends on where it starts
-        const double guess = rad_solver2_f32(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
-        if (mflag == 0) {
-          rt_start[0] = guess * ir;
-          rt_start[1] = Tf * iTp;
-        }
-      }
       flag = gauss_newton(ev, rt_start, rt_zeroes, it1);
       tl.gn_iters += it1;
       tl.solves += 1;
@@ -538,7 +499,8 @@ struct BeHand {
 template <int PREC, int ILIN>
 __device__ __forceinline__ BeHand be_advance(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
                                              const DevFields& F, const double* s_z, const double* s_zz,
-                                             Drop& d, int it, Diag& o, int kguess) {
+                                             Drop& d, int it, Diag& o, int kguess,
+                                             const FieldBox* bx = nullptr) {
   const double dt = C.dt;
   o.vold[0] = d.vx;
   o.vold[1] = d.vy;
@@ -550,6 +512,7 @@ __device__ __forceinline__ BeHand be_advance(const DevGrid& g, const DevConsts& 
   o.kpt = pack_levels(c.kpt, (c.kwpt == -2) ? -1 : c.kwpt + 1);
   Fluid f;
   if (ILIN != 1) f = gather6_f64(g, F, s_z, s_zz, d.x, d.y, d.z, c);
+  else if (bx) f = gather_lin_staged<PREC == PREC_MIXED>(g, F, *bx, s_z, s_zz, d.x, d.y, d.z, c);
   else if (PREC == PREC_MIXED) f = gather_lin_mixed<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
   else f = gather_lin_f64<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
   if (it <= 1) {
@@ -667,8 +630,9 @@ template <int PREC, int ILIN, bool LAZY = false>
 __device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
                                            const DevFields& F, const double* s_z, const double* s_zz,
                                            Drop& d, int it, Diag& o, Tally& tl, int kguess = -1,
-                                           double* flux = nullptr, int64_t mult = 1) {
-  const BeHand h = be_advance<PREC, ILIN>(g, C, fl, F, s_z, s_zz, d, it, o, kguess);
+                                           double* flux = nullptr, int64_t mult = 1,
+                                           const FieldBox* bx = nullptr) {
+  const BeHand h = be_advance<PREC, ILIN>(g, C, fl, F, s_z, s_zz, d, it, o, kguess, bx);
   be_finish<PREC, LAZY>(g, C, fl, s_z, d, h, o.qstar, tl, o.kpt, flux, mult);
   o.kpt = packed_kpt(o.kpt);
 }
@@ -683,7 +647,8 @@ __device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C,
 template <int PREC, int ILIN>
 __device__ __forceinline__ void rk3_droplet(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
                                             const DevFields& F, const double* s_z, const double* s_zz,
-                                            Drop& d, int istage, int it, Diag& o, int kguess = -1) {
+                                            Drop& d, int istage, int it, Diag& o, int kguess = -1,
+                                            const FieldBox* bx = nullptr) {
   const double dt = C.dt;
   const double zs = C.zetas[istage - 1], gs = C.gama[istage - 1];
   if (istage == 1) {
@@ -697,6 +662,7 @@ __device__ __forceinline__ void rk3_droplet(const DevGrid& g, const DevConsts& C
   o.kpt = c.kpt;
   Fluid f;
   if (ILIN != 1) f = gather6_f64(g, F, s_z, s_zz, d.x, d.y, d.z, c);
+  else if (bx) f = gather_lin_staged<PREC == PREC_MIXED>(g, F, *bx, s_z, s_zz, d.x, d.y, d.z, c);
   else if (PREC == PREC_MIXED) f = gather_lin_mixed<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
   else f = gather_lin_f64<true>(g, F, s_z, s_zz, d.x, d.y, d.z, c);
   double Tf = f.T;

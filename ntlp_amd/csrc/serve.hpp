@@ -254,13 +254,15 @@ StepArgs serve_step_args(int cur) {
   a.out_mult = srv.out_mult;
   a.out_cap = srv.cap;
   a.it = 2;  // droplets are created with vp = uf and Tp = Tf at admission
+  a.box_bytes = step_box_bytes(ctx.g.nnz, ctx.stage_gather && ctx.fl.ilin == 1);
   return a;
 }
 
 template <int PREC>
 void serve_launch_step(const StepArgs& a, unsigned grid) {
-  if (ctx.fl.ilin == 1) k_step<2, PREC, 1, true><<<grid, kBlock, 0, srv.stream>>>(a);
-  else k_step<2, PREC, 0, true><<<grid, kBlock, 0, srv.stream>>>(a);
+  const size_t lds = step_lds_bytes(a.g.nnz, a.box_bytes);
+  if (ctx.fl.ilin == 1) k_step<2, PREC, 1, true><<<grid, kBlock, lds, srv.stream>>>(a);
+  else k_step<2, PREC, 0, true><<<grid, kBlock, lds, srv.stream>>>(a);
 }
 
 // The per-iteration kernels for a pool of at most `size` slots: counting sort

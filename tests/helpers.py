@@ -70,3 +70,45 @@ def edge_positions(grid: Grid) -> np.ndarray:
 
 def rel_err(a: np.ndarray, b: np.ndarray, floor: float) -> float:
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))) if a.size else 0.0
+
+
+def radius_outliers(fp64: np.ndarray, mixed: np.ndarray, fp64_aj: np.ndarray, bound: float = 1e-5):
+    """Mixed-precision radii against the FP64 reference path, with the FP64
+    exact-Jacobian path (PREC_FP64_AJ) as the arbiter.  The mixed solve uses
+    the exact Jacobian; where the reference's forward-difference Jacobian
+    makes its solve fail (Gauss-Newton, then Levenberg-Marquardt, then the
+    reset to the pre-step radius, particles.f90:3127-3153) the exact-Jacobian
+    solve converges, and the two FP64 paths differ by that step's growth.
+    Returns (relative errors mixed vs fp64, mixed vs fp64_aj, indices above
+    the bound that the FP64 paths' own disagreement does not explain)."""
+    e_ref = np.abs(mixed / fp64 - 1.0)
+    e_aj = np.abs(mixed / fp64_aj - 1.0)
+    ref_aj = np.abs(fp64 / fp64_aj - 1.0)
+    unexplained = np.flatnonzero((e_ref > bound) & ~((ref_aj > bound) & (e_aj <= bound)))
+    return e_ref, e_aj, unexplained
+
+
+def oracle_spread(ora, before, idx, its, m: int = 24):
+    """(lo, hi), each (2, len(idx)), of the radius and Tp that the oracle's BE
+    steps `its` give for the droplets idx of `before` when each droplet's
+    radius or temperature is moved by 1..m ulps either way: how far last-ulp
+    differences (those of exp/log between the device and the host) move an
+    ill-conditioned solve."""
+    k = np.arange(1, m + 1, dtype=float) * 2.0 ** -52
+    scale = np.concatenate([1 + k, 1 - k])
+    nv = 2 * scale.size
+    idx = np.asarray(idx)
+    q = before.copy()
+    q.soa = np.ascontiguousarray(np.repeat(before.soa[:, idx], nv, axis=1))
+    q.mult = np.repeat(before.mult[idx], nv)
+    rad = q["radius"].reshape(idx.size, nv)
+    tp = q["Tp"].reshape(idx.size, nv)
+    rad[:, : scale.size] *= scale
+    tp[:, scale.size:] *= scale
+    q["radius"] = rad.ravel()
+    q["Tp"] = tp.ravel()
+    for it in its:
+        ora.update_be(q.soa, it)
+    r = q["radius"].reshape(idx.size, nv)
+    t = q["Tp"].reshape(idx.size, nv)
+    return np.stack([r.min(1), t.min(1)]), np.stack([r.max(1), t.max(1)])

@@ -17,7 +17,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from ntlp_amd import COL, MODE_BE, MODE_RK3, PREC_FP64, PREC_MIXED, DropletStep, Flags
+from ntlp_amd import COL, MODE_BE, MODE_RK3, OPT_STAGED_GATHER, PREC_FP64, PREC_MIXED, DropletStep, Flags
 
 from .helpers import rel_err, small_case
 from .test_gpu_parity import TOL64, compare, make_step
@@ -60,6 +60,50 @@ def borderline(want, before, tol=1e-7):
     return np.abs(want["radius"] / want["rc"] - 1.0) < tol
 
 
+def ulp_spread(ora, before, idx, it=2, m=24):
+    """(lo, hi) of the radius and Tp that the oracle's BE step gives for each
+    droplet of idx when its radius or temperature is moved by 1..m ulps either
+    way: how far last-ulp differences (like those of exp/log between the device
+    and the host) can move an ill-conditioned solve.  Arrays (2, len(idx))."""
+    k = np.arange(1, m + 1, dtype=float) * 2.0 ** -52
+    scale = np.concatenate([1 + k, 1 - k])
+    nv = 2 * scale.size
+    idx = np.asarray(idx)
+    q = before.copy()
+    q.soa = np.repeat(before.soa[:, idx], nv, axis=1)
+    q.mult = np.repeat(before.mult[idx], nv)
+    sc = np.tile(scale, idx.size).reshape(idx.size, scale.size)
+    rad = q["radius"].reshape(idx.size, nv)
+    tp = q["Tp"].reshape(idx.size, nv)
+    rad[:, : scale.size] *= sc
+    tp[:, scale.size:] *= sc
+    q["radius"] = rad.ravel()
+    q["Tp"] = tp.ravel()
+    ora.update_be(q.soa, it)
+    r = q["radius"].reshape(idx.size, nv)
+    t = q["Tp"].reshape(idx.size, nv)
+    return np.stack([r.min(1), t.min(1)]), np.stack([r.max(1), t.max(1)])
+
+
+def ulp_outcomes(ora, before, i, it=2, m=24):
+    """The reset outcomes (radius kept and Tp = Tf) that the oracle gives for
+    droplet i of the pre-step state when its radius or temperature is moved
+    by 1..m ulps either way: the spread of outcomes of an ill-conditioned
+    solve under last-ulp differences like those between the device's and the
+    host's exp/log."""
+    k = np.arange(1, m + 1, dtype=float) * 2.0 ** -52
+    scale = np.concatenate([1 + k, 1 - k])
+    nv = 2 * scale.size
+    q = before.copy()
+    q.soa = np.repeat(before.soa[:, i:i + 1], nv, axis=1)
+    q.mult = np.repeat(before.mult[i:i + 1], nv)
+    q["radius"][: scale.size] *= scale
+    q["Tp"][scale.size:] *= scale
+    r0 = q["radius"].copy()
+    ora.update_be(q.soa, it)
+    return set(((q["radius"] == r0) & (np.abs(q["Tp"] - q["Tf"]) < 1e-10)).tolist())
+
+
 def test_newton_counters_lm_fallback_and_guard_match_oracle():
     """A large step (dt = 0.05 s) in RH 80-120 %: Gauss-Newton fails for a fifth
     of the droplets, which fall back to Levenberg-Marquardt (num100); a few fail
@@ -86,11 +130,26 @@ def test_newton_counters_lm_fallback_and_guard_match_oracle():
     reset_o = (want["radius"] == p["radius"]) & (np.abs(want["Tp"] - want["Tf"]) < 1e-10)
     reset_d = (got["radius"] == p["radius"]) & (np.abs(got["Tp"] - got["Tf"]) < 1e-10)
     assert reset_o[:3].all() and reset_d[:3].all() and reset_o.sum() == cnt_o["numimpos"]
-    assert np.count_nonzero(reset_d != reset_o) <= abs(cnt["numimpos"] - cnt_o["numimpos"]) + 2
-    # droplets whose solve is well conditioned agree to rounding; at most the
-    # flipped ones (about 1 %) differ
-    same = (np.abs(got["radius"] / want["radius"] - 1) < 1e-7) & (np.abs(got["Tp"] - want["Tp"]) < 3e-6)
-    assert np.mean(same) >= 0.99, np.mean(same)
+    # a droplet reset on one side only is one whose Levenberg-Marquardt outcome
+    # the oracle itself changes under a few ulps of input: the device's outcome
+    # is among the oracle's
+    flipped = np.flatnonzero(reset_d != reset_o)
+    assert flipped.size <= max(4, 0.02 * cnt_o["num100"]), flipped
+    for i in flipped:
+        assert bool(reset_d[i]) in ulp_outcomes(ora, p, i), (i, bool(reset_d[i]), bool(reset_o[i]))
+    # droplets whose Gauss-Newton solve converges agree to rounding: those that
+    # differ are no more than the oracle's Levenberg-Marquardt fallbacks, and
+    # each differs by no more than the oracle's own result moves under a few
+    # ulps of input
+    same = (np.abs(got["radius"] / want["radius"] - 1) < 1e-9) & (np.abs(got["Tp"] - want["Tp"]) < 3e-9)
+    diff = np.flatnonzero(~same)
+    assert diff.size <= cnt_o["num100"], (diff.size, cnt_o["num100"])
+    lo, hi = ulp_spread(ora, p, diff)
+    tol_r = 2 * (hi[0] - lo[0]) + 1e-9 * want["radius"][diff]
+    tol_t = 2 * (hi[1] - lo[1]) + 3e-9
+    dr, dt_ = np.abs(got["radius"][diff] - want["radius"][diff]), np.abs(got["Tp"][diff] - want["Tp"][diff])
+    bad = diff[(dr > tol_r) | (dt_ > tol_t)]
+    assert bad.size == 0, (bad[:10], dr.max(), dt_.max())
     s.close()
 
 
@@ -310,3 +369,30 @@ def test_mixed_precision_stated_errors():
     assert not bad, bad
     assert max(src_errs) <= 1e-2, src_errs
     s.close()
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_MIXED])
+def test_staged_gather_gives_the_same_bits(prec):
+    """NTLP_OPT_STAGED_GATHER: the fused step's trilinear gathers from the
+    workgroup's field box in LDS give the same bits as the gathers straight
+    from HBM -- BE over 20 steps (a reordering inside, droplets crossing cells
+    and walls, the coupling sources) and RK3 (three gathers per step, at
+    positions that move between the stages).  The coupling sources are sums
+    in LDS atomics, whose order varies from run to run: they agree to 1e-12
+    of each field's largest value."""
+    grid, consts, flags, fields, p, ora = small_case(seed=31, n=20000, kind="stretched")
+    res = {}
+    for staged in (0, 1):
+        s = make_step(grid, consts, flags, fields, p)
+        s.set_option(OPT_STAGED_GATHER, staged)
+        s.step(MODE_BE, prec, nsteps=20, it0=2)
+        be, src = s.get(), s.sources()
+        s.put(p)
+        s.step(MODE_RK3, prec, nsteps=3, it0=2)
+        res[staged] = (be.soa, src, s.get().soa)
+        s.close()
+    assert np.array_equal(res[0][0], res[1][0], equal_nan=True)
+    assert np.array_equal(res[0][2], res[1][2], equal_nan=True)
+    for f in range(6):
+        scale = np.abs(res[0][1][f]).max() + 1e-300
+        assert np.abs(res[0][1][f] - res[1][1][f]).max() <= 1e-12 * scale, f
