@@ -270,13 +270,39 @@ __device__ __forceinline__ Fluid gather_lin_f64(const DevGrid& g, const DevField
   return gather_lin_f64_v<TQ>(g, hbm_view(F), hbm_col00(g, F, c), s_z, s_zz, x, y, z, c);
 }
 
+// Levels k and k+1 of one ext column in one load (the arrays are level
+// fastest, so the two nodes are adjacent): an 8-byte load of two FP32 nodes
+// or a 16-byte load of two FP64 nodes, at the element's own alignment.
+__device__ __forceinline__ void node_pair(const float* base, unsigned idx, float& lo, float& hi) {
+  const char* p = static_cast<const char*>(
+      __builtin_assume_aligned(reinterpret_cast<const char*>(base) + idx * 4u, 4));
+  float v[2];
+  __builtin_memcpy(v, p, 8);
+  lo = v[0];
+  hi = v[1];
+}
+__device__ __forceinline__ void node_pair(const double* base, unsigned idx, double& lo, double& hi) {
+  const char* p = static_cast<const char*>(
+      __builtin_assume_aligned(reinterpret_cast<const char*>(base) + idx * 8u, 8));
+  double v[2];
+  __builtin_memcpy(v, p, 16);
+  lo = v[0];
+  hi = v[1];
+}
+
 // Mixed-precision trilinear gather: FP32 fractional weights (computed from the
 // FP64 offset of the droplet from its cell origin), u, v, w from FP32 field
 // copies accumulated in FP32; T and q from the FP64 fields in the form
 // f000 + sum w_n (f_n - f000), which equals the plain weighted sum because the
 // trilinear weights sum to one, and keeps the FP32 weight rounding off the
 // 300 K offset of T.  Same wall override as the FP64 gather.
-template <bool TQ>
+// PAIR (HBM): each corner column's two levels come in one load (node_pair)
+// from levels kp, kp+1 with kp = min(ku0, kmax-1), which stay inside the
+// column; a droplet whose levels clamp to one (outside the domain
+// vertically) has both vertical weights moved onto that level.  For every
+// other droplet the values and the arithmetic are those of the two-load form
+// (LDS), which the staged gather therefore uses only for such droplets.
+template <bool TQ, bool PAIR = false>
 __device__ __forceinline__ Fluid gather_lin_mixed_v(const DevGrid& g, const NodeView& V, unsigned col00,
                                                     const double* s_z, const double* s_zz,
                                                     double x, double y, double z, const Cell& c) {
@@ -291,28 +317,75 @@ __device__ __forceinline__ Fluid gather_lin_mixed_v(const DevGrid& g, const Node
   const float fz = (float)(z - s_zz[ku0]) / dzu1;
   const float fw = (float)(z - s_z[kw0]) / dzw1;
   const float wtx[2] = {1.0f - fx, fx}, wty[2] = {1.0f - fy, fy};
-  const float wtz[2] = {1.0f - fz, fz}, wtw[2] = {1.0f - fw, fw};
+  float wtz[2] = {1.0f - fz, fz}, wtw[2] = {1.0f - fw, fw};
   float u0 = 0.f, u1 = 0.f, u2 = 0.f;
   double T = 0.0, q = 0.0, T0 = 0.0, q0 = 0.0;
-  if (TQ) {
-    T0 = node_at(V.T, col00 + ku0);
-    q0 = node_at(V.q, col00 + ku0);
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const unsigned col = col00 + V.sj * j + V.si * i;
-      const float wxy = wtx[i] * wty[j];
-      const float a0 = wxy * wtz[0], a1 = wxy * wtz[1];
-      u0 += node_at(V.u32, col + ku0) * a0 + node_at(V.u32, col + ku1) * a1;
-      u1 += node_at(V.v32, col + ku0) * a0 + node_at(V.v32, col + ku1) * a1;
-      u2 += node_at(V.w32, col + kw0) * (wxy * wtw[0]) + node_at(V.w32, col + kw1) * (wxy * wtw[1]);
-      if (TQ) {
-        T += (node_at(V.T, col + ku0) - T0) * (double)a0 + (node_at(V.T, col + ku1) - T0) * (double)a1;
-        q += (node_at(V.q, col + ku0) - q0) * (double)a0 + (node_at(V.q, col + ku1) - q0) * (double)a1;
-      }
+  if (PAIR) {
+    const int kp = min(ku0, kmax - 1), kq = min(kw0, kmax - 1);
+    if (ku1 == ku0) {  // clamped: one level, the lower (kp) or the upper (kp+1)
+      const float s = wtz[0] + wtz[1];
+      wtz[0] = (ku0 == kp) ? s : 0.0f;
+      wtz[1] = (ku0 == kp) ? 0.0f : s;
     }
+    if (kw1 == kw0) {
+      const float s = wtw[0] + wtw[1];
+      wtw[0] = (kw0 == kq) ? s : 0.0f;
+      wtw[1] = (kw0 == kq) ? 0.0f : s;
+    }
+    double Tn[2][2][2], qn[2][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const unsigned col = col00 + V.sj * j + V.si * i;
+        const float wxy = wtx[i] * wty[j];
+        const float a0 = wxy * wtz[0], a1 = wxy * wtz[1];
+        float na, nb;
+        node_pair(V.u32, col + kp, na, nb);
+        u0 += na * a0 + nb * a1;
+        node_pair(V.v32, col + kp, na, nb);
+        u1 += na * a0 + nb * a1;
+        node_pair(V.w32, col + kq, na, nb);
+        u2 += na * (wxy * wtw[0]) + nb * (wxy * wtw[1]);
+        if (TQ) {
+          node_pair(V.T, col + kp, Tn[i][j][0], Tn[i][j][1]);
+          node_pair(V.q, col + kp, qn[i][j][0], qn[i][j][1]);
+        }
+      }
+    if (TQ) {
+      T0 = (ku0 == kp) ? Tn[0][0][0] : Tn[0][0][1];
+      q0 = (ku0 == kp) ? qn[0][0][0] : qn[0][0][1];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const float wxy = wtx[i] * wty[j];
+          const float a0 = wxy * wtz[0], a1 = wxy * wtz[1];
+          T += (Tn[i][j][0] - T0) * (double)a0 + (Tn[i][j][1] - T0) * (double)a1;
+          q += (qn[i][j][0] - q0) * (double)a0 + (qn[i][j][1] - q0) * (double)a1;
+        }
+    }
+  } else {
+    if (TQ) {
+      T0 = node_at(V.T, col00 + ku0);
+      q0 = node_at(V.q, col00 + ku0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const unsigned col = col00 + V.sj * j + V.si * i;
+        const float wxy = wtx[i] * wty[j];
+        const float a0 = wxy * wtz[0], a1 = wxy * wtz[1];
+        u0 += node_at(V.u32, col + ku0) * a0 + node_at(V.u32, col + ku1) * a1;
+        u1 += node_at(V.v32, col + ku0) * a0 + node_at(V.v32, col + ku1) * a1;
+        u2 += node_at(V.w32, col + kw0) * (wxy * wtw[0]) + node_at(V.w32, col + kw1) * (wxy * wtw[1]);
+        if (TQ) {
+          T += (node_at(V.T, col + ku0) - T0) * (double)a0 + (node_at(V.T, col + ku1) - T0) * (double)a1;
+          q += (node_at(V.q, col + ku0) - q0) * (double)a0 + (node_at(V.q, col + ku1) - q0) * (double)a1;
+        }
+      }
+  }
   Fluid f;
   f.u[0] = u0;
   f.u[1] = u1;
@@ -335,7 +408,7 @@ template <bool TQ>
 __device__ __forceinline__ Fluid gather_lin_mixed(const DevGrid& g, const DevFields& F,
                                                   const double* s_z, const double* s_zz,
                                                   double x, double y, double z, const Cell& c) {
-  return gather_lin_mixed_v<TQ>(g, hbm_view(F), hbm_col00(g, F, c), s_z, s_zz, x, y, z, c);
+  return gather_lin_mixed_v<TQ, true>(g, hbm_view(F), hbm_col00(g, F, c), s_z, s_zz, x, y, z, c);
 }
 
 // ------------------------------------------------- LDS-staged gather
@@ -360,8 +433,10 @@ __device__ __forceinline__ bool box_has(const FieldBox& b, const DevGrid& g, con
   const int kmax = g.nnz + 1, k1 = b.k0 + b.nk - 1;
   const int ku0 = clampi(c.kpt, 0, kmax), ku1 = clampi(c.kpt + 1, 0, kmax);
   const int kw0 = clampi(c.kwpt, 0, kmax), kw1 = clampi(c.kwpt + 1, 0, kmax);
+  // a droplet whose levels clamp to one is gathered from HBM (the paired
+  // loads there weight it differently from the two-load form)
   return b.ni > 0 && ii >= 0 && ii <= b.ni - 2 && jj >= 0 && jj <= b.nj - 2 && ku0 >= b.k0 && ku1 <= k1 &&
-         kw0 >= b.k0 && kw1 <= k1;
+         kw0 >= b.k0 && kw1 <= k1 && ku1 != ku0 && kw1 != kw0;
 }
 
 // Trilinear gather of cell c from the box when it holds the cell's nodes,

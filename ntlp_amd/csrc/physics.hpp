@@ -350,14 +350,69 @@ __device__ __forceinline__ double rad_solver2(const DevConsts& C, double Tf, dou
   return guess;
 }
 
+// The same guess for the mixed-precision solve.  The cubic's coefficients,
+// its discriminant M = R^2 - Q^3 and therefore the choice of branch are the
+// FP64 ones.  Where the roots lie well apart (|M| > 1e-4 Q^3 with three real
+// roots, M > 1e-4 R^2 with one), acos/cos or cbrt run in FP32 and one FP64
+// Newton step on r^3 + a r^2 + c polishes the root back to FP64 accuracy;
+// near the double root, where Cardano's argument loses its digits, the
+// FP64 expression is used as it stands.
+__device__ __forceinline__ double cubic_polish(double g, double a, double c) {
+  const double f = g * g * (g + a) + c, fp = g * (3.0 * g + 2.0 * a);
+  return fp != 0.0 ? g - f * rcp64(fp) : g;
+}
+
+__device__ __forceinline__ double rad_solver2_mixed(const DevConsts& C, double Tf, double rhoa, double qinf,
+                                                    double kappa, double m_s, double r, double esa, int& mflag) {
+  mflag = 0;
+  const double L = log((C.Ru * Tf * rhoa * qinf) / (C.Mw * esa));
+  const double iL = rcp64(L);
+  const double a = -(C.Mw2Gam_Ru_rhow * rcp64(Tf)) * iL;
+  const double c = kappa * m_s * C.isol * iL;
+  const double Q = (a * a) * (1.0 / 9.0);
+  const double R = (2.0 * (a * a * a) + 27.0 * c) * (1.0 / 54.0);
+  const double Q3 = Q * Q * Q;
+  const double M = R * R - Q3;
+  const double a3 = a * (1.0 / 3.0);
+  double guess;
+  if (M < 0) {
+    const double sQ2 = 2.0 * sqrt(Q);
+    if (-M > 1.0e-4 * Q3) {
+      const float th = acosf((float)(R * rcp64(sqrt(Q3))));
+      const float p2 = (float)C.pi2;
+      guess = -sQ2 * (double)cosf((th - p2) * (1.0f / 3.0f)) - a3;
+      if (guess < 0) guess = -sQ2 * (double)cosf((th + p2) * (1.0f / 3.0f)) - a3;
+      guess = cubic_polish(guess, a, c);
+    } else {
+      const double theta = acos(R / sqrt(Q3));
+      guess = -(sQ2 * cos((theta - C.pi2) / 3.0)) - a3;
+      if (guess < 0) guess = -(sQ2 * cos((theta + C.pi2) / 3.0)) - a3;
+    }
+  } else {
+    double S;
+    if (M > 1.0e-4 * (R * R)) S = (double)cbrtf((float)(fabs(R) + sqrt(M)));
+    else S = cbrt(fabs(R) + sqrt(M));
+    S = (R > 0) ? -S : S;
+    guess = S + Q * rcp64(S) - a3;
+    if (M > 1.0e-4 * (R * R)) guess = cubic_polish(guess, a, c);
+    if (guess < 0) {
+      guess = r;
+      mflag = 1;
+    }
+  }
+  return guess;
+}
+
 // Radius/temperature block of particle_update_BE (3103-3178) on a droplet
 // whose velocity is already the new one.  Updates d.r, d.Tp, activation
 // bookkeeping and the tally.  PREC_FP64: reference residual and FD
 // Jacobian.  Otherwise the closed-form residual with its exact Jacobian.
-// Every precision starts from the reference's FP64 Koehler guess: the BE
-// equations can have more than one solution near activation, and which one
-// Newton (or Levenberg-Marquardt) reaches depends on where it starts; an FP32
-// guess (Cardano's discriminant cancels near the Koehler maximum) sent a few
+// Every precision starts from the reference's Koehler guess to FP64
+// accuracy (the mixed solve through rad_solver2_mixed, whose branch choice
+// and result are the FP64 ones): the BE equations can have more than one
+// solution near activation, and which one Newton (or Levenberg-Marquardt)
+// reaches depends on where it starts; a guess formed wholly in FP32
+// (Cardano's discriminant cancels near the Koehler maximum) sent a few
 // hundred droplets per step of the stiff activation case into failed solves.
 template <int PREC>
 __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFlags& fl, Drop& d,
@@ -370,8 +425,10 @@ __device__ __forceinline__ void be_microphysics(const DevConsts& C, const DevFla
     // reciprocals of the closed-form solve (the reference-order FP64 solve divides)
     const double ir = (PREC == PREC_FP64) ? 0.0 : rcp64(d.r);
     const double iTp = (PREC == PREC_FP64) ? 0.0 : rcp64(d.Tp);
-    {  // the reference's start: the FP64 Koehler guess
-      const double guess = rad_solver2(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
+    {  // the reference's start: the Koehler guess
+      const double guess = (PREC == PREC_MIXED)
+                               ? rad_solver2_mixed(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag)
+                               : rad_solver2(C, Tf, rhoa, qinf, d.kappa, d.m_s, d.r, esa, mflag);
       if (mflag == 0) {
         rt_start[0] = (PREC == PREC_FP64) ? guess / d.r : guess * ir;
         rt_start[1] = (PREC == PREC_FP64) ? Tf / d.Tp : Tf * iTp;
