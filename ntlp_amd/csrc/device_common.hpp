@@ -20,8 +20,10 @@ struct DevGrid {
   int interleaved;  // zz(k) < z(k) < zz(k+1) strictly: one search yields kpt and kwpt
   int nsteps;       // steps of the fixed-length binary search over nnz+2 levels
   int pad_;
+  float rdx32, rdy32;  // 1/dx, 1/dy rounded to FP32 (mixed-precision gather weights)
   double dx, dy, xl, yl;
   const double *z, *zz, *dzw, *dzu;  // device copies of the grid metadata
+  const float *rdzw32, *rdzu32;      // 1/dzw, 1/dzu rounded to FP32
 };
 
 struct DevConsts {
@@ -290,8 +292,8 @@ __device__ __forceinline__ void node_pair(const double* base, unsigned idx, doub
   hi = v[1];
 }
 
-// Mixed-precision trilinear gather: FP32 fractional weights (computed from the
-// FP64 offset of the droplet from its cell origin), u, v, w from FP32 field
+// Mixed-precision trilinear gather: FP32 fractional weights (the FP64 offset
+// of the droplet from its cell origin times the FP32 reciprocal cell size), u, v, w from FP32 field
 // copies accumulated in FP32; T and q from the FP64 fields in the form
 // f000 + sum w_n (f_n - f000), which equals the plain weighted sum because the
 // trilinear weights sum to one, and keeps the FP32 weight rounding off the
@@ -308,14 +310,14 @@ __device__ __forceinline__ Fluid gather_lin_mixed_v(const DevGrid& g, const Node
                                                     double x, double y, double z, const Cell& c) {
   const int nnz = g.nnz, kmax = nnz + 1;
   const int kpt = c.kpt, kwpt = c.kwpt;
-  const float dzu1 = (float)g.dzu[clampi(kpt + 1, 0, nnz + 2)];
-  const float dzw1 = (float)g.dzw[clampi(kwpt + 1, 0, nnz + 2)];
+  const float rdzu1 = g.rdzu32[clampi(kpt + 1, 0, nnz + 2)];
+  const float rdzw1 = g.rdzw32[clampi(kwpt + 1, 0, nnz + 2)];
   const int ku0 = clampi(kpt, 0, kmax), ku1 = clampi(kpt + 1, 0, kmax);
   const int kw0 = clampi(kwpt, 0, kmax), kw1 = clampi(kwpt + 1, 0, kmax);
-  const float fx = (float)(x - g.dx * (double)(c.ipt - 1)) / (float)g.dx;
-  const float fy = (float)(y - g.dy * (double)(c.jpt - 1)) / (float)g.dy;
-  const float fz = (float)(z - s_zz[ku0]) / dzu1;
-  const float fw = (float)(z - s_z[kw0]) / dzw1;
+  const float fx = (float)(x - g.dx * (double)(c.ipt - 1)) * g.rdx32;
+  const float fy = (float)(y - g.dy * (double)(c.jpt - 1)) * g.rdy32;
+  const float fz = (float)(z - s_zz[ku0]) * rdzu1;
+  const float fw = (float)(z - s_z[kw0]) * rdzw1;
   const float wtx[2] = {1.0f - fx, fx}, wty[2] = {1.0f - fy, fy};
   float wtz[2] = {1.0f - fz, fz}, wtw[2] = {1.0f - fw, fw};
   float u0 = 0.f, u1 = 0.f, u2 = 0.f;

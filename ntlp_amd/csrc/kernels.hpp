@@ -430,8 +430,9 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
       for (int st = 1; st <= 3; ++st)
         rk3_droplet<PREC, ILIN>(a.g, C, a.fl, a.F, s_z, s_zz, d, st, a.it, o, o.kpt, &bx);
     } else {
+      // q* is a diagnostic of the record: formed on the step that stores it
       be_droplet<PREC, ILIN, LAZY>(a.g, C, a.fl, a.F, s_z, s_zz, d, a.it, o, tl, kg, SERVE ? nullptr : a.flux,
-                                   mult, &bx);
+                                   mult, &bx, !SERVE && a.write_cold);
     }
     if (SERVE && a.serve->last[home >> kServeIdxBits]) {
       const int64_t row = a.serve->out_off[home >> kServeIdxBits] + (home & kServeIdxMask);

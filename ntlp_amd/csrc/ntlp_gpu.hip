@@ -52,7 +52,7 @@ struct Ctx {
   DevGrid g{};
   DevConsts C{};
   DevFlags fl{};
-  double* d_grid = nullptr;  // z | zz | dzw | dzu
+  double* d_grid = nullptr;  // z | zz | dzw | dzu | FP32 1/dzw, 1/dzu
   DevConsts* d_consts = nullptr;  // device copy of C
   // fields
   double* ext[5] = {};
@@ -812,11 +812,19 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   for (int k = 1; k < nlev; ++k)
     if (!(grid->z[k] > grid->z[k - 1]) || !(grid->zz[k] > grid->zz[k - 1]))
       return fail(NTLP_E_INVALID, "z and zz must be strictly increasing");
-  CHK(dalloc(ctx.d_grid, 2 * nlev + 2 * nd, "grid"));
+  CHK(dalloc(ctx.d_grid, 2 * nlev + 3 * nd, "grid"));
   HIPCHK(hipMemcpy(ctx.d_grid, grid->z, sizeof(double) * nlev, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx.d_grid + nlev, grid->zz, sizeof(double) * nlev, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx.d_grid + 2 * nlev, grid->dzw, sizeof(double) * nd, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx.d_grid + 2 * nlev + nd, grid->dzu, sizeof(double) * nd, hipMemcpyHostToDevice));
+  {
+    std::vector<float> rcp(2 * nd);
+    for (int k = 0; k < nd; ++k) {
+      rcp[k] = (float)(1.0 / grid->dzw[k]);
+      rcp[nd + k] = (float)(1.0 / grid->dzu[k]);
+    }
+    HIPCHK(hipMemcpy(ctx.d_grid + 2 * nlev + 2 * nd, rcp.data(), sizeof(float) * 2 * nd, hipMemcpyHostToDevice));
+  }
   DevGrid& g = ctx.g;
   g.nnx = grid->nnx;
   g.nny = grid->nny;
@@ -829,6 +837,10 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   g.zz = ctx.d_grid + nlev;
   g.dzw = ctx.d_grid + 2 * nlev;
   g.dzu = ctx.d_grid + 2 * nlev + nd;
+  g.rdzw32 = reinterpret_cast<const float*>(ctx.d_grid + 2 * nlev + 2 * nd);
+  g.rdzu32 = g.rdzw32 + nd;
+  g.rdx32 = (float)(1.0 / grid->dx);
+  g.rdy32 = (float)(1.0 / grid->dy);
   int steps = 0;
   while ((1 << steps) <= nlev) ++steps;
   g.nsteps = steps;

@@ -362,10 +362,37 @@ __device__ __forceinline__ double cubic_polish(double g, double a, double c) {
   return fp != 0.0 ? g - f * rcp64(fp) : g;
 }
 
+// ln(A/B) for A, B > 0.  Near A = B (|u| <= 1/4 with u = (A-B)/(A+B), a
+// saturation ratio in [0.6, 1.67]) through ln(A/B) = 2 atanh(u), whose series
+// u (1 + u^2/3 + u^4/5 + ...) is summed to u^26 (the next term is below
+// 1e-17 of the sum); A - B is exact there (Sterbenz), so the result carries a
+// few ulps of L itself rather than the ulp of A/B that log(A/B) carries.
+// Elsewhere log(A/B).
+__device__ __forceinline__ double log_ratio(double A, double B) {
+  const double u = (A - B) * rcp64(A + B);
+  if (fabs(u) > 0.25) return log(A * rcp64(B));
+  const double w = u * u;
+  double p = 1.0 / 27;
+  p = fma(p, w, 1.0 / 25);
+  p = fma(p, w, 1.0 / 23);
+  p = fma(p, w, 1.0 / 21);
+  p = fma(p, w, 1.0 / 19);
+  p = fma(p, w, 1.0 / 17);
+  p = fma(p, w, 1.0 / 15);
+  p = fma(p, w, 1.0 / 13);
+  p = fma(p, w, 1.0 / 11);
+  p = fma(p, w, 1.0 / 9);
+  p = fma(p, w, 1.0 / 7);
+  p = fma(p, w, 1.0 / 5);
+  p = fma(p, w, 1.0 / 3);
+  p = fma(p, w, 1.0);
+  return 2.0 * u * p;
+}
+
 __device__ __forceinline__ double rad_solver2_mixed(const DevConsts& C, double Tf, double rhoa, double qinf,
                                                     double kappa, double m_s, double r, double esa, int& mflag) {
   mflag = 0;
-  const double L = log((C.Ru * Tf * rhoa * qinf) / (C.Mw * esa));
+  const double L = log_ratio(C.Ru * Tf * rhoa * qinf, C.Mw * esa);
   const double iL = rcp64(L);
   const double a = -(C.Mw2Gam_Ru_rhow * rcp64(Tf)) * iL;
   const double c = kappa * m_s * C.isol * iL;
@@ -663,17 +690,21 @@ __device__ __forceinline__ void be_flux(const DevGrid& g, const DevConsts& C, co
 }
 
 // Second half (3103-3238 and the wall reflection of particle_bcs_nonperiodic):
-// radius/temperature solve, q*, residence times, flux statistics (flux not
-// null; levels: the packed levels be_advance left), then walls at the new
-// radius.
+// radius/temperature solve, q* (a diagnostic of the record: formed when
+// with_qstar, i.e. when the caller stores it), residence times, flux
+// statistics (flux not null; levels: the packed levels be_advance left), then
+// walls at the new radius.
 template <int PREC, bool LAZY = false>
 __device__ __forceinline__ void be_finish(const DevGrid& g, const DevConsts& C, const DevFlags& fl,
                                           const double* s_z, Drop& d, const BeHand& h, double& qstar,
-                                          Tally& tl, int levels, double* flux = nullptr, int64_t mult = 1) {
+                                          Tally& tl, int levels, double* flux = nullptr, int64_t mult = 1,
+                                          bool with_qstar = true) {
   const double dt = C.dt;
   const double esa = (PREC == PREC_FP64) ? mod_magnus(h.Tf) : mod_magnus_fast(h.Tf);
   be_microphysics<PREC>(C, fl, d, h.Tf, h.qinf, h.rhoa, h.diffnorm, h.Rep, esa, tl);
-  qstar = (PREC == PREC_MIXED) ? qstar_f32(C, d, h.Tf, h.rhoa, esa) : qstar_f64(C, d, h.Tf, h.rhoa, esa);
+  qstar = 0.0;
+  if (with_qstar)
+    qstar = (PREC == PREC_MIXED) ? qstar_f32(C, d, h.Tf, h.rhoa, esa) : qstar_f64(C, d, h.Tf, h.rhoa, esa);
   if (!LAZY) {  // LAZY: left to the end of the call (k_res_settle)
     d.res = d.res + dt;
     d.actres = d.actres + dt;
@@ -688,9 +719,9 @@ __device__ __forceinline__ void be_droplet(const DevGrid& g, const DevConsts& C,
                                            const DevFields& F, const double* s_z, const double* s_zz,
                                            Drop& d, int it, Diag& o, Tally& tl, int kguess = -1,
                                            double* flux = nullptr, int64_t mult = 1,
-                                           const FieldBox* bx = nullptr) {
+                                           const FieldBox* bx = nullptr, bool with_qstar = true) {
   const BeHand h = be_advance<PREC, ILIN>(g, C, fl, F, s_z, s_zz, d, it, o, kguess, bx);
-  be_finish<PREC, LAZY>(g, C, fl, s_z, d, h, o.qstar, tl, o.kpt, flux, mult);
+  be_finish<PREC, LAZY>(g, C, fl, s_z, d, h, o.qstar, tl, o.kpt, flux, mult, with_qstar);
   o.kpt = packed_kpt(o.kpt);
 }
 
