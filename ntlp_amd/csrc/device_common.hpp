@@ -24,6 +24,8 @@ struct DevGrid {
   double dx, dy, xl, yl;
   const double *z, *zz, *dzw, *dzu;  // device copies of the grid metadata
   const float *rdzw32, *rdzu32;      // 1/dzw, 1/dzu rounded to FP32
+  const double *rdzu64, *rdVu64;     // 1/dzu and 1/(dx*dy*dzu) in FP64
+  double rdx64, rdy64;               // 1/dx, 1/dy in FP64
 };
 
 struct DevConsts {

@@ -812,7 +812,7 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   for (int k = 1; k < nlev; ++k)
     if (!(grid->z[k] > grid->z[k - 1]) || !(grid->zz[k] > grid->zz[k - 1]))
       return fail(NTLP_E_INVALID, "z and zz must be strictly increasing");
-  CHK(dalloc(ctx.d_grid, 2 * nlev + 3 * nd, "grid"));
+  CHK(dalloc(ctx.d_grid, 2 * nlev + 5 * nd, "grid"));
   HIPCHK(hipMemcpy(ctx.d_grid, grid->z, sizeof(double) * nlev, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx.d_grid + nlev, grid->zz, sizeof(double) * nlev, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx.d_grid + 2 * nlev, grid->dzw, sizeof(double) * nd, hipMemcpyHostToDevice));
@@ -824,6 +824,12 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
       rcp[nd + k] = (float)(1.0 / grid->dzu[k]);
     }
     HIPCHK(hipMemcpy(ctx.d_grid + 2 * nlev + 2 * nd, rcp.data(), sizeof(float) * 2 * nd, hipMemcpyHostToDevice));
+    std::vector<double> r64(2 * nd);
+    for (int k = 0; k < nd; ++k) {
+      r64[k] = 1.0 / grid->dzu[k];
+      r64[nd + k] = 1.0 / (grid->dx * grid->dy * grid->dzu[k]);
+    }
+    HIPCHK(hipMemcpy(ctx.d_grid + 2 * nlev + 3 * nd, r64.data(), sizeof(double) * 2 * nd, hipMemcpyHostToDevice));
   }
   DevGrid& g = ctx.g;
   g.nnx = grid->nnx;
@@ -839,6 +845,10 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   g.dzu = ctx.d_grid + 2 * nlev + nd;
   g.rdzw32 = reinterpret_cast<const float*>(ctx.d_grid + 2 * nlev + 2 * nd);
   g.rdzu32 = g.rdzw32 + nd;
+  g.rdzu64 = ctx.d_grid + 2 * nlev + 3 * nd;
+  g.rdVu64 = ctx.d_grid + 2 * nlev + 4 * nd;
+  g.rdx64 = 1.0 / grid->dx;
+  g.rdy64 = 1.0 / grid->dy;
   g.rdx32 = (float)(1.0 / grid->dx);
   g.rdy32 = (float)(1.0 / grid->dy);
   int steps = 0;

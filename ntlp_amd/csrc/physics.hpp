@@ -403,9 +403,9 @@ __device__ __forceinline__ double rad_solver2_mixed(const DevConsts& C, double T
   const double a3 = a * (1.0 / 3.0);
   double guess;
   if (M < 0) {
-    const double sQ2 = 2.0 * sqrt(Q);
+    const double sQ = sqrt(Q), sQ2 = 2.0 * sQ;
     if (-M > 1.0e-4 * Q3) {
-      const float th = acosf((float)(R * rcp64(sqrt(Q3))));
+      const float th = acosf((float)(R * rcp64(Q * sQ)));  // R / sqrt(Q^3)
       const float p2 = (float)C.pi2;
       guess = -sQ2 * (double)cosf((th - p2) * (1.0f / 3.0f)) - a3;
       if (guess < 0) guess = -sQ2 * (double)cosf((th + p2) * (1.0f / 3.0f)) - a3;
