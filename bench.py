@@ -219,7 +219,7 @@ def cpu_baseline(wl_id: int, seed: int, n_full: int, target_s: float = 8.0):
     r1, reps1, t1, z1, _ = leg(20000, False)
     rp, repsp, tp, zp, cores = leg(200000, True)
     return {"value": rp, "unit": "droplet-updates/s", "cores": cores, "kind": "port",
-            "single_thread_value": r1,
+            "cpu_model": cpu_model(), "single_thread_value": r1,
             "sample": f"oracle (C, FP64, -O2): BE step + coupling deposit of the same ensemble and fields; "
                       f"{cores} OpenMP threads on 200000 droplets x {repsp} steps ({tp:.1f} s) and 1 thread on "
                       f"20000 droplets x {reps1} steps ({t1:.1f} s); zeroing the sources costs {zp * 1e3:.0f} ms "
