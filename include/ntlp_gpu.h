@@ -18,6 +18,13 @@
  *    which any thread may call while the serving loop is open.
  *  - Droplets keep the order in which they were put: ntlp_gpu_particles_get
  *    returns them in that order, whatever order the device keeps internally.
+ *  - A population of no droplets (put with n = 0, or nothing put yet) is
+ *    valid, like a rank whose particle list is empty: the updates return at
+ *    once, the coupling sources, counters and flux statistics are zero, and
+ *    calc_tauc_min and the minima/maxima of particle_xy_stats keep the
+ *    reference's start values.
+ *  - ntlp_gpu_init checks all its arguments before it touches the device, so
+ *    a rejected call leaves an existing context as it was.
  */
 #ifndef NTLP_GPU_H
 #define NTLP_GPU_H
@@ -34,7 +41,7 @@ extern "C" {
 #define NTLP_E_NOMEM (-4)       /* device allocation failed                  */
 #define NTLP_E_RANGE (-5)       /* size beyond a documented maximum          */
 #define NTLP_E_UNSUPPORTED (-6) /* flag combination this build does not run  */
-#define NTLP_E_STATE (-7)       /* call out of order (no fields, no droplets) */
+#define NTLP_E_STATE (-7)       /* call out of order (no fields, a failed step) */
 
 /* Column order of the droplet real SoA block: column c of droplet i lives at
  * soa[c*n + i] (a Fortran array real(8) :: soa(n, NTLP_NREAL)).  These are

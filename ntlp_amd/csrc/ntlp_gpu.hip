@@ -238,9 +238,11 @@ int need_fields() {
   if (!ctx.have_fields) return fail(NTLP_E_STATE, "no fields: call ntlp_gpu_set_fields first");
   return 0;
 }
+// Fields and a consistent droplet population, which may be empty: a rank
+// whose particle list is empty runs every update as a loop over no droplets
+// (the entry points return at once, with zero sources and counters).
 int need_drops() {
   CHK(need_fields());
-  if (ctx.n <= 0) return fail(NTLP_E_STATE, "no droplets: call ntlp_gpu_particles_put first");
   if (ctx.state_bad)
     return fail(NTLP_E_STATE, "a failed step call left the droplets inconsistent: put them again");
   return 0;
@@ -717,6 +719,21 @@ int settle(int steps) {
   return launch("res_settle", [&] { k_res_settle<<<grid_flat(n), kBlock, 0, ctx.stream>>>(hot, n, steps, dt); });
 }
 
+// nsteps steps of an empty population: each closes its counter row (zero,
+// one step) and, for BE, its zero flux statistics; the coupling sources are
+// zero.
+int empty_steps(int mode, int nsteps, bool scatter) {
+  for (int k = 0; k < nsteps; ++k) CHK(close_step(mode == NTLP_MODE_BE));
+  if (scatter) {
+    CHK(fill_f64(ctx.src, 6 * ctx.nsrc, 0.0));
+    ctx.src_valid = true;
+    ctx.xsrc_valid = false;
+  }
+  ctx.amounts_valid = true;
+  HIPCHK(hipStreamSynchronize(ctx.stream));
+  return 0;
+}
+
 int run_fused_steps(int mode, int prec, int nsteps, int it0, bool scatter, bool lazy, int& done) {
   const unsigned mask = mode == NTLP_MODE_RK3 ? kDiagRK3 : kDiagBE;
   if (ctx.diag_mask & ~mask) CHK(flush_diag());
@@ -803,15 +820,17 @@ int ntlp_gpu_init(const ntlp_grid* grid, const ntlp_consts* c, const ntlp_flags*
   // the trilinear gathers address the ext arrays with 32-bit byte offsets
   if ((int64_t)(grid->nnz + 2) * (grid->nny + 5) * (grid->nnx + 5) * 8 > (int64_t)UINT32_MAX)
     return fail(NTLP_E_RANGE, "ext field arrays beyond 4 GB each (nnx*nny*nnz above about 5.3e8)");
+  // monotone z and zz are required by the minloc semantics
+  for (int k = 1; k < grid->nnz + 2; ++k)
+    if (!(grid->z[k] > grid->z[k - 1]) || !(grid->zz[k] > grid->zz[k - 1]))
+      return fail(NTLP_E_INVALID, "z and zz must be strictly increasing");
+  // every argument is checked before the device is touched: a rejected call
+  // leaves an existing context as it was
   release_all();
   CHK(setup_scratch());
   HIPCHK(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
   const int nz = grid->nnz;
   const int nlev = nz + 2, nd = nz + 3;
-  // monotone z and zz are required by the minloc semantics
-  for (int k = 1; k < nlev; ++k)
-    if (!(grid->z[k] > grid->z[k - 1]) || !(grid->zz[k] > grid->zz[k - 1]))
-      return fail(NTLP_E_INVALID, "z and zz must be strictly increasing");
   CHK(dalloc(ctx.d_grid, 2 * nlev + 5 * nd, "grid"));
   HIPCHK(hipMemcpy(ctx.d_grid, grid->z, sizeof(double) * nlev, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx.d_grid + nlev, grid->zz, sizeof(double) * nlev, hipMemcpyHostToDevice));
@@ -1330,7 +1349,7 @@ int ntlp_gpu_particles_get_column(int32_t column, double* out, int64_t n) {
 
 int ntlp_gpu_crit_radius(void) {
   CHK(need_init());
-  if (ctx.n <= 0) return fail(NTLP_E_STATE, "no droplets");
+  if (ctx.n == 0) return NTLP_OK;
   CHK(upload_radval(ctx.stream));
   StepArgs a = base_args();
   CHK(launch("crit_radius", [&] { k_crit_radius<<<grid_flat(ctx.n), kBlock, 0, ctx.stream>>>(a); }));
@@ -1340,6 +1359,7 @@ int ntlp_gpu_crit_radius(void) {
 
 int ntlp_gpu_interp(int32_t set_vp, int32_t set_tp) {
   CHK(need_drops());
+  if (ctx.n == 0) return NTLP_OK;
   CHK(flush_diag());
   if (set_vp || set_tp) ctx.amounts_valid = false;
   StepArgs a = base_args();
@@ -1361,6 +1381,7 @@ int ntlp_gpu_particle_update_rk3_prec(int32_t istage, int32_t it, int32_t prec) 
   CHK(need_drops());
   CHK(check_prec(prec));
   if (istage < 1 || istage > 3) return fail(NTLP_E_INVALID, "istage must be 1, 2 or 3");
+  if (ctx.n == 0) return NTLP_OK;
   CHK(flush_diag());
   ctx.keys_valid = false;
   ctx.amounts_valid = false;
@@ -1381,6 +1402,7 @@ int ntlp_gpu_particle_update_rk3(int32_t istage, int32_t it) {
 int ntlp_gpu_particle_update_be_prec(int32_t it, int32_t prec) {
   CHK(need_drops());
   CHK(check_prec(prec));
+  if (ctx.n == 0) return empty_steps(NTLP_MODE_BE, 1, false);
   return run_fused(NTLP_MODE_BE, prec, 1, it, false);
 }
 
@@ -1391,6 +1413,15 @@ int ntlp_gpu_particle_coupling_update(double* src_out, int32_t deterministic) {
   if (deterministic < 0 || deterministic > 2)
     return fail(NTLP_E_INVALID, "coupling method " + std::to_string(deterministic) +
                                     ": 0 (box), 1 (fixed order) or 2 (segmented atomics)");
+  if (ctx.n == 0) {  // no droplets: zero sources
+    CHK(fill_f64(ctx.src, 6 * ctx.nsrc, 0.0));
+    ctx.src_valid = true;
+    ctx.xsrc_valid = false;
+    if (src_out)
+      HIPCHK(hipMemcpyAsync(src_out, ctx.src, sizeof(double) * 6 * ctx.nsrc, hipMemcpyDeviceToHost, ctx.stream));
+    HIPCHK(hipStreamSynchronize(ctx.stream));
+    return NTLP_OK;
+  }
   if (!ctx.amounts_valid) {  // amounts from the state and its old values in cold
     CHK(flush_diag());
     StepArgs a = base_args();
@@ -1480,7 +1511,11 @@ int ntlp_gpu_get_sources(double* src_out) {
 
 int ntlp_gpu_bin(int32_t* cellidx, int32_t* counts, int32_t* offsets, int32_t* perm) {
   CHK(need_init());
-  if (ctx.n <= 0) return fail(NTLP_E_STATE, "no droplets");
+  if (ctx.n == 0) {  // every cell empty
+    if (counts) std::fill(counts, counts + ctx.ncell, 0);
+    if (offsets) std::fill(offsets, offsets + ctx.ncell + 1, 0);
+    return NTLP_OK;
+  }
   const int64_t n = ctx.n;
   int32_t* ci_dev = nullptr;
   int32_t* ci_put = nullptr;
@@ -1554,6 +1589,10 @@ int ntlp_gpu_wcell_counts(int32_t* num_array) {
 int ntlp_gpu_calc_tauc_min(double* tauc_min) {
   CHK(need_drops());
   if (!tauc_min) return fail(NTLP_E_INVALID, "null output");
+  if (ctx.n == 0) {  // the start value 1e10 (particles.f90:3327), lowered by no droplet, times the
+    *tauc_min = 1.0e10 * 5.0;  // fudge factor 5 (3363)
+    return NTLP_OK;
+  }
   const int64_t ncell = (int64_t)ctx.g.nnx * ctx.g.nny * ctx.g.nnz;
   double *rad = nullptr, *nc = nullptr;
   int32_t* num = nullptr;
@@ -1643,6 +1682,7 @@ int ntlp_gpu_step(int32_t mode, int32_t prec, int32_t nsteps, int32_t it0) {
   CHK(check_prec(prec));
   if (mode != NTLP_MODE_RK3 && mode != NTLP_MODE_BE) return fail(NTLP_E_INVALID, "unknown mode");
   if (nsteps < 1) return fail(NTLP_E_INVALID, "nsteps must be >= 1");
+  if (ctx.n == 0) return empty_steps(mode, nsteps, src_mask() != 0);
   return run_fused(mode, prec, nsteps, it0, true);
 }
 
