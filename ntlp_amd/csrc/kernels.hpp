@@ -1388,35 +1388,50 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const int32_t* in, i
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = bsum[blockIdx.x] + total;
 }
 
-// Counting-sort scatter: each run of equal keys in a wave takes a block of
-// slots inside its cell with one atomic; counts are decremented back to zero.
-// Scatter of slot indices into their cells' ranges (counts run down to zero).
-// Slots with a negative key are skipped.  n_dev, when set, holds the slot
-// count (graph replays with a changing pool).
+// Counting-sort scatter: slot t goes to offsets[key] + its rank among the
+// cell's droplets, taken from counts[key] (the cell's count, run down to zero
+// by atomicSub).  A run of equal keys in lane order takes one atomic; slots
+// with a negative key are skipped; n_dev, when set, holds the slot count
+// (graph replays with a changing pool).  Each wave
+// handles kBinChunks consecutive 64-droplet chunks: it loads all their keys,
+// then issues all their atomics, then writes perm, so that the atomics' round
+// trips overlap instead of following one another (launched with
+// bin_blocks(n) workgroups).
+constexpr int kBinChunks = 4;  // 0.073 ms at 20M slots; 1 chunk: 0.097, 8: 0.077, 16: 0.089
 __global__ void __launch_bounds__(kBlock) k_bin_scatter(const int32_t* keys, int64_t n, const int32_t* offsets,
                                                         int32_t* counts, int32_t* perm, const int64_t* n_dev) {
   if (n_dev) n = *n_dev;
-  const int64_t t = xcd_block() * kBlock + threadIdx.x;
-  const bool valid = t < n;
   const int lane = lane_id();
-  const int key = valid ? keys[t] : -1;
-  const bool live = key >= 0;
-  const int prev = __shfl_up(key, 1, 64);
-  const bool head = live && (lane == 0 || prev != key);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long brk = __ballot(head || !live);
-  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-  const int seg_start = max(0, 63 - __clzll((long long)(heads & upto)));
-  int base = 0;
-  if (head) {
-    const unsigned long long after = (lane == 63) ? 0ull : (brk >> (lane + 1)) << (lane + 1);
-    const int end = after ? (__ffsll((long long)after) - 1) : 64;
-    const int len = end - lane;
-    const int old = atomicSub(&counts[key], len);
-    base = offsets[key] + old - len;
+  const int64_t w0 = (xcd_block() * (kBlock / 64) + (threadIdx.x >> 6)) * (64 * kBinChunks);
+  int key[kBinChunks];
+#pragma unroll
+  for (int r = 0; r < kBinChunks; ++r) {
+    const int64_t t = w0 + r * 64 + lane;
+    key[r] = t < n ? keys[t] : -1;
   }
-  base = __shfl(base, seg_start, 64);
-  if (live) perm[base + (lane - seg_start)] = (int32_t)t;
+  int seg[kBinChunks], base[kBinChunks];
+#pragma unroll
+  for (int r = 0; r < kBinChunks; ++r) {
+    const bool live = key[r] >= 0;
+    const int prev = __shfl_up(key[r], 1, 64);
+    const bool head = live && (lane == 0 || prev != key[r]);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long brk = __ballot(head || !live);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    seg[r] = max(0, 63 - __clzll((long long)(heads & upto)));
+    base[r] = 0;
+    if (head) {
+      const unsigned long long after = (lane == 63) ? 0ull : (brk >> (lane + 1)) << (lane + 1);
+      const int end = after ? (__ffsll((long long)after) - 1) : 64;
+      const int len = end - lane;
+      base[r] = offsets[key[r]] - len + atomicSub(&counts[key[r]], len);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kBinChunks; ++r) {
+    const int b = __shfl(base[r], seg[r], 64);
+    if (key[r] >= 0) perm[b + (lane - seg[r])] = (int32_t)(w0 + r * 64 + lane);
+  }
 }
 
 // Makes the counting sort stable and deterministic: orders every cell's

@@ -189,6 +189,12 @@ unsigned grid8(int64_t n) {
   b = (b + 7) / 8 * 8;
   return (unsigned)std::max<int64_t>(b, 8);
 }
+// workgroups of k_bin_scatter over n slots (kBinChunks wave chunks each)
+unsigned bin_blocks(int64_t n) {
+  int64_t b = (n + kBlock * kBinChunks - 1) / (kBlock * kBinChunks);
+  b = (b + 7) / 8 * 8;
+  return (unsigned)std::max<int64_t>(b, 8);
+}
 unsigned grid_flat(int64_t n) {
   int64_t b = (n + kBlock - 1) / kBlock;
   return (unsigned)std::min<int64_t>(std::max<int64_t>(b, 1), 1 << 20);
@@ -387,7 +393,7 @@ int scan_counts() {
 int finish_bin(const int32_t* tie, bool canonical = true) {
   CHK(scan_counts());
   CHK(launch("bin_scatter", [&] {
-    k_bin_scatter<<<grid8(ctx.n), kBlock, 0, ctx.stream>>>(ctx.keys[ctx.cur], ctx.n, ctx.offsets, ctx.counts, ctx.perm,
+    k_bin_scatter<<<bin_blocks(ctx.n), kBlock, 0, ctx.stream>>>(ctx.keys[ctx.cur], ctx.n, ctx.offsets, ctx.counts, ctx.perm,
                                                               nullptr);
   }));
   if (canonical)

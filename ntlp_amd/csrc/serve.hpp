@@ -275,7 +275,7 @@ void serve_enqueue_step(int64_t size, int cur) {
   k_scan_reduce<<<(unsigned)nb, kScanThreads, 0, srv.stream>>>(srv.counts, ctx.ncell, srv.bsum);
   k_scan_bsum<<<1, kScanThreads, 0, srv.stream>>>(srv.bsum, nb);
   k_scan_down<<<(unsigned)nb, kScanThreads, 0, srv.stream>>>(srv.counts, ctx.ncell, srv.bsum, srv.offsets);
-  k_bin_scatter<<<grid8(size), kBlock, 0, srv.stream>>>(srv.keys[cur], size, srv.offsets, srv.counts, srv.perm,
+  k_bin_scatter<<<bin_blocks(size), kBlock, 0, srv.stream>>>(srv.keys[cur], size, srv.offsets, srv.counts, srv.perm,
                                                         &srv.ctl_dev->hw);
   const StepArgs a = serve_step_args(cur);
   if (srv.prec == PREC_FP64) serve_launch_step<PREC_FP64>(a, grid8(size));
