@@ -702,9 +702,14 @@ __global__ void __launch_bounds__(kBlock) k_scatter(ScatterArgs a) {
   const bool above = act && lane + 1 < nseg && nkey == skey + 1 && (nkey % nz1) != 0;
   int64_t node[8];
   key_nodes(a.g, act ? skey : 0, node);
+  double am6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // the six loads issued together
+  if (valid) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) am6[f] = (double)amounts[aix(f, t)];
+  }
   for (int f = 0; f < 6; ++f) {
     if (!((a.fmask >> f) & 1u)) continue;
-    const double am = valid ? (double)amounts[aix(f, t)] : 0.0;
+    const double am = am6[f];
     double v[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) v[c] = am * dp.w[c];
@@ -785,9 +790,12 @@ __global__ void __launch_bounds__(kBlock) k_scatter_pull(PullArgs a) {
           const int64_t s = a.perm[p];
           const double w = corner_weight(a.g, s_zz, cx[xa] + 1, cy[ya] + 1, k0, ci[xa], cj[ya], ck,
                                          a.hot[hix(kHX, s)], a.hot[hix(kHY, s)], a.hot[hix(kHZ, s)]);
+          double am[6];  // loads issued together, masked after they land
+#pragma unroll
+          for (int f = 0; f < 6; ++f) am[f] = (double)amounts[aix(f, s)];
 #pragma unroll
           for (int f = 0; f < 6; ++f)
-            if ((a.fmask >> f) & 1u) acc[f] += (double)amounts[aix(f, s)] * w;
+            if ((a.fmask >> f) & 1u) acc[f] += am[f] * w;
         }
       }
 #pragma unroll
@@ -870,6 +878,9 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
   __shared__ double s_acc[6 * kNodes];
   __shared__ int s_beg[kMaxSeg], s_pre[kMaxSeg + 1];
   __shared__ int s_wave[kBlock / 64];
+  // the box's cell levels klo..khi: zz(k) (also k = khi + 1), 1/dzu(k+1) and
+  // 1/dV(k+1), so that a droplet's weights wait on no further global load
+  __shared__ double s_lev[3][TK + 2];
   const int64_t b = xcd_block();
   if (b >= (int64_t)a.ntx * a.nty * a.ntz) return;  // whole block
   // boxes with k fastest: consecutive boxes of one XCD share their droplets'
@@ -892,6 +903,14 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
   const int nseg = ncx * ncy;
   const int klo = max(0, K0 - 1), khi = min(nnz, K1);
   const int nz1 = nnz + 1, ny1 = nny + 1;
+  if (threadIdx.x < khi - klo + 2) {
+    const int k = klo + (int)threadIdx.x;
+    s_lev[0][threadIdx.x] = a.g.zz[k];
+    if (k <= khi) {
+      s_lev[1][threadIdx.x] = a.g.rdzu64[k + 1];  // 1.0 / dzu(k+1), formed on the host
+      s_lev[2][threadIdx.x] = a.g.rdVu64[k + 1];  // 1.0 / (dx * dy * dzu(k+1)), likewise
+    }
+  }
   int len = 0, beg = 0;
   if (threadIdx.x < nseg) {
     const int ci = threadIdx.x / ncy, cj = threadIdx.x % ncy;
@@ -940,9 +959,17 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
       x[u] = a.hot[hix(kHX, t)];
       y[u] = a.hot[hix(kHY, t)];
       z[u] = a.hot[hix(kHZ, t)];
+      // the six amount loads are issued together with x, y and z (every
+      // step writes all six); a load under the field-mask test would be
+      // compiled as a branch that waits for it before the next one is issued
 #pragma unroll
-      for (int f = 0; f < 6; ++f) am[u][f] = ((a.fmask >> f) & 1u) ? (double)amounts[aix(f, t)] : 0.0;
+      for (int f = 0; f < 6; ++f) am[u][f] = (double)amounts[aix(f, t)];
     }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int f = 0; f < 6; ++f)
+        if (!((a.fmask >> f) & 1u)) am[u][f] = 0.0;
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       if (slot[u] < 0) continue;
@@ -951,15 +978,16 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
       const int i0 = key[u] / (nz1 * ny1);
       // corner weights of particles.f90:1197-1205 (divisions by dx, dy,
       // dzu(kpt+1) and dV as products with their reciprocals)
-      const double rdz = a.g.rdzu64[kpt + 1];  // 1.0 / dzu1, formed on the host
-      const double rdV = a.g.rdVu64[kpt + 1];  // 1.0 / (dx * dy * dzu1), likewise
+      const int kl = kpt - klo;  // the droplets come from levels klo..khi
+      const double rdz = s_lev[1][kl];
+      const double rdV = s_lev[2][kl];
       double wtx[2], wty[2], wtz[2];
       int lx[2], ly[2], lz[2];
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         wtx[c] = 1.0 - fabs(x[u] - dx * (double)(c + i0)) * rdx;
         wty[c] = 1.0 - fabs(y[u] - dy * (double)(c + j0)) * rdy;
-        wtz[c] = 1.0 - fabs(z[u] - a.g.zz[kpt + c]) * rdz;  // two levels, cached
+        wtz[c] = 1.0 - fabs(z[u] - s_lev[0][kl + c]) * rdz;
         int xi = i0 + 1 + c, yj = j0 + 1 + c;
         if (xi > nnx + 1) xi -= nnx;
         if (yj > nny + 1) yj -= nny;
