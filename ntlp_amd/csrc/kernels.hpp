@@ -931,23 +931,28 @@ __global__ void __launch_bounds__(kBlock) k_scatter_box(TileArgs a) {
   const double dx = a.g.dx, dy = a.g.dy;
   const double rdx = a.g.rdx64, rdy = a.g.rdy64;  // 1.0 / dx, 1.0 / dy
   const int rot = lane_id() & 7;
+  int step0 = 1;  // the largest power of two with 2 * step0 >= nseg
+  while (2 * step0 < nseg) step0 *= 2;
   for (int p0 = threadIdx.x; p0 < total; p0 += kU * kBlock) {
+    // the run of p: the last run whose prefix is <= p, so that
+    // s_pre[lo] <= p < s_pre[lo + 1] (runs of length 0 repeat a prefix value
+    // and are never chosen); a fixed number of steps, the droplets' searches
+    // interleaved
+    int lo[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) lo[u] = 0;
+    for (int st = step0; st > 0; st >>= 1) {
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int p = p0 + u * kBlock;
+        if (lo[u] + st < nseg && s_pre[lo[u] + st] <= p) lo[u] += st;
+      }
+    }
     int32_t slot[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const int p = p0 + u * kBlock;
-      slot[u] = -1;
-      if (p < total) {
-        // the run of p: s_pre[lo] <= p < s_pre[lo + 1] (runs of length 0
-        // repeat a prefix value and are never chosen)
-        int lo = 0, hi = nseg;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (s_pre[mid] <= p) lo = mid;
-          else hi = mid;
-        }
-        slot[u] = a.perm[s_beg[lo] + (p - s_pre[lo])];
-      }
+      slot[u] = p < total ? a.perm[s_beg[lo[u]] + (p - s_pre[lo[u]])] : -1;
     }
     int key[kU];
     double x[kU], y[kU], z[kU], am[kU][6];
