@@ -160,7 +160,19 @@ __device__ __forceinline__ void flush_tally(const Tally& tl, unsigned long long*
     for (int k = 0; k < 7; ++k)
       if (v[k]) atomicAdd(&s_red[k], v[k]);
   }
-  if (tl.hist >= 0) atomicAdd(&s_red[kTHist0 + (tl.hist < 63 ? tl.hist : 63)], 1);
+  // iteration histogram: one LDS add per distinct value in the wave (a wave
+  // holds two to four), not one per lane onto the same few words
+  {
+    const int hv = tl.hist < 63 ? tl.hist : 63;
+    unsigned long long rem = __ballot(tl.hist >= 0);
+    while (rem) {
+      const int lead = __ffsll((long long)rem) - 1;
+      const int h = __shfl(hv, lead, 64);
+      const unsigned long long same = __ballot(tl.hist >= 0 && hv == h);
+      if (lane_id() == lead) atomicAdd(&s_red[kTHist0 + h], __popcll(same));
+      rem &= ~same;
+    }
+  }
   __threadfence_block();  // this wave's adds precede its arrival
   int last = 0;
   if (lane_id() == 0) last = atomicAdd(&s_red[kTallyWidth], 1) == (int)(blockDim.x >> 6) - 1;
@@ -442,7 +454,8 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     } else {
       if (!SERVE) {
         double am[6];
-        coupling_amounts<PREC == PREC_MIXED>(C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
+        if (PREC == PREC_MIXED) coupling_amounts_f32(C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
+        else coupling_amounts(C, a.fl, d, o.Tf, o.vold, o.Tpold, o.rold, mult, am);
         if (PREC == PREC_MIXED) {  // stored in FP32, summed in FP64 by the scatter
           float* af = reinterpret_cast<float*>(a.amounts);
 #pragma unroll
@@ -549,7 +562,7 @@ __global__ void __launch_bounds__(kBlock) k_amounts(StepArgs a) {
   const int32_t h = a.src.home[t];
   const double vold[3] = {a.cold[kCVXOLD * a.cap + h], a.cold[kCVYOLD * a.cap + h], a.cold[kCVZOLD * a.cap + h]};
   double am[6];
-  coupling_amounts<false>(a.C, a.fl, d, a.cold[kCTF * a.cap + h], vold, a.cold[kCTPOLD * a.cap + h],
+  coupling_amounts(a.C, a.fl, d, a.cold[kCTF * a.cap + h], vold, a.cold[kCTPOLD * a.cap + h],
                           a.cold[kCRADIUSOLD * a.cap + h], a.src.mult[t], am);
 #pragma unroll
   for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];

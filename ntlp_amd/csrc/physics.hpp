@@ -846,44 +846,64 @@ __device__ __forceinline__ void rk3_droplet(const DevGrid& g, const DevConsts& C
 
 // -------------------------------------------------------- coupling amounts
 
-// Per-droplet source amounts of particle_coupling_update (1187-1264): the
-// node contribution is amount * wtt / dV.  a[0..2] momentum (icouple),
-// a[3] heat (iTcouple), a[4] vapour and a[5] enthalpy (iHcouple).  FAST
-// (mixed precision) replaces the divisions by refined reciprocals; the sums
-// stay FP64.
-template <bool FAST>
+// The amounts below for a mixed-precision step, which stores them in FP32:
+// the changes over the step (v - v_old, r - r_old, Tp - Tp_old) and Tf - Tp
+// are formed in FP64, everything after them in FP32 with hardware
+// reciprocals.  The enthalpy amount is vap * Cpv/Cpa * (Tf - Tp) * mult, the
+// reference's -A*Tp + A*Tf without the cancellation of its two terms.
+__device__ __forceinline__ void coupling_amounts_f32(const DevConsts& C, const DevFlags& fl, const Drop& d,
+                                                     double Tf, const double vold[3], double Tpold,
+                                                     double rold, int64_t mult, double a[6]) {
+  const float idt = (float)C.idt;
+  const float r = (float)d.r;
+  const float radrhs = (float)(d.r - rold) * idt;
+  const float V = (float)C.pi2_23 * (r * r * r);
+  const float rhop = (float)d.m_s * __builtin_amdgcn_rcpf(V) + (float)C.rhow;
+  const double rhoa = (fl.iexner == 1) ? func_p_base(C, d.z) / C.Rd / Tf : C.surf_rho;
+  const float irhoa = (fl.iexner == 1) ? (float)rcp64(rhoa) : (float)C.isurf_rho;
+  const float rm = (float)mult;
+  const float vap = (float)(C.rhow * C.pi2 * 2) * irhoa * (r * r) * radrhs;
+  for (int k = 0; k < 6; ++k) a[k] = 0.0;
+  if (fl.icouple == 1) {
+    const float pm = rhop * V * irhoa;
+    a[0] = (double)(-(pm * ((float)(d.vx - vold[0]) * idt - (float)C.g[0]) + vap * (float)d.vx) * rm);
+    a[1] = (double)(-(pm * ((float)(d.vy - vold[1]) * idt - (float)C.g[1]) + vap * (float)d.vy) * rm);
+    a[2] = (double)(-(pm * ((float)(d.vz - vold[2]) * idt - (float)C.g[2]) + vap * (float)d.vz) * rm);
+  }
+  if (fl.iTcouple == 1) {
+    const float ir = __builtin_amdgcn_rcpf(r);
+    const float Tprhs_s = (float)(d.Tp - Tpold) * idt - (float)C.Lv3_Cpp * ir * radrhs;
+    const float taup_i = 4.5f * (float)(rhoa * C.nuf) * __builtin_amdgcn_rcpf(rhop) * ir * ir;
+    const float itp = __builtin_amdgcn_rcpf(rhop * (float)C.CpaCpp * taup_i);
+    a[3] = (double)(-(Tprhs_s * 6.0f * (float)C.rhow * itp * (float)(C.pi2 / 2.0) * r * (float)C.nuf) * rm);
+  }
+  if (fl.iHcouple == 1) {
+    a[4] = (double)(-vap * rm);
+    a[5] = (double)(vap * (float)C.Cpv_Cpa * (float)(Tf - d.Tp) * rm);
+  }
+}
+
+// Per-droplet source amounts of particle_coupling_update (1187-1264), in the
+// reference's operations: the node contribution is amount * wtt / dV.
+// a[0..2] momentum (icouple), a[3] heat (iTcouple), a[4] vapour and a[5]
+// enthalpy (iHcouple).
 __device__ __forceinline__ void coupling_amounts(const DevConsts& C, const DevFlags& fl, const Drop& d,
                                                  double Tf, const double vold[3], double Tpold,
                                                  double rold, int64_t mult, double a[6]) {
   const double dt = C.dt;
-  const double idt = FAST ? C.idt : 0.0;
   const double r = d.r;
-  double vrhs0, vrhs1, vrhs2, radrhs, Tprhs_L, Tprhs_s, rhoa, rhop, taup_i;
   const double V = C.pi2_23 * (r * r * r);  // C.pi2 * 2.0 / 3.0 * (r * r * r)
-  if (FAST) {
-    vrhs0 = (d.vx - vold[0]) * idt;
-    vrhs1 = (d.vy - vold[1]) * idt;
-    vrhs2 = (d.vz - vold[2]) * idt;
-    radrhs = (r - rold) * idt;
-    const double ir = rcp64(r);
-    Tprhs_L = C.Lv3_Cpp * ir * radrhs;
-    Tprhs_s = (d.Tp - Tpold) * idt - Tprhs_L;
-    rhoa = (fl.iexner == 1) ? func_p_base(C, d.z) / C.Rd / Tf : C.surf_rho;
-    rhop = d.m_s * rcp64(V) + C.rhow;
-    taup_i = 4.5 * rhoa * C.nuf * rcp64(rhop) * ir * ir;
-  } else {
-    vrhs0 = (d.vx - vold[0]) / dt;
-    vrhs1 = (d.vy - vold[1]) / dt;
-    vrhs2 = (d.vz - vold[2]) / dt;
-    radrhs = (r - rold) / dt;
-    Tprhs_L = 3.0 * C.Lv / C.Cpp / r * radrhs;
-    Tprhs_s = (d.Tp - Tpold) / dt - Tprhs_L;
-    rhoa = (fl.iexner == 1) ? func_p_base(C, d.z) / C.Rd / Tf : C.surf_rho;
-    rhop = (d.m_s + V * C.rhow) / V;
-    taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
-  }
-  const double irhoa = FAST ? ((fl.iexner == 1) ? rcp64(rhoa) : C.isurf_rho) : 1.0 / rhoa;
-  const double partmass = FAST ? rhop * V : rhop * 2.0 / 3.0 * C.pi2 * (r * r * r);
+  const double vrhs0 = (d.vx - vold[0]) / dt;
+  const double vrhs1 = (d.vy - vold[1]) / dt;
+  const double vrhs2 = (d.vz - vold[2]) / dt;
+  const double radrhs = (r - rold) / dt;
+  const double Tprhs_L = 3.0 * C.Lv / C.Cpp / r * radrhs;
+  const double Tprhs_s = (d.Tp - Tpold) / dt - Tprhs_L;
+  const double rhoa = (fl.iexner == 1) ? func_p_base(C, d.z) / C.Rd / Tf : C.surf_rho;
+  const double rhop = (d.m_s + V * C.rhow) / V;
+  const double taup_i = 18.0 * rhoa * C.nuf / rhop / ((2.0 * r) * (2.0 * r));
+  const double irhoa = 1.0 / rhoa;
+  const double partmass = rhop * 2.0 / 3.0 * C.pi2 * (r * r * r);
   const double rm = (double)mult;
   const double vap = C.rhow * irhoa * C.pi2 * 2 * (r * r) * radrhs;
   for (int k = 0; k < 6; ++k) a[k] = 0.0;
@@ -893,13 +913,12 @@ __device__ __forceinline__ void coupling_amounts(const DevConsts& C, const DevFl
     a[2] = -(partmass * irhoa * (vrhs2 - C.g[2]) + vap * d.vz) * rm;
   }
   if (fl.iTcouple == 1) {
-    const double itp = FAST ? rcp64(rhop * C.CpaCpp * taup_i) : 1.0 / (rhop * C.CpaCpp * taup_i);
+    const double itp = 1.0 / (rhop * C.CpaCpp * taup_i);
     a[3] = -(Tprhs_s * 6.0 * C.rhow * itp * (C.pi2 / 2.0) * r * C.nuf) * rm;
   }
   if (fl.iHcouple == 1) {
     a[4] = -vap * rm;
-    a[5] = FAST ? (-vap * C.Cpv_Cpa * d.Tp + vap * C.Cpv_Cpa * Tf) * rm
-                : (-vap * C.Cpv / C.Cpa * d.Tp + vap * C.Cpv / C.Cpa * Tf) * rm;
+    a[5] = (-vap * C.Cpv / C.Cpa * d.Tp + vap * C.Cpv / C.Cpa * Tf) * rm;
   }
 }
 
