@@ -79,6 +79,34 @@ def kernel_bytes(name: str, n: int, grid, precision: int) -> float | None:
     return None
 
 
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 vector
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 vector
+
+
+def pmc_compute(kernel: str, workload: str, avg_ms: float, precision_key: str = "step_be"):
+    """The compute side of a step kernel's roofline: FP64 and FP32 FLOPs per
+    launch from the committed rocprofv3 instruction-mix summary (the
+    SQ_INSTS_VALU_FLOPS_* counters, wave instructions x 64 lanes) over the
+    kernel's average launch time measured in this run, when that summary covers
+    this workload; with the VALU instruction mix that bounds it."""
+    path = os.path.join(ROOT, "profiles", "step_scatter_20M_256x256x128_valu_mix.json")
+    if not os.path.exists(path) or not avg_ms:
+        return None
+    with open(path) as fh:
+        pm = json.load(fh)
+    k = pm.get("kernels", {}).get(precision_key)
+    if pm.get("workload") != workload or k is None:
+        return None
+    t = avg_ms * 1e-3
+    f64, f32 = k["fp64_flops"] / t / 1e12, k["fp32_flops"] / t / 1e12
+    return {"unit": "TFLOP/s", "fp64_achieved": f64, "fp64_peak": FP64_PEAK_TFLOPS, "fp64_frac": f64 / FP64_PEAK_TFLOPS,
+            "fp32_achieved": f32, "fp32_peak": FP32_PEAK_TFLOPS, "fp32_frac": f32 / FP32_PEAK_TFLOPS,
+            "valu_insts_per_launch": k["valu_insts"],
+            "valu_mix": {"fp64": k["fp64_insts"] / k["valu_insts"], "fp32": k["fp32_insts"] / k["valu_insts"],
+                         "int": k["int_insts"] / k["valu_insts"], "cvt": k["cvt_insts"] / k["valu_insts"],
+                         "other": k["other_insts"] / k["valu_insts"]}}
+
+
 def cpu_model() -> str:
     try:
         out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, timeout=10).stdout
@@ -513,6 +541,9 @@ def main():
             "lm_fallbacks": cnt_fp64["num100"], "failed_solves": cnt_fp64["numimpos"],
             "kernel_ms": {k: v[1] / v[0] for k, v in ktimes64.items()},
         }
+        if "step_be" in ktimes64:
+            fp64["compute"] = pmc_compute("step_be", wl.name, ktimes64["step_be"][1] / ktimes64["step_be"][0],
+                                          "step_be_fp64")
         accuracy = {"radius_max_rel_err_mixed_vs_fp64": float(rr.max()),
                     "radius_p99_rel_err_mixed_vs_fp64": float(np.quantile(rr, 0.99)),
                     "radius_max_rel_err_mixed_vs_fp64_fd_solve_converged": float(rr[~fd_fail].max()),
@@ -553,6 +584,9 @@ def main():
         ach = traffic_alg / (avg_ms * 1e-3) / 1e9
         roofline.update({"achieved": ach, "frac": ach / HBM_PEAK_GBS})
     roofline["traffic"] = pmc_traffic(dom_name, wl.name)
+    # the step kernel is also FP64-vector work: its FLOP rate against the
+    # vector peaks, from the committed instruction-mix counters
+    roofline["compute"] = pmc_compute(dom_name, wl.name, avg_ms) if dom_name == "step_be" else None
     if not args.no_copy:
         copy_gbps = s.copy_bandwidth(2.0, 10)
         log(f"[bench] device copy {copy_gbps:.0f} GB/s")
