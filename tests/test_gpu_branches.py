@@ -396,3 +396,29 @@ def test_staged_gather_gives_the_same_bits(prec):
     for f in range(6):
         scale = np.abs(res[0][1][f]).max() + 1e-300
         assert np.abs(res[0][1][f] - res[1][1][f]).max() <= 1e-12 * scale, f
+
+
+@pytest.mark.gpu
+def test_mixed_step_fp32_amounts_match_fp64_amounts_of_the_same_state():
+    """The coupling amounts a mixed-precision step forms in FP32 (after FP64
+    differences over the step) against the reference-order FP64 amounts of the
+    same droplet states: the sources of three mixed BE steps, then the same
+    record put back and deposited again, which forms the amounts in FP64 from
+    the record's old values (particles.f90:1187-1264).  Only the amount
+    arithmetic and the summation order differ, so the sources agree to FP32
+    rounding of each droplet's amount: within 1e-5 of each field's largest
+    value.  The smallest term of the momentum amount, the vapour's momentum
+    vap*v, is 2e-4 of the drag term on average over these droplets (up to
+    3e-2), so dropping it would fail the check."""
+    grid, consts, flags, fields, p, ora = small_case(seed=12, n=6000, kind="stretched")
+    s = make_step(grid, consts, flags, fields, p)
+    s.step(MODE_BE, PREC_MIXED, nsteps=3, it0=2)
+    src32 = s.sources().copy()
+    rec = s.get()
+    s.put(rec)
+    src64 = s.particle_coupling_update(deterministic=0)
+    errs = [float(np.max(np.abs(src32[f] - src64[f])) / (np.max(np.abs(src64[f])) + 1e-300)) for f in range(6)]
+    print("FP32 against FP64 amounts, max error per field / field max:", errs)
+    assert all(np.max(np.abs(src64[f])) > 0 for f in range(6))
+    assert max(errs) <= 1e-5, errs
+    s.close()
