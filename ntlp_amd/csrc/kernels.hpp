@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(kBlock) k_amounts(StepArgs a) {
   const double vold[3] = {a.cold[kCVXOLD * a.cap + h], a.cold[kCVYOLD * a.cap + h], a.cold[kCVZOLD * a.cap + h]};
   double am[6];
   coupling_amounts(a.C, a.fl, d, a.cold[kCTF * a.cap + h], vold, a.cold[kCTPOLD * a.cap + h],
-                          a.cold[kCRADIUSOLD * a.cap + h], a.src.mult[t], am);
+                   a.cold[kCRADIUSOLD * a.cap + h], a.src.mult[t], am);
 #pragma unroll
   for (int k = 0; k < 6; ++k) a.amounts[aix(k, t)] = am[k];
 }
