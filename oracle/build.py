@@ -2,7 +2,7 @@
 
 Compiles oracle/ntlp_oracle.c with gcc into oracle/liboracle.so.  The
 reference itself (Fortran, needing Intel mpiifort, an MPI library and netCDF)
-is not built here; see oracle/README.md.
+is not built here.
 """
 from __future__ import annotations
 

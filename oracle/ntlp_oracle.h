@@ -11,8 +11,9 @@
  * lines it restates.  The reference compiles with -r8, so every `real` there
  * is an IEEE binary64 `double` here.
  *
- * Parity pins (what checks each function against something other than a
- * re-typing of its own formula) are listed in oracle/README.md.
+ * Each function is pinned by a check that does not re-type its formula:
+ * known-answer vectors, closed forms, mpmath roots, conservation budgets or
+ * hand cases, in tests/test_oracle_pins.py and tests/test_oracle_budgets.py.
  */
 #ifndef NTLP_ORACLE_H
 #define NTLP_ORACLE_H
