@@ -40,7 +40,7 @@ extern "C" {
 #define NTLP_E_HIP (-3)         /* a HIP runtime call failed                 */
 #define NTLP_E_NOMEM (-4)       /* device allocation failed                  */
 #define NTLP_E_RANGE (-5)       /* size beyond a documented maximum          */
-#define NTLP_E_UNSUPPORTED (-6) /* flag combination this build does not run  */
+#define NTLP_E_UNSUPPORTED (-6) /* flag combination the library does not run */
 #define NTLP_E_STATE (-7)       /* call out of order (no fields, a failed step) */
 
 /* Column order of the droplet real SoA block: column c of droplet i lives at
