@@ -255,15 +255,16 @@ def cpu_baseline(wl_id: int, seed: int, n_full: int, target_s: float = 8.0):
 
 
 def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budget_bytes: int = 32 << 30,
-                nmax: int = 4_000_000):
+                nmax: int = 4_000_000, mode: int | None = None, dt: float | None = None):
     """The serving stream against resident 128^3 fields: Poisson arrivals at
     `rate` req/s for `duration_s`, request sizes log-uniform on [1e3, 4e6]
     droplets, each request its own ensemble -- its own seed, radius median
     drawn from 2-20 um (sigma_ln 0.4), aerosol median from 0.02-0.2 um
     (sigma_ln 0.5) -- drawn on the device at admission
-    (ntlp_serve_submit_ensemble) and advancing `steps` mixed-precision BE
-    steps.  Results (14 columns and mult) land in a pinned FIFO ring of 3 x 4M
-    rows.  Phases:
+    (ntlp_serve_submit_ensemble) and advancing `steps` mixed-precision steps
+    of `mode` (BE by default; RK3 with a `dt` at which the explicit stages are
+    stable for the ensembles' smallest droplets).  Results (14 columns and
+    mult) land in a pinned FIFO ring of 3 x 4M rows.  Phases:
 
       stream     Poisson arrivals; p50/p99 latency from submission to result
                  in host memory;
@@ -282,7 +283,10 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
     from ntlp_amd import (COL, MODE_BE, PREC_MIXED, SERVE_OUT, DropletStep, Ensemble, Flags, PinnedArray, ServeLoop,
                           SALT_DRY_LOGNORMAL, synth)
 
+    mode = MODE_BE if mode is None else mode
     wl = synth.workload(4, n=1, seed=seed, droplets=False)
+    if dt is not None:
+        wl.consts.dt = dt
     flags = Flags(icouple=0, iTcouple=0, iHcouple=0)
     s = DropletStep(wl.grid, wl.consts, flags)
     s.set_fields(*wl.fields)
@@ -317,7 +321,7 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
                         salt_kind=SALT_DRY_LOGNORMAL, salt_p=(rng.uniform(0.02e-6, 0.2e-6), 0.5)) for i in range(k)]
         return sz, ens
 
-    loop = ServeLoop(s, budget_bytes, steps=steps, precision=PREC_MIXED)
+    loop = ServeLoop(s, budget_bytes, steps=steps, precision=PREC_MIXED, mode=mode)
 
     def submit(ring, e, n):
         row = ring.take(n)
@@ -391,7 +395,7 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
             rows += int(sizes_s[j])
             j += 1
         s.generate(ens_s[i:j], sizes_s[i:j])
-        s.step(MODE_BE, PREC_MIXED, nsteps=steps, it0=2)
+        s.step(mode, PREC_MIXED, nsteps=steps, it0=2)
         for k, c in enumerate(cols):
             s._check(lib.ntlp_gpu_particles_get_column(c, outbuf.array[k].ctypes.data_as(ctypes.c_void_p), rows))
         s._check(lib.ntlp_gpu_particles_get(None, outmult.array.ctypes.data_as(ctypes.c_void_p), None, None, rows))
@@ -407,9 +411,10 @@ def serve_bench(seed: int, duration_s: float, rate: float, steps: int = 10, budg
     s.close()
     outbuf.free()
     outmult.free()
+    mname = "BE" if mode == MODE_BE else f"RK3 (dt = {wl.consts.dt:g} s)"
     return {"workload": "128^3 resident fields, Poisson arrivals, sizes log-uniform 1e3-4e6, each request its own "
                         f"seed, radius median U(2,20) um and aerosol median U(0.02,0.2) um drawn on the device, "
-                        f"{steps} BE steps each, mixed precision, HBM budget {budget_bytes >> 30} GiB",
+                        f"{steps} {mname} steps each, mixed precision, HBM budget {budget_bytes >> 30} GiB",
             "stream": stream, "saturated": saturated, "graphs": tot["graphs"], "capacity": tot["capacity"],
             "newton": tot["newton"]}
 
@@ -496,10 +501,12 @@ def main():
               "droplets with an all-reduce of the coupling fields; that path is not built).", file=sys.stderr)
         sys.exit(2)
 
-    from ntlp_amd import MODE_BE, PREC_FP64, PREC_FP64_AJ, PREC_MIXED, synth
+    from ntlp_amd import MODE_BE, MODE_RK3, PREC_FP64, PREC_FP64_AJ, PREC_MIXED, synth
 
     if args.serve_only:
-        print(json.dumps(serve_bench(args.seed, args.serve_seconds, args.serve_rate)), flush=True)
+        print(json.dumps({"serve": serve_bench(args.seed, args.serve_seconds, args.serve_rate),
+                          "serve_rk3": serve_bench(args.seed, args.serve_seconds, args.serve_rate, mode=MODE_RK3,
+                                                   dt=1e-6)}), flush=True)
         return
 
     wl = prepare(args.workload, args.droplets, args.seed)
@@ -605,10 +612,15 @@ def main():
         cpu = cpu_baseline(args.workload, args.seed, n)
         log(f"[bench] cpu baseline {cpu['value']:.3g} droplet-updates/s")
 
-    serve = None
+    serve = serve_rk3 = None
     if not args.no_serve:
         log("[bench] serving stream ...")
         serve = serve_bench(args.seed, args.serve_seconds, args.serve_rate)
+        # the same stream with RK3 steps: dt = 1e-6 s keeps the explicit
+        # stages stable down to the smallest droplets of ~2e8 drawn from these
+        # ensembles (about 0.2 um, response time ~6e-7 s)
+        log("[bench] serving stream, RK3 steps ...")
+        serve_rk3 = serve_bench(args.seed, args.serve_seconds, args.serve_rate, mode=MODE_RK3, dt=1e-6)
 
     value = n * args.steps / t_mixed
     per_step_iters = (rows[:, 4] / np.maximum(rows[:, 6], 1)).tolist() if len(rows) else []
@@ -647,6 +659,7 @@ def main():
         "config5": config5,
         "cpu_baseline": cpu,
         "serve": serve,
+        "serve_rk3": serve_rk3,
     }
     print(json.dumps(out), flush=True)
 

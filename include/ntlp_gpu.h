@@ -339,9 +339,10 @@ int ntlp_gpu_sync(void);
  * counterpart in the reference (NTLP advances one droplet population per
  * run).  A request is a droplet ensemble that is created the way
  * create_particle does it (vp = uf, Tp = Tf, rc = crit_radius) and advanced
- * `steps` backward-Euler steps (particle_update_BE, particles.f90:3000-3280,
- * without the coupling scatter) against the fields of the last
- * ntlp_gpu_set_fields.  Requests are admitted FIFO into a device pool whose
+ * `steps` steps of the configured mode -- backward Euler (particle_update_BE,
+ * particles.f90:3000-3280) or the three RK3 stages (particle_update_rk3,
+ * particles.f90:2845-2998), without the coupling scatter -- against the
+ * fields of the last ntlp_gpu_set_fields.  Requests are admitted FIFO into a device pool whose
  * capacity is hbm_budget_bytes / NTLP_SERVE_BYTES_PER_DROPLET; every
  * iteration steps all pooled droplets together (one captured hipGraph per
  * pool size class when use_graph=1) and releases the requests that finish.
@@ -364,7 +365,7 @@ typedef struct ntlp_serve_config {
   int32_t steps;            /* steps per request                                 */
   int32_t precision;        /* NTLP_PREC_*                                       */
   int32_t use_graph;        /* 1: replay captured hipGraphs; 0: launch directly   */
-  int32_t reserved;
+  int32_t mode;             /* NTLP_MODE_BE (or 0) or NTLP_MODE_RK3               */
 } ntlp_serve_config;
 typedef struct ntlp_serve_totals {
   int64_t submitted, completed, queued;

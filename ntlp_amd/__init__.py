@@ -219,7 +219,7 @@ class Ensemble:
 
 class _CServeConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("steps", ctypes.c_int32), ("precision", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("mode", ctypes.c_int32)]
 
 
 class _CServeTotals(ctypes.Structure):
@@ -549,6 +549,7 @@ class ServeLoop:
     """The continuous-batching serving loop (ntlp_serve_*) over the fields
     resident in an open :class:`DropletStep`.
 
+    Every request advances ``steps`` steps of ``mode`` (MODE_BE or MODE_RK3).
     ``submit(inp, mult, out, out_mult)`` queues one droplet ensemble --
     ``inp`` is (6, n) FP64 in ``SERVE_IN`` order, ``out`` (14, n) FP64 receives
     ``SERVE_OUT`` -- and returns a ticket at once; the arrays must stay alive
@@ -556,10 +557,10 @@ class ServeLoop:
     arrays make the copies asynchronous DMA."""
 
     def __init__(self, step: DropletStep, hbm_budget_bytes: int, steps: int = 10, precision: int = PREC_MIXED,
-                 use_graph: bool = True):
+                 use_graph: bool = True, mode: int = MODE_BE):
         self.step = step
         self.lib = step.lib
-        cfg = _CServeConfig(int(hbm_budget_bytes), int(steps), int(precision), int(use_graph), 0)
+        cfg = _CServeConfig(int(hbm_budget_bytes), int(steps), int(precision), int(use_graph), int(mode))
         cap = ctypes.c_int64()
         self._check(self.lib.ntlp_serve_open(ctypes.byref(cfg), ctypes.byref(cap)))
         self.capacity = int(cap.value)

@@ -118,7 +118,7 @@ struct Serve {
   hipStream_t stream = nullptr;  // steps
   hipStream_t up = nullptr;      // uploads
   hipStream_t down = nullptr;    // downloads
-  int steps = 10, prec = PREC_MIXED;
+  int steps = 10, prec = PREC_MIXED, mode = NTLP_MODE_BE;
   bool use_graph = true;
   int64_t cap = 0;
   // device pool
@@ -261,8 +261,13 @@ StepArgs serve_step_args(int cur) {
 template <int PREC>
 void serve_launch_step(const StepArgs& a, unsigned grid) {
   const size_t lds = step_lds_bytes(a.g.nnz, a.box_bytes);
-  if (ctx.fl.ilin == 1) k_step<2, PREC, 1, true><<<grid, kBlock, lds, srv.stream>>>(a);
-  else k_step<2, PREC, 0, true><<<grid, kBlock, lds, srv.stream>>>(a);
+  if (srv.mode == NTLP_MODE_RK3) {
+    if (ctx.fl.ilin == 1) k_step<1, PREC, 1, true><<<grid, kBlock, lds, srv.stream>>>(a);
+    else k_step<1, PREC, 0, true><<<grid, kBlock, lds, srv.stream>>>(a);
+  } else {
+    if (ctx.fl.ilin == 1) k_step<2, PREC, 1, true><<<grid, kBlock, lds, srv.stream>>>(a);
+    else k_step<2, PREC, 0, true><<<grid, kBlock, lds, srv.stream>>>(a);
+  }
 }
 
 // The per-iteration kernels for a pool of at most `size` slots: counting sort
@@ -579,6 +584,8 @@ int ntlp_serve_open(const ntlp_serve_config* cfg, int64_t* capacity) {
   if (!cfg) return fail(NTLP_E_INVALID, "null config");
   CHK(check_prec(cfg->precision));
   if (cfg->steps < 1) return fail(NTLP_E_INVALID, "steps must be >= 1");
+  if (cfg->mode != 0 && cfg->mode != NTLP_MODE_BE && cfg->mode != NTLP_MODE_RK3)
+    return fail(NTLP_E_INVALID, "serving mode must be NTLP_MODE_BE (or 0) or NTLP_MODE_RK3");
   const int64_t cap = cfg->hbm_budget_bytes / kServeBytesPerDroplet;
   if (cap < 4096) return fail(NTLP_E_RANGE, "HBM budget below 4096 droplets");
   if (cap > INT32_MAX) return fail(NTLP_E_RANGE, "HBM budget above 2^31 droplets");
@@ -586,6 +593,7 @@ int ntlp_serve_open(const ntlp_serve_config* cfg, int64_t* capacity) {
   HIPCHK(hipGetDevice(&srv.device));
   srv.steps = cfg->steps;
   srv.prec = cfg->precision;
+  srv.mode = cfg->mode == NTLP_MODE_RK3 ? NTLP_MODE_RK3 : NTLP_MODE_BE;
   srv.use_graph = cfg->use_graph != 0;
   srv.cap = cap;
   int rc = 0;

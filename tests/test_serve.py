@@ -11,7 +11,7 @@ import threading
 import numpy as np
 import pytest
 
-from ntlp_amd import (COL, MODE_BE, PREC_FP64, PREC_MIXED, SERVE_BYTES_PER_DROPLET, SERVE_IN, SERVE_OUT,
+from ntlp_amd import (COL, MODE_BE, MODE_RK3, PREC_FP64, PREC_MIXED, SERVE_BYTES_PER_DROPLET, SERVE_IN, SERVE_OUT,
                       DropletStep, Flags, NtlpError, Particles, ServeLoop, synth)
 
 from .helpers import small_grid
@@ -36,7 +36,7 @@ def make_request(grid, seed: int, n: int, r_med: float, a_med: float):
     return np.ascontiguousarray(inp), mult
 
 
-def plain_result(s: DropletStep, inp, mult, prec):
+def plain_result(s: DropletStep, inp, mult, prec, mode=MODE_BE):
     """The same droplets through the plain entry points."""
     n = inp.shape[1]
     p = Particles.empty(n)
@@ -46,15 +46,15 @@ def plain_result(s: DropletStep, inp, mult, prec):
     s.put(p)
     s.interp(set_vp=True, set_tp=True)
     s.crit_radius()
-    s.step(MODE_BE, prec, nsteps=STEPS, it0=2)
+    s.step(mode, prec, nsteps=STEPS, it0=2)
     q = s.get()
     return np.stack([q[c] for c in SERVE_OUT]), q.mult
 
 
-def serve_case(kind="stretched", seed=3):
+def serve_case(kind="stretched", seed=3, dt=5e-4):
     grid = small_grid(kind, nnx=24, nny=20, nnz=24)
     from ntlp_amd import Consts
-    consts = Consts(dt=5e-4, radius_init=5e-6)
+    consts = Consts(dt=dt, radius_init=5e-6)
     fields = synth.make_fields(grid, consts, np.random.default_rng(seed), T0=300.0, T_sd=0.1, rh_uniform=(99.7, 100.4))
     s = DropletStep(grid, consts, Flags())
     s.set_fields(*fields)
@@ -97,6 +97,35 @@ def test_served_results_equal_plain_steps_alone_and_in_a_mix(prec):
     assert tot["completed"] == len(reqs) + 1 and tot["queued"] == 0
     assert tot["newton"]["solves"] == STEPS * (sizes[0] + sum(sizes))
     assert tot["graphs"] >= 1 and tot["graph_launches"] == tot["iterations"]
+
+
+@pytest.mark.parametrize("prec", [PREC_MIXED, PREC_FP64])
+def test_served_rk3_results_equal_plain_rk3_steps_alone_and_in_a_mix(prec):
+    """Requests that advance RK3 steps (particle_update_rk3's three stages per
+    step, particles.f90:2845-2998): each result equals the plain fused RK3
+    path on the same droplets, bit for bit, alone and in a mix.  dt = 2e-6 s
+    keeps the explicit stages stable for the smallest droplets of these
+    ensembles (dt below the droplet response time down to about 0.3 um)."""
+    grid, s = serve_case(dt=2e-6)
+    sizes = [4000, 37, 1500, 9000, 1]
+    reqs = [make_request(grid, 300 + i, n, r, a)
+            for i, (n, r, a) in enumerate(zip(sizes, (2e-6, 15e-6, 6e-6, 1.2e-6, 9e-6),
+                                              (0.02e-6, 0.2e-6, 0.05e-6, 0.1e-6, 0.03e-6)))]
+    want = [plain_result(s, inp, mult, prec, mode=MODE_RK3) for inp, mult in reqs]
+    with pytest.raises(NtlpError):
+        ServeLoop(s, hbm_budget_bytes=64 << 20, steps=STEPS, precision=prec, mode=7)
+    loop = ServeLoop(s, hbm_budget_bytes=64 << 20, steps=STEPS, precision=prec, mode=MODE_RK3)
+    alone = run_served(loop, reqs[:1], concurrent=False)
+    mixed = run_served(loop, reqs, concurrent=True)
+    tot = loop.close()
+    assert np.array_equal(alone[0][0], want[0][0]) and np.array_equal(alone[0][1], want[0][1])
+    for (out, om), (w, wm), (inp, mult) in zip(mixed, want, reqs):
+        assert np.isfinite(w).all()
+        assert np.array_equal(out, w), np.max(np.abs(out - w))
+        assert np.array_equal(om, wm) and np.array_equal(om, mult)
+    assert tot["completed"] == len(reqs) + 1 and tot["queued"] == 0
+    assert tot["newton"]["solves"] == 0  # the RK3 stages have no implicit solve
+    s.close()
 
 
 def test_direct_launches_match_graphs_and_requests_arriving_late_join():
