@@ -23,6 +23,9 @@ module ntlp_gpu
   integer(c_int32_t), parameter, public :: NTLP_LAYOUT_EXT = 0, NTLP_LAYOUT_CORE = 1, NTLP_LAYOUT_XPENCIL = 2
   integer(c_int32_t), parameter, public :: NTLP_MODE_RK3 = 1, NTLP_MODE_BE = 2
   integer(c_int32_t), parameter, public :: NTLP_PREC_FP64 = 0, NTLP_PREC_MIXED = 1, NTLP_PREC_FP64_AJ = 2
+  integer(c_int32_t), parameter, public :: NTLP_DIST_LOGNORMAL = 0, NTLP_DIST_LOGUNIFORM = 1
+  integer(c_int32_t), parameter, public :: NTLP_SALT_FIXED = 0, NTLP_SALT_DRY_LOGNORMAL = 1, NTLP_SALT_CHOICE3 = 2
+  integer(c_int32_t), parameter, public :: NTLP_OPT_STAGED_GATHER = 1
   integer, parameter, public :: NTLP_NREAL = 34, NTLP_RECORD_BYTES = 352, NTLP_GN_HIST = 64
 
   ! column numbers (1-based) of real(8) :: soa(n, NTLP_NREAL)
@@ -51,6 +54,18 @@ module ntlp_gpu
      integer(c_int32_t) :: ilin, icouple, iTcouple, iHcouple, ievap, iexner, icase, reserved
   end type ntlp_flags_t
 
+  ! A droplet ensemble drawn on the device (ntlp_gpu_particles_generate).
+  type, bind(c), public :: ntlp_ensemble_t
+     integer(c_int64_t) :: seed
+     integer(c_int32_t) :: radius_kind, radius2_kind
+     real(c_double) :: radius_p(2), radius2_p(2), frac2
+     integer(c_int32_t) :: salt_kind, reserved
+     real(c_double) :: salt_p(3)
+     real(c_double) :: kappa_s
+     integer(c_int64_t) :: mult
+     integer(c_int64_t) :: first
+  end type ntlp_ensemble_t
+
   type, bind(c), public :: ntlp_counters_t
      integer(c_int64_t) :: num100, numimpos, actnum, denum
      integer(c_int64_t) :: gn_iters, lm_iters, solves, steps
@@ -65,8 +80,11 @@ module ntlp_gpu
        ntlp_gpu_particle_coupling_exchange, ntlp_gpu_apply_particle_coupling, &
        ntlp_gpu_bin, ntlp_gpu_wcell_counts, ntlp_gpu_num_ucells, ntlp_gpu_step, ntlp_gpu_counters, &
        ntlp_gpu_calc_tauc_min, ntlp_gpu_particle_xy_stats, &
-       ntlp_gpu_step_counters, ntlp_gpu_sync, ntlp_gpu_be_flux
-  public :: gpu_be_flux
+       ntlp_gpu_step_counters, ntlp_gpu_sync, ntlp_gpu_be_flux, ntlp_gpu_particles_get_column, &
+       ntlp_gpu_particles_generate, ntlp_gpu_save_particles, ntlp_gpu_read_part_res, &
+       ntlp_gpu_particle_xy_stats_noalloc, ntlp_gpu_set_option, ntlp_gpu_timing, ntlp_gpu_kernel_times, &
+       ntlp_gpu_copy_bandwidth
+  public :: gpu_be_flux, gpu_save_particles, gpu_read_part_res, gpu_get_column, gpu_generate, gpu_set_option
   public :: gpu_check, gpu_setup, gpu_set_fields, gpu_put, gpu_get, gpu_interp, gpu_crit_radius, &
        gpu_particle_update_rk3, gpu_particle_update_BE, gpu_particle_coupling_update, gpu_step, gpu_counters, &
        gpu_particle_coupling_exchange, gpu_apply_particle_coupling, gpu_calc_tauc_min, gpu_particle_xy_stats
@@ -122,6 +140,29 @@ module ntlp_gpu
      integer(c_int64_t) function ntlp_gpu_particles_count() bind(c, name="ntlp_gpu_particles_count")
        import :: c_int64_t
      end function ntlp_gpu_particles_count
+     integer(c_int32_t) function ntlp_gpu_particles_get_column(column, out, n) &
+          bind(c, name="ntlp_gpu_particles_get_column")
+       import :: c_int32_t, c_int64_t, c_double
+       integer(c_int32_t), value :: column
+       real(c_double), intent(out) :: out(*)
+       integer(c_int64_t), value :: n
+     end function ntlp_gpu_particles_get_column
+     integer(c_int32_t) function ntlp_gpu_particles_generate(ens, counts, nens) &
+          bind(c, name="ntlp_gpu_particles_generate")
+       import :: c_int32_t, c_int64_t, ntlp_ensemble_t
+       type(ntlp_ensemble_t), intent(in) :: ens(*)
+       integer(c_int64_t), intent(in) :: counts(*)
+       integer(c_int32_t), value :: nens
+     end function ntlp_gpu_particles_generate
+     integer(c_int32_t) function ntlp_gpu_save_particles(path) bind(c, name="ntlp_gpu_save_particles")
+       import :: c_int32_t, c_char
+       character(kind=c_char), intent(in) :: path(*)
+     end function ntlp_gpu_save_particles
+     integer(c_int32_t) function ntlp_gpu_read_part_res(path, kept) bind(c, name="ntlp_gpu_read_part_res")
+       import :: c_int32_t, c_int64_t, c_char
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int64_t), intent(out) :: kept
+     end function ntlp_gpu_read_part_res
      integer(c_int32_t) function ntlp_gpu_interp(set_vp, set_tp) bind(c, name="ntlp_gpu_interp")
        import :: c_int32_t
        integer(c_int32_t), value :: set_vp, set_tp
@@ -183,6 +224,11 @@ module ntlp_gpu
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: s0, s1, lwp
      end function ntlp_gpu_particle_xy_stats
+     integer(c_int32_t) function ntlp_gpu_particle_xy_stats_noalloc(s0, s1, lwp) &
+          bind(c, name="ntlp_gpu_particle_xy_stats_noalloc")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: s0, s1, lwp
+     end function ntlp_gpu_particle_xy_stats_noalloc
      integer(c_int64_t) function ntlp_gpu_num_ucells() bind(c, name="ntlp_gpu_num_ucells")
        import :: c_int64_t
      end function ntlp_gpu_num_ucells
@@ -204,6 +250,29 @@ module ntlp_gpu
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: pflux, pmassflux, penegflux
      end function ntlp_gpu_be_flux
+     integer(c_int32_t) function ntlp_gpu_timing(enable) bind(c, name="ntlp_gpu_timing")
+       import :: c_int32_t
+       integer(c_int32_t), value :: enable
+     end function ntlp_gpu_timing
+     integer(c_int32_t) function ntlp_gpu_kernel_times(names, launches, total_ms, cap) &
+          bind(c, name="ntlp_gpu_kernel_times")
+       import :: c_int32_t, c_int64_t, c_double, c_char
+       character(kind=c_char), intent(out) :: names(*)
+       integer(c_int64_t), intent(out) :: launches(*)
+       real(c_double), intent(out) :: total_ms(*)
+       integer(c_int32_t), value :: cap
+     end function ntlp_gpu_kernel_times
+     integer(c_int32_t) function ntlp_gpu_copy_bandwidth(bytes, reps, gbps) bind(c, name="ntlp_gpu_copy_bandwidth")
+       import :: c_int32_t, c_int64_t, c_double
+       integer(c_int64_t), value :: bytes
+       integer(c_int32_t), value :: reps
+       real(c_double), intent(out) :: gbps
+     end function ntlp_gpu_copy_bandwidth
+     integer(c_int32_t) function ntlp_gpu_set_option(option, val) bind(c, name="ntlp_gpu_set_option")
+       import :: c_int32_t, c_int64_t
+       integer(c_int32_t), value :: option
+       integer(c_int64_t), value :: val
+     end function ntlp_gpu_set_option
      integer(c_int32_t) function ntlp_gpu_sync() bind(c, name="ntlp_gpu_sync")
        import :: c_int32_t
      end function ntlp_gpu_sync
@@ -356,6 +425,46 @@ contains
     real(c_double), intent(out), target :: pflux(0:), pmassflux(0:), penegflux(0:)
     call gpu_check(ntlp_gpu_be_flux(c_loc(pflux), c_loc(pmassflux), c_loc(penegflux)), 'ntlp_gpu_be_flux')
   end subroutine gpu_be_flux
+
+  ! save_particles (particles.f90:1988-2046) for one rank: an int32 droplet
+  ! count followed by one 352-byte record per droplet.
+  subroutine gpu_save_particles(path)
+    character(*), intent(in) :: path
+    call gpu_check(ntlp_gpu_save_particles(trim(path) // c_null_char), 'save_particles')
+  end subroutine gpu_save_particles
+
+  ! read_part_res (particles.f90:2048-2146) for one rank: keeps the records with
+  ! 0 < x < xl and 0 < y < yl and replaces the droplets on the device.
+  subroutine gpu_read_part_res(path, kept)
+    character(*), intent(in) :: path
+    integer(c_int64_t), intent(out), optional :: kept
+    integer(c_int64_t) :: k
+    call gpu_check(ntlp_gpu_read_part_res(trim(path) // c_null_char, k), 'read_part_res')
+    if (present(kept)) kept = k
+  end subroutine gpu_read_part_res
+
+  ! One column (C_X ... C_RADIUSOLD, 1-based) of every droplet, in put order.
+  subroutine gpu_get_column(column, out)
+    integer, intent(in) :: column
+    real(c_double), intent(out) :: out(:)
+    call gpu_check(ntlp_gpu_particles_get_column(int(column - 1, c_int32_t), out, int(size(out), c_int64_t)), &
+         'ntlp_gpu_particles_get_column')
+  end subroutine gpu_get_column
+
+  ! particle_init / create_particle's drawing (particles.f90:2226-2300,
+  ! 2440-2520) done on the device: counts(e) droplets of ens(e), in order.
+  subroutine gpu_generate(ens, counts)
+    type(ntlp_ensemble_t), intent(in) :: ens(:)
+    integer(c_int64_t), intent(in) :: counts(:)
+    if (size(counts) /= size(ens)) call gpu_check(-1_c_int32_t, 'ntlp_gpu_particles_generate (sizes)')
+    call gpu_check(ntlp_gpu_particles_generate(ens, counts, int(size(ens), c_int32_t)), 'ntlp_gpu_particles_generate')
+  end subroutine gpu_generate
+
+  subroutine gpu_set_option(option, val)
+    integer, intent(in) :: option
+    integer, intent(in) :: val
+    call gpu_check(ntlp_gpu_set_option(int(option, c_int32_t), int(val, c_int64_t)), 'ntlp_gpu_set_option')
+  end subroutine gpu_set_option
 
   subroutine gpu_counters(c, reset)
     type(ntlp_counters_t), intent(out) :: c

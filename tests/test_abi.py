@@ -33,6 +33,18 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_fortran_module_binds_every_entry_point_a_time_loop_calls():
+    """Every C entry point has a bind(c) interface in the Fortran module,
+    except the serving loop and its pinned host buffers, which serve callers
+    other than the Fortran time loop."""
+    text = open(os.path.join(ROOT, "fortran", "ntlp_gpu_mod.f90")).read()
+    bound = set(re.findall(r'bind\(c,\s*name="(ntlp_\w+)"\)', text))
+    unbound = [n for n in declared_functions()
+               if n not in bound and not n.startswith("ntlp_serve_") and n not in ("ntlp_host_alloc", "ntlp_host_free")]
+    assert not unbound, unbound
+    assert not bound - set(declared_functions())
+
+
 def test_python_binding_loads():
     lib = load_library()
     assert lib.ntlp_gpu_last_error() is not None
