@@ -23,6 +23,12 @@
 !   real8  soa(n, 34); int64 mult(n); int32 pidx(n), procidx(n)
 !   real8  partsrc_t(0:nnz+1,1:nny+1,1:nnx+1,1:3), partTsrc_t, partHsrc_t, partTEsrc_t
 !   int64  num100, numimpos, actnum, denum, gn_iters, lm_iters, solves, steps
+!   int64  droplets kept by read_part_res of the file save_particles wrote
+!   real8  the largest |radius difference| between the droplets read back and
+!          those saved (one column, fetched with gpu_get_column)
+!
+! The restart round trip (save_particles + read_part_res, particles.f90:
+! 1988-2146) writes <output file>.restart.
 program ntlp_dropin_driver
   use iso_c_binding
   use ntlp_gpu
@@ -41,6 +47,9 @@ program ntlp_dropin_driver
   type(ntlp_flags_t) :: flags
   type(ntlp_counters_t) :: cnt
   character(len=4096) :: fin, fout
+  integer(c_int64_t) :: kept
+  real(c_double), allocatable :: col(:)
+  real(c_double) :: maxdiff
 
   if (command_argument_count() /= 2) then
      write(*, '(a)') 'usage: ntlp_dropin_driver <input> <output>'
@@ -80,12 +89,21 @@ program ntlp_dropin_driver
 
   call gpu_get(soa, mult, pidx, procidx)
   call gpu_counters(cnt, .false.)
+
+  ! restart round trip: the records saved, read back, and one column compared
+  call gpu_save_particles(trim(fout) // '.restart')
+  call gpu_read_part_res(trim(fout) // '.restart', kept)
+  allocate(col(kept))
+  call gpu_get_column(C_RADIUS, col)
+  maxdiff = -1.0_c_double
+  if (kept == n) maxdiff = maxval(abs(col - soa(:, C_RADIUS)))
   call gpu_check(ntlp_gpu_finalize(), 'ntlp_gpu_finalize')
 
   open(newunit=uout, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
   write(uout) soa, mult, pidx, procidx
   write(uout) partsrc, partTsrc, partHsrc, partTEsrc
   write(uout) cnt%num100, cnt%numimpos, cnt%actnum, cnt%denum, cnt%gn_iters, cnt%lm_iters, cnt%solves, cnt%steps
+  write(uout) kept, maxdiff
   close(uout)
   write(*, '(a,i0,a,i0,a,i0,a)') 'ntlp_dropin_driver: ', n, ' droplets, ', nsteps, ' steps, ', cnt%solves, ' solves'
 end program ntlp_dropin_driver

@@ -60,8 +60,10 @@ def read_output(path, n, grid):
     # Fortran (0:nnz+1, 1:nny+1, 1:nnx+1) column-major == C (nnx+1, nny+1, nnz+2)
     src = take(6 * m, np.float64).reshape(6, grid.nnx + 1, grid.nny + 1, grid.nnz + 2).copy()
     cnt = take(8, np.int64)
+    kept = int(take(1, np.int64)[0])
+    maxdiff = float(take(1, np.float64)[0])
     assert off == len(raw)
-    return Particles(soa, mult, pidx, procidx), src, cnt
+    return Particles(soa, mult, pidx, procidx), src, cnt, (kept, maxdiff)
 
 
 @pytest.mark.parametrize("kind", ["uniform", "stretched"])
@@ -74,7 +76,12 @@ def test_fortran_driver_matches_binding_and_oracle(kind, tmp_path):
     write_input(fin, grid, consts, flags, fields, p, nsteps, it0, PREC_FP64)
     proc = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
     assert proc.returncode == 0, proc.stdout + proc.stderr
-    got, src, cnt = read_output(fout, p.n, grid)
+    got, src, cnt, (kept, maxdiff) = read_output(fout, p.n, grid)
+    # the restart round trip through the module's save_particles / read_part_res
+    # wrappers keeps every droplet (all inside 0 < x < xl, 0 < y < yl) and
+    # gives back the radii bit for bit
+    assert kept == p.n and maxdiff == 0.0, (kept, maxdiff)
+    assert os.path.getsize(fout + ".restart") == 4 + 352 * p.n
 
     # the same calls through the ctypes binding
     s = make_step(grid, consts, flags, fields, p)
