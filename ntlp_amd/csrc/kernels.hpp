@@ -402,17 +402,15 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   const int nlev = a.g.nnz + 2;
   double* s_z = s_dyn;
   double* s_zz = s_dyn + nlev;
-  stage_grid(a.g, s_z, s_zz);
-  zero_tally_lds(s_red);
-  __syncthreads();
   const int64_t t = xcd_block() * kBlock + threadIdx.x;
   const bool valid = t < (SERVE ? a.serve->n_live : a.n);
   Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
   int key = -1;
   const bool inplace = !SERVE && a.inplace;
   static_assert(!LAZY || (MODE == 2 && !SERVE), "lazy residence times: library BE steps only");
-  // the droplet's loads are issued before the field box is staged, so that
-  // they are in flight while the box loads
+  // the droplet's loads are issued before the levels and the field box are
+  // staged (a barrier orders every memory access around it), so that they are
+  // in flight while those load
   int64_t s = t;
   int kin = -1;
   Drop d;
@@ -425,6 +423,9 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     mult = a.src.mult[s];
     home = inplace ? 0 : a.src.home[s];
   }
+  stage_grid(a.g, s_z, s_zz);
+  zero_tally_lds(s_red);
+  __syncthreads();
   // LDS-staged gather: the trilinear field nodes of the workgroup's cells
   FieldBox bx;
   bx.ni = 0;
