@@ -504,14 +504,19 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
   const DevConsts& C = *a.Cp;
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
   __shared__ int s_red[kTallyWidth + 1];
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
+  // the droplet's loads go out before the barrier that follows the staging
+  Drop d;
+  int32_t home = 0;
+  if (t < a.n) {
+    d = load_drop(a.src.hot, t);
+    home = a.src.home[t];
+  }
   stage_grid(a.g, s_z, s_zz);
   zero_tally_lds(s_red);
   __syncthreads();
-  const int64_t t = xcd_block() * kBlock + threadIdx.x;
-  Tally tl = {0, 0, 0, 0, 0, 0, 0, -1};
   if (t < a.n) {
-    Drop d = load_drop(a.src.hot, t);
-    const int32_t home = a.src.home[t];
     Diag o;
     if (MODE == 1) {
       load_rhs(a.cold, a.cap, home, o);
@@ -532,11 +537,16 @@ __global__ void __launch_bounds__(kBlock) k_update(StepArgs a) {
 template <int ILIN>
 __global__ void __launch_bounds__(kBlock) k_interp(StepArgs a, int set_vp, int set_tp) {
   __shared__ double s_z[kMaxLevels], s_zz[kMaxLevels];
+  const int64_t t = xcd_block() * kBlock + threadIdx.x;
+  double x = 0.0, y = 0.0, z = 0.0;  // loaded before the barrier that follows the staging
+  if (t < a.n) {
+    x = a.src.hot[hix(kHX, t)];
+    y = a.src.hot[hix(kHY, t)];
+    z = a.src.hot[hix(kHZ, t)];
+  }
   stage_grid(a.g, s_z, s_zz);
   __syncthreads();
-  const int64_t t = xcd_block() * kBlock + threadIdx.x;
   if (t >= a.n) return;
-  const double x = a.src.hot[hix(kHX, t)], y = a.src.hot[hix(kHY, t)], z = a.src.hot[hix(kHZ, t)];
   const Cell c = cell_of(a.g, s_z, s_zz, x, y, z);
   Fluid f = (ILIN == 1) ? gather_lin_f64<true>(a.g, a.F, s_z, s_zz, x, y, z, c)
                         : gather6_f64(a.g, a.F, s_z, s_zz, x, y, z, c);
